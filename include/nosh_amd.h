@@ -1,0 +1,259 @@
+/* nosh_amd.h -- C ABI of the MI355X (gfx950) NLS Newton-Krylov library.
+ *
+ * Replaces the operator layer of nosh (the mesh edge cache, the
+ * explicit_values magnetic field, the parameter matrices keo / DkeoDP, the
+ * jacobian_operator, keo_regularized and the model_evaluator::nls) with
+ * HIP-resident objects behind opaque handles, plus the in-repo caller
+ * (Newton-MINRES) that replaces NOX/Belos.
+ *
+ * Conventions
+ *  - Every function returns NOSH_OK (0) or an error code; nosh_last_error()
+ *    gives the message of the last failure on the calling thread. No C++
+ *    exception crosses this boundary.
+ *  - Vectors are real R^{2n}, interleaved complex: x[2k] = Re psi_k,
+ *    x[2k+1] = Im psi_k, in the vertex order of the caller's mesh ("reference
+ *    order"; src/mesh.cpp:595-606). The bandwidth-reducing vertex permutation
+ *    (reverse Cuthill-McKee) is internal.
+ *  - Host-pointer variants copy through PCIe and synchronise. *_device
+ *    variants take device pointers in reference order and a hipStream_t
+ *    (NULL = the library stream) and are asynchronous.
+ *  - Parameters are passed as (count, names, values), the C form of the
+ *    reference's std::map<std::string,double>; a missing required name returns
+ *    NOSH_ERR_MISSING_PARAM (the reference throws std::out_of_range from
+ *    params.at(), e.g. src/jacobian_operator.cpp:156).
+ *  - Operator applies support only mode = NOSH_NO_TRANS, alpha = 1, beta = 0,
+ *    as the reference (src/jacobian_operator.cpp:48-59,
+ *    src/keo_regularized.cpp:98-100); anything else returns
+ *    NOSH_ERR_UNSUPPORTED. Y is overwritten. Multi-column X is applied column
+ *    by column (src/jacobian_operator.cpp:80).
+ *  - Threading: apply calls on one handle may run concurrently on distinct
+ *    streams once its rebuild/set_parameters has completed; rebuild,
+ *    set_parameters and destroy must be externally synchronised.
+ */
+#ifndef NOSH_AMD_H
+#define NOSH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NOSH_OK 0
+#define NOSH_ERR_INVALID 1
+#define NOSH_ERR_UNSUPPORTED 2
+#define NOSH_ERR_MISSING_PARAM 3
+#define NOSH_ERR_HIP 4
+#define NOSH_ERR_NOT_BUILT 5
+#define NOSH_ERR_NOT_CONVERGED 6
+
+#define NOSH_NO_TRANS 0
+#define NOSH_TRANS 1
+#define NOSH_CONJ_TRANS 2
+
+typedef struct nosh_mesh_s* nosh_mesh_t;
+typedef struct nosh_vfield_s* nosh_vfield_t;
+typedef struct nosh_sfield_s* nosh_sfield_t;
+typedef struct nosh_keo_s* nosh_keo_t;
+typedef struct nosh_jac_s* nosh_jac_t;
+typedef struct nosh_prec_s* nosh_prec_t;
+typedef struct nosh_nls_s* nosh_nls_t;
+
+/* Message of the last error on this thread ("" if none). */
+const char* nosh_last_error(void);
+
+/* ---------------------------------------------------------------- mesh ---
+ * Builds the edge/covolume cache on the device: edges numbered by first
+ * appearance over the cells in order, local edges in MOAB canonical order,
+ * endpoints (lower id, higher id) (nosh::read + mesh::build_entity_relations_,
+ * src/mesh_reader.cpp:117-125, src/mesh.cpp:629-691); edge lengths and
+ * covolumes (mesh_tri/mesh_tetra::compute_edge_data_,
+ * src/mesh_tri.cpp:44-151, src/mesh_tetra.cpp:41-160); control volumes
+ * (src/mesh_tri.cpp:153-210, src/mesh_tetra.cpp:162-271); and the CSR edge
+ * graph, RCM-reordered when reorder != 0 (src/mesh.cpp:787-881).
+ * coords: num_vertices x 3; cells: num_cells x vertices_per_cell (3 or 4). */
+int nosh_mesh_create(const double* coords, int64_t num_vertices, const int32_t* cells,
+                     int64_t num_cells, int vertices_per_cell, int reorder, nosh_mesh_t* out);
+int nosh_mesh_destroy(nosh_mesh_t mesh);
+int nosh_mesh_sizes(nosh_mesh_t mesh, int64_t* num_vertices, int64_t* num_edges,
+                    int64_t* num_cells);
+/* mesh::my_edges() (src/mesh.hpp:103-107): num_edges x 2 (lower, higher). */
+int nosh_mesh_edges(nosh_mesh_t mesh, int32_t* edges);
+/* relations_.cell_edges (src/mesh.hpp:259 area): num_cells x (3|6), ascending. */
+int nosh_mesh_cell_edges(nosh_mesh_t mesh, int32_t* cell_edges);
+/* mesh::get_edge_data() (src/mesh.hpp:41-44,192-194). */
+int nosh_mesh_edge_data(nosh_mesh_t mesh, double* length, double* covolume);
+/* mesh::control_volumes() (src/mesh.hpp:188-190). */
+int nosh_mesh_control_volumes(nosh_mesh_t mesh, double* control_volumes);
+/* Internal vertex order: perm[i] = reference vertex stored at internal slot i. */
+int nosh_mesh_permutation(nosh_mesh_t mesh, int32_t* perm);
+/* Device vectors between reference order and the internal order (stream
+ * may be NULL). Callers that keep state on the device in internal order use
+ * the *_apply_internal entry points and skip the permutation per apply. */
+int nosh_mesh_to_internal(nosh_mesh_t mesh, const double* x_ref, double* x_int, void* stream);
+int nosh_mesh_to_reference(nosh_mesh_t mesh, const double* x_int, double* x_ref, void* stream);
+
+/* -------------------------------------------------------------- fields ---
+ * vector_field::explicit_values(mesh, "A", mu)
+ * (src/vector_field_explicit_values.cpp:13-50); A: num_vertices x 3. The
+ * only scalar parameter is "mu" (:56-69). */
+int nosh_vfield_explicit_values_create(nosh_mesh_t mesh, const double* A, double mu,
+                                       nosh_vfield_t* out);
+/* get_edge_projection(k) = mu * 0.5 (A_lo + A_hi).(x_lo - x_hi) for all edges
+ * at the field's current mu (src/vector_field_explicit_values.cpp:71-76). */
+int nosh_vfield_edge_projections(nosh_vfield_t field, double* out);
+int nosh_vfield_destroy(nosh_vfield_t field);
+/* scalar_field::constant(mesh, c, param_name, param_init)
+ * (src/scalar_field_constant.cpp:14-75): value c (+ params[param_name] if that
+ * name is given and present). param_name may be NULL or "". */
+int nosh_sfield_constant_create(nosh_mesh_t mesh, double c, const char* param_name,
+                                double param_init, nosh_sfield_t* out);
+/* scalar_field::explicit_values (src/scalar_field_explicit_values.cpp:33-57). */
+int nosh_sfield_explicit_values_create(nosh_mesh_t mesh, const double* values,
+                                       nosh_sfield_t* out);
+int nosh_sfield_destroy(nosh_sfield_t field);
+
+/* ------------------------------------------------------------ keo / dK ---
+ * parameter_matrix::keo(mesh, thickness, mvp)
+ * (src/parameter_matrix_keo.cpp:25-42). */
+int nosh_keo_create(nosh_mesh_t mesh, nosh_sfield_t thickness, nosh_vfield_t mvp,
+                    nosh_keo_t* out);
+/* parameter_matrix::DkeoDP(mesh, thickness, mvp, param_name)
+ * (src/parameter_matrix_dkeo_dp.cpp:60-155): dK/d(param_name). */
+int nosh_dkeo_dp_create(nosh_mesh_t mesh, nosh_sfield_t thickness, nosh_vfield_t mvp,
+                        const char* param_name, nosh_keo_t* out);
+/* parameter_object::set_parameters (src/parameter_object.cpp:6-47): needs
+ * "mu"; re-assembles (keo::refill_, src/parameter_matrix_keo.cpp:74-184) only
+ * when mu changed. */
+int nosh_keo_set_parameters(nosh_keo_t keo, int nparams, const char* const* names,
+                            const double* values);
+/* Tpetra::CrsMatrix::apply on the KEO (called at
+ * src/model_evaluator_nls.cpp:537, src/jacobian_operator.cpp:65). x, y:
+ * ncols columns of 2n values with leading dimension ld (>= 2n). */
+int nosh_keo_apply(nosh_keo_t keo, const double* x, double* y, int ncols, int64_t ld, int mode,
+                   double alpha, double beta);
+int nosh_keo_apply_device(nosh_keo_t keo, const double* x, double* y, void* stream);
+/* Device pointers in internal order. */
+int nosh_keo_apply_internal(nosh_keo_t keo, const double* x, double* y, void* stream);
+int nosh_keo_destroy(nosh_keo_t keo);
+
+/* --------------------------------------------------- jacobian_operator ---
+ * jacobian_operator(mesh, scalar_potential, thickness, keo)
+ * (src/jacobian_operator.hpp:29-34). The keo handle is shared, as in the
+ * reference (src/model_evaluator_nls.cpp:257-266). */
+int nosh_jac_create(nosh_mesh_t mesh, nosh_sfield_t scalar_potential, nosh_sfield_t thickness,
+                    nosh_keo_t keo, nosh_jac_t* out);
+/* jacobian_operator::rebuild(params, psi) (src/jacobian_operator.cpp:110-200):
+ * needs "g" and "mu". psi: 2n values (host). */
+int nosh_jac_rebuild(nosh_jac_t jac, int nparams, const char* const* names,
+                     const double* values, const double* psi);
+/* jacobian_operator::apply (src/jacobian_operator.cpp:38-108). */
+int nosh_jac_apply(nosh_jac_t jac, const double* x, double* y, int ncols, int64_t ld, int mode,
+                   double alpha, double beta);
+int nosh_jac_apply_device(nosh_jac_t jac, const double* x, double* y, void* stream);
+/* Device pointers in internal order; the _fp32 form applies the FP32 copy of
+ * the operator (values and blocks rounded to float) to FP32 vectors. */
+int nosh_jac_apply_internal(nosh_jac_t jac, const double* x, double* y, void* stream);
+int nosh_jac_apply_internal_fp32(nosh_jac_t jac, const float* x, float* y, void* stream);
+int nosh_jac_destroy(nosh_jac_t jac);
+
+/* ----------------------------------------------------- keo_regularized ---
+ * keo_regularized(mesh, thickness, mvp) (src/keo_regularized.cpp:39-66).
+ * Its inverse, which the reference approximates with one MueLu AMG V-cycle
+ * (:106-108, :290-336), is a Chebyshev iteration with 2x2 block-Jacobi
+ * scaling here: `iters` iterations (iters-1 products with P) on the
+ * interval [upper/ratio, upper] of D^{-1}P; upper <= 0 selects the bound
+ * 1 + max_k sum_{j!=k}|K_kj| / lambda_min(D_k) (<= 2 for nonnegative edge
+ * weights). Defaults: iters 4, ratio 30, upper 0. */
+int nosh_prec_create(nosh_mesh_t mesh, nosh_sfield_t thickness, nosh_vfield_t mvp,
+                     nosh_prec_t* out);
+int nosh_prec_set_chebyshev(nosh_prec_t prec, int iters, double ratio, double upper);
+/* keo_regularized::rebuild(params, psi) (src/keo_regularized.cpp:181-264):
+ * P = K + 2x2 blocks from g*cv*t*psi (added only when g > 0). */
+int nosh_prec_rebuild(nosh_prec_t prec, int nparams, const char* const* names,
+                      const double* values, const double* psi);
+/* keo_regularized::apply: y = M^{-1} x (the approximate inverse). */
+int nosh_prec_apply(nosh_prec_t prec, const double* x, double* y, int ncols, int64_t ld,
+                    int mode, double alpha, double beta);
+int nosh_prec_apply_device(nosh_prec_t prec, const double* x, double* y, void* stream);
+/* y = P x, the assembled regularized KEO itself. */
+int nosh_prec_apply_operator(nosh_prec_t prec, const double* x, double* y);
+/* Spectral upper bound in use after the last rebuild. */
+int nosh_prec_upper_bound(nosh_prec_t prec, double* upper);
+int nosh_prec_destroy(nosh_prec_t prec);
+
+/* ------------------------------------------------- model_evaluator::nls ---
+ * nls(mesh, mvp, scalar_potential, g, thickness, initial_x, deriv_parameter)
+ * (src/model_evaluator_nls.hpp:62-70, src/model_evaluator_nls.cpp:56-138).
+ * initial_x may be NULL (psi = 1). Parameter names are the sorted union
+ * {"g"} + potential params + {"mu"} (src/model_evaluator_nls.cpp:102-111). */
+int nosh_nls_create(nosh_mesh_t mesh, nosh_vfield_t mvp, nosh_sfield_t scalar_potential,
+                    double g, nosh_sfield_t thickness, const double* initial_x,
+                    const char* deriv_parameter, nosh_nls_t* out);
+/* get_p_names(0) / nominal p (src/model_evaluator_nls.cpp:196-205). */
+int nosh_nls_num_params(nosh_nls_t nls, int* count);
+int nosh_nls_param_name(nosh_nls_t nls, int index, char* buf, int buflen);
+int nosh_nls_param_nominal(nosh_nls_t nls, int index, double* value);
+/* evalModel with out_args.f (compute_f_, src/model_evaluator_nls.cpp:527-628). */
+int nosh_nls_eval_f(nosh_nls_t nls, const double* x, int nparams, const char* const* names,
+                    const double* values, double* f);
+int nosh_nls_eval_f_device(nosh_nls_t nls, const double* x, int nparams,
+                           const char* const* names, const double* values, double* f,
+                           void* stream);
+/* Column `col` of DfDp, in get_p_names order (computeDFDP_,
+ * src/model_evaluator_nls.cpp:630-695, including its use of the dK/dp of the
+ * model's deriv_parameter for every column). */
+int nosh_nls_eval_dfdp(nosh_nls_t nls, const double* x, int nparams, const char* const* names,
+                       const double* values, int col, double* dfdp);
+/* create_W_op (src/model_evaluator_nls.cpp:253-267) and create_W_prec
+ * (:301-314); evalModel with W_op / W_prec rebuilds them (:491-522). */
+int nosh_nls_create_w_op(nosh_nls_t nls, nosh_jac_t* out);
+int nosh_nls_create_w_prec(nosh_nls_t nls, nosh_prec_t* out);
+int nosh_nls_destroy(nosh_nls_t nls);
+
+/* --------------------------------------------------------- solvers -------
+ * The in-repo replacement of Belos MINRES (selected at examples/conf.xml:105,
+ * :117-123) and the NOX full-step Newton (examples/conf.xml:77-190). */
+typedef struct {
+  double tol;           /* stop when ||F(psi)||_2 <= tol (default 1e-10) */
+  int max_iters;        /* Newton steps (default 30) */
+  double minres_rtol;   /* MINRES relative tolerance in the M^{-1} norm (1e-10) */
+  int minres_max_iters; /* (default 1000) */
+  int cheb_iters;       /* Chebyshev iterations of the preconditioner, 0 = none (4) */
+  double cheb_ratio;    /* interval ratio (30) */
+  int precision;        /* 0 = FP64 throughout; 1 = FP32 inner MINRES / FP64 outer */
+  int forcing;          /* 0 = constant minres_rtol; 1 = Eisenstat-Walker choice 2 */
+  double forcing_min;   /* lower bound of the forcing term (mixed: 1e-6) */
+  double forcing_max;   /* upper bound of the forcing term (0.1) */
+} nosh_newton_options;
+
+typedef struct {
+  int newton_iters;
+  int minres_iters_total;
+  int converged;
+  double final_norm_f;
+  double seconds;
+  double norm_f_history[64];
+  int minres_history[64];
+} nosh_newton_stats;
+
+/* Fills *opts with the defaults listed above. */
+int nosh_newton_default_options(nosh_newton_options* opts);
+/* Newton-MINRES on F(psi) = 0 at the given parameters, preconditioned by
+ * keo_regularized. psi: 2n values (host), overwritten by the solution. */
+int nosh_newton_solve(nosh_nls_t nls, double* psi, int nparams, const char* const* names,
+                      const double* values, const nosh_newton_options* opts,
+                      nosh_newton_stats* stats);
+int nosh_newton_solve_device(nosh_nls_t nls, double* psi, int nparams,
+                             const char* const* names, const double* values,
+                             const nosh_newton_options* opts, nosh_newton_stats* stats,
+                             void* stream);
+/* MINRES on J x = b (x starts at 0) with preconditioner prec (NULL = none).
+ * Both operators must have been rebuilt. */
+int nosh_minres_solve(nosh_jac_t jac, nosh_prec_t prec, const double* b, double* x,
+                      double rtol, int max_iters, int* iters, double* relres);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NOSH_AMD_H */

@@ -1,0 +1,532 @@
+"""nosh_amd: MI355X-native NLS Newton-Krylov core behind nosh's operator API.
+
+Python mirror of the reference classes over the C ABI of libnosh_amd.so
+(include/nosh_amd.h). Names, argument meaning and error behaviour follow the
+reference:
+
+=====================================  =========================================
+reference (nosh tree)                  here
+=====================================  =========================================
+nosh::read / nosh::mesh                Mesh(coords, cells)
+vector_field::explicit_values          VectorFieldExplicitValues(mesh, A, mu)
+scalar_field::constant                 ScalarFieldConstant(mesh, c, name, init)
+scalar_field::explicit_values          ScalarFieldExplicitValues(mesh, values)
+parameter_matrix::keo                  Keo(mesh, thickness, mvp)
+parameter_matrix::DkeoDP               DkeoDP(mesh, thickness, mvp, name)
+jacobian_operator                      JacobianOperator(mesh, V, thickness, keo)
+keo_regularized                        KeoRegularized(mesh, thickness, mvp)
+model_evaluator::nls                   Nls(mesh, mvp, V, g, thickness, x0, name)
+=====================================  =========================================
+
+Operators take and return numpy vectors of length 2n (interleaved complex,
+reference vertex order) and implement ``apply(X, mode, alpha, beta)`` with the
+reference's restrictions: only NO_TRANS, alpha=1, beta=0
+(src/jacobian_operator.cpp:48-59); anything else raises ``LogicError``
+(std::logic_error). A parameter map lacking a required key raises ``KeyError``
+(std::out_of_range from params.at()).
+
+The library must be built (``python -m nosh_amd.build``); there is no CPU
+fallback: importing works without it, but every object construction raises
+``LibraryNotBuilt``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Iterable, Optional
+
+import numpy as np
+
+__all__ = [
+    "Mesh", "VectorFieldExplicitValues", "ScalarFieldConstant", "ScalarFieldExplicitValues",
+    "Keo", "DkeoDP", "JacobianOperator", "KeoRegularized", "Nls", "NewtonOptions",
+    "newton_solve", "minres_solve", "LogicError", "NoshError", "LibraryNotBuilt",
+    "NO_TRANS", "TRANS", "CONJ_TRANS", "library_path",
+]
+
+NO_TRANS, TRANS, CONJ_TRANS = 0, 1, 2
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_HERE, "libnosh_amd.so")
+
+
+class NoshError(RuntimeError):
+    """A failure reported by the library (HIP error, invalid argument ...)."""
+
+
+class LogicError(NoshError):
+    """Unsupported operator mode (the reference throws std::logic_error)."""
+
+
+class LibraryNotBuilt(NoshError):
+    """libnosh_amd.so is missing: build it with ``python -m nosh_amd.build``."""
+
+
+_OK, _INVALID, _UNSUPPORTED, _MISSING, _HIP, _NOTBUILT, _NOTCONV = range(7)
+
+
+class NewtonOptions(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iters", C.c_int), ("minres_rtol", C.c_double),
+                ("minres_max_iters", C.c_int), ("cheb_iters", C.c_int),
+                ("cheb_ratio", C.c_double), ("precision", C.c_int), ("forcing", C.c_int),
+                ("forcing_min", C.c_double), ("forcing_max", C.c_double)]
+
+
+class NewtonStats(C.Structure):
+    _fields_ = [("newton_iters", C.c_int), ("minres_iters_total", C.c_int),
+                ("converged", C.c_int), ("final_norm_f", C.c_double), ("seconds", C.c_double),
+                ("norm_f_history", C.c_double * 64), ("minres_history", C.c_int * 64)]
+
+    def as_dict(self) -> dict:
+        k = min(self.newton_iters, 63)
+        return {"newton_iters": self.newton_iters, "minres_iters_total": self.minres_iters_total,
+                "converged": bool(self.converged), "final_norm_f": self.final_norm_f,
+                "seconds": self.seconds,
+                "norm_f_history": list(self.norm_f_history[:k + 1]),
+                "minres_history": list(self.minres_history[:k])}
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(library_path):
+        raise LibraryNotBuilt(f"{library_path} not found; run `python -m nosh_amd.build`")
+    lib = C.CDLL(library_path)
+    vp, i32, i64, d = C.c_void_p, C.c_int, C.c_int64, C.c_double
+    pvp = C.POINTER(C.c_void_p)
+    cpp = C.POINTER(C.c_char_p)
+    sig = {
+        "nosh_last_error": (C.c_char_p, []),
+        "nosh_mesh_create": (i32, [vp, i64, vp, i64, i32, i32, pvp]),
+        "nosh_mesh_destroy": (i32, [vp]),
+        "nosh_mesh_sizes": (i32, [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]),
+        "nosh_mesh_edges": (i32, [vp, vp]),
+        "nosh_mesh_cell_edges": (i32, [vp, vp]),
+        "nosh_mesh_edge_data": (i32, [vp, vp, vp]),
+        "nosh_mesh_control_volumes": (i32, [vp, vp]),
+        "nosh_mesh_permutation": (i32, [vp, vp]),
+        "nosh_mesh_to_internal": (i32, [vp, vp, vp, vp]),
+        "nosh_mesh_to_reference": (i32, [vp, vp, vp, vp]),
+        "nosh_vfield_explicit_values_create": (i32, [vp, vp, d, pvp]),
+        "nosh_vfield_edge_projections": (i32, [vp, vp]),
+        "nosh_vfield_destroy": (i32, [vp]),
+        "nosh_sfield_constant_create": (i32, [vp, d, C.c_char_p, d, pvp]),
+        "nosh_sfield_explicit_values_create": (i32, [vp, vp, pvp]),
+        "nosh_sfield_destroy": (i32, [vp]),
+        "nosh_keo_create": (i32, [vp, vp, vp, pvp]),
+        "nosh_dkeo_dp_create": (i32, [vp, vp, vp, C.c_char_p, pvp]),
+        "nosh_keo_set_parameters": (i32, [vp, i32, cpp, vp]),
+        "nosh_keo_apply": (i32, [vp, vp, vp, i32, i64, i32, d, d]),
+        "nosh_keo_apply_device": (i32, [vp, vp, vp, vp]),
+        "nosh_keo_apply_internal": (i32, [vp, vp, vp, vp]),
+        "nosh_keo_destroy": (i32, [vp]),
+        "nosh_jac_create": (i32, [vp, vp, vp, vp, pvp]),
+        "nosh_jac_rebuild": (i32, [vp, i32, cpp, vp, vp]),
+        "nosh_jac_apply": (i32, [vp, vp, vp, i32, i64, i32, d, d]),
+        "nosh_jac_apply_device": (i32, [vp, vp, vp, vp]),
+        "nosh_jac_apply_internal": (i32, [vp, vp, vp, vp]),
+        "nosh_jac_apply_internal_fp32": (i32, [vp, vp, vp, vp]),
+        "nosh_jac_destroy": (i32, [vp]),
+        "nosh_prec_create": (i32, [vp, vp, vp, pvp]),
+        "nosh_prec_set_chebyshev": (i32, [vp, i32, d, d]),
+        "nosh_prec_rebuild": (i32, [vp, i32, cpp, vp, vp]),
+        "nosh_prec_apply": (i32, [vp, vp, vp, i32, i64, i32, d, d]),
+        "nosh_prec_apply_device": (i32, [vp, vp, vp, vp]),
+        "nosh_prec_apply_operator": (i32, [vp, vp, vp]),
+        "nosh_prec_upper_bound": (i32, [vp, C.POINTER(d)]),
+        "nosh_prec_destroy": (i32, [vp]),
+        "nosh_nls_create": (i32, [vp, vp, vp, d, vp, vp, C.c_char_p, pvp]),
+        "nosh_nls_num_params": (i32, [vp, C.POINTER(i32)]),
+        "nosh_nls_param_name": (i32, [vp, i32, C.c_char_p, i32]),
+        "nosh_nls_param_nominal": (i32, [vp, i32, C.POINTER(d)]),
+        "nosh_nls_eval_f": (i32, [vp, vp, i32, cpp, vp, vp]),
+        "nosh_nls_eval_f_device": (i32, [vp, vp, i32, cpp, vp, vp, vp]),
+        "nosh_nls_eval_dfdp": (i32, [vp, vp, i32, cpp, vp, i32, vp]),
+        "nosh_nls_create_w_op": (i32, [vp, pvp]),
+        "nosh_nls_create_w_prec": (i32, [vp, pvp]),
+        "nosh_nls_destroy": (i32, [vp]),
+        "nosh_newton_default_options": (i32, [C.POINTER(NewtonOptions)]),
+        "nosh_newton_solve": (i32, [vp, vp, i32, cpp, vp, C.POINTER(NewtonOptions),
+                                    C.POINTER(NewtonStats)]),
+        "nosh_newton_solve_device": (i32, [vp, vp, i32, cpp, vp, C.POINTER(NewtonOptions),
+                                           C.POINTER(NewtonStats), vp]),
+        "nosh_minres_solve": (i32, [vp, vp, vp, vp, d, i32, C.POINTER(i32), C.POINTER(d)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc == _OK:
+        return
+    msg = _lib.nosh_last_error().decode(errors="replace")
+    if rc == _UNSUPPORTED:
+        raise LogicError(msg)
+    if rc == _MISSING:
+        raise KeyError(msg)
+    if rc == _INVALID:
+        raise ValueError(msg)
+    if rc == _NOTBUILT:
+        raise NoshError("not built: " + msg)
+    raise NoshError(f"nosh error {rc}: {msg}")
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _vec(x, m: int) -> np.ndarray:
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    if a.ndim != 1 or a.size != m:
+        raise ValueError(f"expected a vector of length {m}, got shape {a.shape}")
+    return a
+
+
+class _Params:
+    """Marshals a {name: value} map into the (count, names, values) C form."""
+
+    def __init__(self, params: Dict[str, float]):
+        items = sorted(params.items())
+        self._names = [k.encode() for k, _ in items]
+        self.count = len(items)
+        self.names = (C.c_char_p * max(1, self.count))(*self._names)
+        self.values = np.array([v for _, v in items] or [0.0], dtype=np.float64)
+
+    @property
+    def args(self):
+        return (self.count, self.names, _ptr(self.values))
+
+
+class _Handle:
+    _destroy = ""
+
+    def __init__(self):
+        self._h = C.c_void_p()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value and _lib is not None:
+            getattr(_lib, self._destroy)(h)
+            self._h = C.c_void_p()
+
+
+class Mesh(_Handle):
+    """Edge/covolume cache built on the GPU (nosh::read + mesh_tri/mesh_tetra).
+
+    coords: (n, 3) float64; cells: (T, 3) triangles or (T, 4) tetrahedra.
+    """
+    _destroy = "nosh_mesh_destroy"
+
+    def __init__(self, coords, cells, reorder: bool = True):
+        super().__init__()
+        lib = _load()
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        if coords.ndim != 2 or coords.shape[1] != 3:
+            raise ValueError("coords must be (n, 3)")
+        if cells.ndim != 2 or cells.shape[1] not in (3, 4):
+            raise ValueError("cells must be (T, 3) or (T, 4)")
+        _check(lib.nosh_mesh_create(_ptr(coords), coords.shape[0], _ptr(cells), cells.shape[0],
+                                    cells.shape[1], 1 if reorder else 0, C.byref(self._h)))
+        n, E, T = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib.nosh_mesh_sizes(self._h, C.byref(n), C.byref(E), C.byref(T)))
+        self.num_vertices, self.num_edges, self.num_cells = n.value, E.value, T.value
+        self.vertices_per_cell = cells.shape[1]
+
+    def my_edges(self) -> np.ndarray:
+        out = np.empty((self.num_edges, 2), dtype=np.int32)
+        _check(_lib.nosh_mesh_edges(self._h, _ptr(out)))
+        return out
+
+    def cell_edges(self) -> np.ndarray:
+        ne = 3 if self.vertices_per_cell == 3 else 6
+        out = np.empty((self.num_cells, ne), dtype=np.int32)
+        _check(_lib.nosh_mesh_cell_edges(self._h, _ptr(out)))
+        return out
+
+    def get_edge_data(self):
+        """(length, covolume) per edge (mesh::get_edge_data)."""
+        ln = np.empty(self.num_edges)
+        cov = np.empty(self.num_edges)
+        _check(_lib.nosh_mesh_edge_data(self._h, _ptr(ln), _ptr(cov)))
+        return ln, cov
+
+    def control_volumes(self) -> np.ndarray:
+        out = np.empty(self.num_vertices)
+        _check(_lib.nosh_mesh_control_volumes(self._h, _ptr(out)))
+        return out
+
+    def permutation(self) -> np.ndarray:
+        out = np.empty(self.num_vertices, dtype=np.int32)
+        _check(_lib.nosh_mesh_permutation(self._h, _ptr(out)))
+        return out
+
+
+class VectorFieldExplicitValues(_Handle):
+    """Magnetic vector potential from per-vertex values A (n, 3), scaled by mu."""
+    _destroy = "nosh_vfield_destroy"
+
+    def __init__(self, mesh: Mesh, A, mu: float):
+        super().__init__()
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        if A.shape != (mesh.num_vertices, 3):
+            raise ValueError("A must be (n, 3)")
+        self.mesh = mesh
+        _check(_load().nosh_vfield_explicit_values_create(mesh._h, _ptr(A), mu,
+                                                          C.byref(self._h)))
+
+    def get_scalar_parameters(self):
+        return {"mu"}
+
+    def edge_projections(self) -> np.ndarray:
+        out = np.empty(self.mesh.num_edges)
+        _check(_lib.nosh_vfield_edge_projections(self._h, _ptr(out)))
+        return out
+
+
+class ScalarFieldConstant(_Handle):
+    """Constant scalar field c (+ params[param_name] when given)."""
+    _destroy = "nosh_sfield_destroy"
+
+    def __init__(self, mesh: Mesh, c: float, param_name: str = "", param_init: float = 0.0):
+        super().__init__()
+        self.mesh = mesh
+        self.param_name = param_name
+        _check(_load().nosh_sfield_constant_create(mesh._h, c, param_name.encode(), param_init,
+                                                   C.byref(self._h)))
+
+
+class ScalarFieldExplicitValues(_Handle):
+    _destroy = "nosh_sfield_destroy"
+
+    def __init__(self, mesh: Mesh, values):
+        super().__init__()
+        self.mesh = mesh
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.shape != (mesh.num_vertices,):
+            raise ValueError("values must have one entry per vertex")
+        _check(_load().nosh_sfield_explicit_values_create(mesh._h, _ptr(v), C.byref(self._h)))
+
+
+class _Operator(_Handle):
+    _apply = ""
+
+    def _m(self) -> int:
+        return 2 * self.mesh.num_vertices
+
+    def apply(self, X, mode: int = NO_TRANS, alpha: float = 1.0, beta: float = 0.0):
+        """Y = Op X for a vector (2n,) or column block (2n, k); Y is a new array."""
+        X = np.asarray(X, dtype=np.float64)
+        m = self._m()
+        if X.shape[0] != m:
+            raise ValueError(f"expected leading dimension {m}")
+        cols = 1 if X.ndim == 1 else X.shape[1]
+        Xf = np.ascontiguousarray(X.reshape(m, cols).T)  # columns contiguous
+        Yf = np.empty_like(Xf)
+        _check(getattr(_lib, self._apply)(self._h, _ptr(Xf), _ptr(Yf), cols, m, mode, alpha,
+                                          beta))
+        return Yf[0].copy() if X.ndim == 1 else np.ascontiguousarray(Yf.T)
+
+    def __call__(self, x):
+        return self.apply(x)
+
+
+class Keo(_Operator):
+    """parameter_matrix::keo: the magnetic kinetic energy operator K."""
+    _destroy = "nosh_keo_destroy"
+    _apply = "nosh_keo_apply"
+
+    def __init__(self, mesh: Mesh, thickness, mvp: VectorFieldExplicitValues, *, _h=None):
+        super().__init__()
+        self.mesh, self._keep = mesh, (thickness, mvp)
+        if _h is None:
+            _check(_load().nosh_keo_create(mesh._h, thickness._h, mvp._h, C.byref(self._h)))
+        else:
+            self._h = _h
+
+    def set_parameters(self, params: Dict[str, float], vector_params=None):
+        p = _Params(params)
+        _check(_lib.nosh_keo_set_parameters(self._h, *p.args))
+
+    def apply_device(self, x_ptr: int, y_ptr: int, stream: int = 0):
+        _check(_lib.nosh_keo_apply_device(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr),
+                                          C.c_void_p(stream)))
+
+
+class DkeoDP(Keo):
+    """parameter_matrix::DkeoDP: dK/d(param_name)."""
+
+    def __init__(self, mesh: Mesh, thickness, mvp, param_name: str):
+        h = C.c_void_p()
+        _check(_load().nosh_dkeo_dp_create(mesh._h, thickness._h, mvp._h, param_name.encode(),
+                                           C.byref(h)))
+        super().__init__(mesh, thickness, mvp, _h=h)
+
+
+class JacobianOperator(_Operator):
+    """jacobian_operator: J(psi) phi = (K + V + 2g|psi|^2) phi + g psi^2 conj(phi)."""
+    _destroy = "nosh_jac_destroy"
+    _apply = "nosh_jac_apply"
+
+    def __init__(self, mesh: Mesh, scalar_potential, thickness, keo: Keo, *, _h=None):
+        super().__init__()
+        self.mesh, self._keep = mesh, (scalar_potential, thickness, keo)
+        if _h is None:
+            _check(_load().nosh_jac_create(mesh._h, scalar_potential._h, thickness._h, keo._h,
+                                           C.byref(self._h)))
+        else:
+            self._h = _h
+
+    def rebuild(self, params: Dict[str, float], current_x):
+        p = _Params(params)
+        x = _vec(current_x, self._m())
+        _check(_lib.nosh_jac_rebuild(self._h, *p.args, _ptr(x)))
+
+    def apply_device(self, x_ptr: int, y_ptr: int, stream: int = 0):
+        _check(_lib.nosh_jac_apply_device(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr),
+                                          C.c_void_p(stream)))
+
+    def apply_internal(self, x_ptr: int, y_ptr: int, stream: int = 0):
+        _check(_lib.nosh_jac_apply_internal(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr),
+                                            C.c_void_p(stream)))
+
+    def apply_internal_fp32(self, x_ptr: int, y_ptr: int, stream: int = 0):
+        _check(_lib.nosh_jac_apply_internal_fp32(self._h, C.c_void_p(x_ptr),
+                                                 C.c_void_p(y_ptr), C.c_void_p(stream)))
+
+
+class KeoRegularized(_Operator):
+    """keo_regularized: P = K + g-blocks of psi; apply() is the approximate
+    inverse (Chebyshev iteration with 2x2 block-Jacobi scaling)."""
+    _destroy = "nosh_prec_destroy"
+    _apply = "nosh_prec_apply"
+
+    def __init__(self, mesh: Mesh, thickness, mvp, *, _h=None):
+        super().__init__()
+        self.mesh, self._keep = mesh, (thickness, mvp)
+        if _h is None:
+            _check(_load().nosh_prec_create(mesh._h, thickness._h, mvp._h, C.byref(self._h)))
+        else:
+            self._h = _h
+
+    def set_chebyshev(self, iters: int = 4, ratio: float = 30.0, upper: float = 0.0):
+        _check(_lib.nosh_prec_set_chebyshev(self._h, iters, ratio, upper))
+
+    def rebuild(self, params: Dict[str, float], current_x):
+        p = _Params(params)
+        x = _vec(current_x, self._m())
+        _check(_lib.nosh_prec_rebuild(self._h, *p.args, _ptr(x)))
+
+    def apply_operator(self, x) -> np.ndarray:
+        """y = P x (the regularized KEO itself, not its inverse)."""
+        x = _vec(x, self._m())
+        y = np.empty_like(x)
+        _check(_lib.nosh_prec_apply_operator(self._h, _ptr(x), _ptr(y)))
+        return y
+
+    def upper_bound(self) -> float:
+        v = C.c_double()
+        _check(_lib.nosh_prec_upper_bound(self._h, C.byref(v)))
+        return v.value
+
+
+class Nls(_Handle):
+    """model_evaluator::nls: F(psi) = (K + V + g|psi|^2) psi and its derivatives."""
+    _destroy = "nosh_nls_destroy"
+
+    def __init__(self, mesh: Mesh, mvp, scalar_potential, g: float, thickness,
+                 initial_x=None, deriv_parameter: str = "mu"):
+        super().__init__()
+        self.mesh, self._keep = mesh, (mvp, scalar_potential, thickness)
+        x0 = None if initial_x is None else _vec(initial_x, 2 * mesh.num_vertices)
+        self._x0 = x0
+        _check(_load().nosh_nls_create(mesh._h, mvp._h, scalar_potential._h, g, thickness._h,
+                                       None if x0 is None else _ptr(x0),
+                                       deriv_parameter.encode(), C.byref(self._h)))
+
+    def get_p_names(self):
+        n = C.c_int()
+        _check(_lib.nosh_nls_num_params(self._h, C.byref(n)))
+        out = []
+        buf = C.create_string_buffer(256)
+        for k in range(n.value):
+            _check(_lib.nosh_nls_param_name(self._h, k, buf, 256))
+            out.append(buf.value.decode())
+        return out
+
+    def get_nominal_values(self) -> Dict[str, float]:
+        out = {}
+        for k, name in enumerate(self.get_p_names()):
+            v = C.c_double()
+            _check(_lib.nosh_nls_param_nominal(self._h, k, C.byref(v)))
+            out[name] = v.value
+        return out
+
+    def eval_f(self, x, params: Dict[str, float]) -> np.ndarray:
+        m = 2 * self.mesh.num_vertices
+        x = _vec(x, m)
+        f = np.empty(m)
+        p = _Params(params)
+        _check(_lib.nosh_nls_eval_f(self._h, _ptr(x), *p.args, _ptr(f)))
+        return f
+
+    def eval_dfdp(self, x, params: Dict[str, float], column: int) -> np.ndarray:
+        m = 2 * self.mesh.num_vertices
+        x = _vec(x, m)
+        out = np.empty(m)
+        p = _Params(params)
+        _check(_lib.nosh_nls_eval_dfdp(self._h, _ptr(x), *p.args, column, _ptr(out)))
+        return out
+
+    def create_W_op(self) -> JacobianOperator:
+        h = C.c_void_p()
+        _check(_lib.nosh_nls_create_w_op(self._h, C.byref(h)))
+        op = JacobianOperator(self.mesh, None, None, None, _h=h)
+        op._keep = (self,)  # shares this model's keo
+        return op
+
+    def create_W_prec(self) -> KeoRegularized:
+        h = C.c_void_p()
+        _check(_lib.nosh_nls_create_w_prec(self._h, C.byref(h)))
+        op = KeoRegularized(self.mesh, None, None, _h=h)
+        op._keep = (self,)
+        return op
+
+
+def default_newton_options(**kw) -> NewtonOptions:
+    o = NewtonOptions()
+    _check(_load().nosh_newton_default_options(C.byref(o)))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    return o
+
+
+def newton_solve(model: Nls, psi0, params: Dict[str, float], options: Optional[NewtonOptions] = None,
+                 **kw):
+    """Newton-MINRES on F(psi) = 0; returns (psi, stats dict)."""
+    o = options if options is not None else default_newton_options(**kw)
+    psi = np.array(_vec(psi0, 2 * model.mesh.num_vertices), copy=True)
+    st = NewtonStats()
+    p = _Params(params)
+    _check(_lib.nosh_newton_solve(model._h, _ptr(psi), *p.args, C.byref(o), C.byref(st)))
+    return psi, st.as_dict()
+
+
+def minres_solve(jac: JacobianOperator, prec: Optional[KeoRegularized], b, rtol: float = 1e-10,
+                 max_iters: int = 1000):
+    """MINRES on J x = b from x = 0; returns (x, iterations, relative residual)."""
+    b = _vec(b, 2 * jac.mesh.num_vertices)
+    x = np.empty_like(b)
+    it, rr = C.c_int(), C.c_double()
+    _check(_lib.nosh_minres_solve(jac._h, None if prec is None else prec._h, _ptr(b), _ptr(x),
+                                  rtol, max_iters, C.byref(it), C.byref(rr)))
+    return x, it.value, rr.value

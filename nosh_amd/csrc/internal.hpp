@@ -1,0 +1,139 @@
+// internal.hpp -- device-resident objects of the nosh_amd library.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace nosh {
+
+// Rows per SELL slice: one wavefront, one row per lane.
+constexpr int kSliceH = 64;
+
+// Edge/covolume cache and the internal sparse layout of the complex graph.
+//
+// Reference-order arrays follow the caller's vertex and cell numbering.
+// Internal arrays are in the bandwidth-reduced vertex order (perm).
+// The off-diagonal structure of every operator is a sliced ELLPACK over the
+// internal rows: slice s holds rows [64 s, 64 s + 64); entry k of row r is
+// at slice_ptr[s] + 64 k + (r % 64). scol is the internal column, sedge the
+// reference edge id e coded as 2e (row = lower endpoint) or 2e+1 (row =
+// higher endpoint), -1 for padding (col = own row, value 0).
+struct MeshCache {
+  int nv = 3, ne = 3;
+  int64_t n = 0, E = 0, T = 0;
+  DBuf<double> coords;      // 3n
+  DBuf<int32_t> cells;      // T nv
+  DBuf<int32_t> edges;      // 2E
+  DBuf<int32_t> cell_edges; // T ne
+  DBuf<double> len, covol;  // E
+  DBuf<double> cv;          // n, reference order
+  DBuf<int32_t> perm;       // n: internal -> reference
+  DBuf<int32_t> iperm;      // n: reference -> internal
+  int64_t nslices = 0, nnz_pad = 0;
+  int max_width = 0;
+  DBuf<int32_t> slice_ptr;  // nslices + 1
+  DBuf<int32_t> scol;       // nnz_pad
+  DBuf<int32_t> sedge;      // nnz_pad
+  DBuf<double> cv_int;      // n, internal order
+};
+
+std::unique_ptr<MeshCache> build_mesh_cache(const double* coords, int64_t n, const int32_t* cells,
+                                            int64_t T, int nv, bool reorder, hipStream_t s);
+
+// Scalar field on vertices (constant with optional parameter, or explicit).
+struct ScalarField {
+  MeshCache* mesh = nullptr;
+  bool constant = true;
+  double c = 0.0;
+  std::string param;  // "" = none
+  double param_init = 0.0;
+  std::vector<double> values;  // explicit values, reference order (host copy)
+  DBuf<double> dev;            // internal order, current parameters
+  double dev_value = 0.0;      // constant value currently in dev
+  bool dev_valid = false;
+  std::map<std::string, double> scalar_parameters() const;
+  // Value at the given parameters, written to dev (internal order).
+  const double* get_v(const std::map<std::string, double>& p, hipStream_t s);
+  // dV/dp for parameter name (1 if it is this field's parameter, else 0).
+  double dvdp_constant(const std::string& name) const { return (constant && name == param) ? 1.0 : 0.0; }
+};
+
+// explicit_values magnetic potential: per-edge projection a_e / mu.
+struct VectorField {
+  MeshCache* mesh = nullptr;
+  double mu = 0.0;
+  DBuf<double> atilde;  // E, reference edge order: 0.5 (A_lo + A_hi).(x_lo - x_hi)
+};
+
+// Per-edge weights alpha_e = covolume/length * mean thickness, built once.
+struct EdgeWeights {
+  DBuf<double> alpha;  // E
+};
+
+template <class T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+
+// Assembled off-diagonal values of K (and of dK/dmu) in the SELL layout,
+// with the row sums of alpha (the diagonal of K) and of |alpha|.
+struct KeoValues {
+  DBuf<double2> val;   // nnz_pad
+  DBuf<double2> dval;  // nnz_pad (dK/dp), allocated on demand
+  DBuf<float2> val32;  // FP32 copy for the mixed-precision inner solver
+  DBuf<double> diag;   // n: sum of alpha over the row
+  DBuf<double> rowabs; // n: sum of |alpha| over the row
+  double mu = 0.0;
+  bool valid = false, dvalid = false, valid32 = false;
+};
+
+// Launchers (operators.hip).
+void launch_atilde(const MeshCache& m, const double* A_dev, double* atilde, hipStream_t s);
+void launch_alpha(const MeshCache& m, const double* thickness_ref, double* alpha, hipStream_t s);
+void launch_keo_fill(const MeshCache& m, const double* alpha, const double* atilde, double mu,
+                     bool dkeo, bool deriv_is_mu, KeoValues& kv, hipStream_t s);
+void launch_keo_to_fp32(const MeshCache& m, KeoValues& kv, hipStream_t s);
+void launch_permute_in(int64_t n, const int32_t* perm, const double* x_ref, double* x_int,
+                       hipStream_t s);
+void launch_permute_out(int64_t n, const int32_t* perm, const double* x_int, double* x_ref,
+                        hipStream_t s);
+void launch_gather_ref(int64_t n, const int32_t* perm, const double* v_ref, double* v_int,
+                       hipStream_t s);
+
+// y = K x (internal order).
+void launch_keo_apply(const MeshCache& m, const KeoValues& kv, const double* x, double* y,
+                      hipStream_t s);
+// y = dK x + (is_g ? ct |x|^2 x : ct dvdp x)   (dF/dp column).
+void launch_dfdp(const MeshCache& m, const KeoValues& kv, const double* ct, const double* x,
+                 bool is_g, double dvdp, double* y, hipStream_t s);
+// f = K x + ct (V + g |x|^2) x; optional ||f||^2 into *nrm2 (device scalar).
+void launch_compute_f(const MeshCache& m, const KeoValues& kv, const double* ct,
+                      const double* V, double g, const double* x, double* f, double* nrm2,
+                      hipStream_t s);
+// Per-vertex 2x2 blocks: J (with V) and P (keo_regularized), and the
+// Chebyshev bound into *bound (device).
+void launch_rebuild_blocks(const MeshCache& m, const KeoValues& kv, const double* ct,
+                           const double* V, double g, const double* psi, double* jblk,
+                           double* pblk, double* bound, hipStream_t s);
+// y = Koff x + B x with 3-component blocks B (J or P).
+void launch_block_apply(const MeshCache& m, const KeoValues& kv, const double* blk,
+                        const double* x, double* y, hipStream_t s);
+
+// Elementwise helpers.
+void launch_fill(double* a, int64_t n, double v, hipStream_t s);
+void launch_mul(const double* a, const double* b, double* c, int64_t n, hipStream_t s);
+// y = K x (internal order), FP32 values and vectors.
+void launch_block_apply32(const MeshCache& m, const KeoValues& kv, const float* blk,
+                          const float* x, float* y, hipStream_t s);
+
+}  // namespace nosh

@@ -1,0 +1,122 @@
+// kernels.hpp -- device building blocks shared by the operator and solver
+// kernels: the sliced-ELLPACK row product, XCD-aware block mapping and
+// deterministic block / grid reductions.
+#pragma once
+
+#include "internal.hpp"
+
+namespace nosh {
+
+template <class T>
+using V2 = typename Vec2<T>::type;
+
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs.
+// Remap so that XCD x processes one contiguous range of workgroup indices,
+// keeping each XCD's gathers of x within its own L2.
+__device__ __forceinline__ int64_t xcd_block(bool remap) {
+  const int64_t b = blockIdx.x;
+  if (!remap) return b;
+  const int64_t nb = gridDim.x;
+  const int64_t q = nb / 8, r = nb % 8;
+  const int64_t xcd = b % 8, idx = b / 8;
+  return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+__device__ __forceinline__ int32_t uniform(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Off-diagonal product of one internal row: sum_k val_k * x[col_k]
+// (complex multiply), in slot order.
+template <class T>
+__device__ __forceinline__ V2<T> sell_row(int64_t row, const int32_t* __restrict__ slice_ptr,
+                                          const int32_t* __restrict__ scol,
+                                          const V2<T>* __restrict__ val,
+                                          const V2<T>* __restrict__ x) {
+  const int32_t s = uniform(int32_t(row >> 6));
+  const int32_t b = uniform(slice_ptr[s]);
+  const int32_t e = uniform(slice_ptr[s + 1]);
+  const int lane = int(row & 63);
+  T ar = T(0), ai = T(0);
+#pragma unroll 4
+  for (int32_t j = b + lane; j < e; j += kSliceH) {
+    const int32_t c = __builtin_nontemporal_load(scol + j);
+    const V2<T> v = val[j];
+    const V2<T> xv = x[c];
+    ar += v.x * xv.x - v.y * xv.y;
+    ai += v.x * xv.y + v.y * xv.x;
+  }
+  V2<T> r;
+  r.x = ar;
+  r.y = ai;
+  return r;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Block-wide sums of NV values; the totals end up in every lane of wave 0.
+// lds must hold 16*NV doubles (blockDim <= 1024).
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    v[q] = wave_sum(v[q]);
+    if (lane == 0) lds[q * 16 + wid] = v[q];
+  }
+  __syncthreads();
+  if (wid == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = wave_sum(lane < nw ? lds[q * 16 + lane] : 0.0);
+  }
+  __syncthreads();
+}
+
+// Grid-wide deterministic sums: every block stores its partial sums, the
+// last block to finish adds all partials in block order and returns true
+// (totals in wave 0). partials: NV * gridDim.x doubles; counter starts at 0
+// and is reset by the last block.
+template <int NV>
+__device__ __forceinline__ bool grid_sum(double (&v)[NV], double* __restrict__ partials,
+                                         unsigned* __restrict__ counter) {
+  __shared__ double lds[16 * NV];
+  __shared__ int last;
+  block_sum<NV>(v, lds);
+  const unsigned nb = gridDim.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) partials[q * nb + blockIdx.x] = v[q];
+    __threadfence();
+    const unsigned prev = atomicAdd(counter, 1u);
+    last = (prev == nb - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double s = 0.0;
+    for (unsigned i = threadIdx.x; i < nb; i += blockDim.x)
+      s += __builtin_nontemporal_load(partials + q * nb + i);
+    v[q] = s;
+  }
+  block_sum<NV>(v, lds);
+  if (threadIdx.x == 0) *counter = 0u;
+  return true;
+}
+
+// Workspace for grid reductions (device memory).
+struct RedWs {
+  double* partials = nullptr;  // >= 4 * max grid
+  unsigned* counter = nullptr; // one per concurrent reduction slot
+};
+
+}  // namespace nosh

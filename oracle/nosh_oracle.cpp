@@ -1,0 +1,793 @@
+// nosh_oracle.cpp -- CPU restatement of nosh's NLS Newton-Krylov path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker for the HIP library in
+// nosh_amd/csrc: only tests/, __graft_entry__.smoke() and bench.py's CPU
+// baseline leg may load it. The product path never links or calls it.
+//
+// Each routine follows a file of the reference (paths relative to the nosh
+// source tree) and is written to be obviously equivalent to it, not fast:
+//   mesh edges / relations ...... src/mesh.cpp:629-691 (+ MOAB edge creation,
+//                                 src/mesh_reader.cpp:117-125)
+//   edge coefficients ........... src/mesh_tri.cpp:44-151, src/mesh_tetra.cpp:41-160
+//   control volumes ............. src/mesh_tri.cpp:185-210, src/mesh.cpp:895-1027,
+//                                 src/mesh_tetra.cpp:194-415
+//   edge projection of A ........ src/vector_field_explicit_values.cpp:13-90
+//   alpha cache ................. src/parameter_matrix_keo.cpp:186-232
+//   KEO / dKEO/dp assembly ...... src/parameter_matrix_keo.cpp:74-184,
+//                                 src/parameter_matrix_dkeo_dp.cpp:60-155
+//   F, dF/dp .................... src/model_evaluator_nls.cpp:527-695
+//   Jacobian .................... src/jacobian_operator.cpp:38-200
+//   keo_regularized operator .... src/keo_regularized.cpp:181-264
+// The Krylov and Newton layers (Belos MINRES, NOX Newton) are external to the
+// reference; MINRES below is the Paige-Saunders preconditioned MINRES and
+// Newton is a full-step Newton on ||F||_2, both as this project defines them.
+// The AMG inverse of keo_regularized (MueLu) is replaced by a Chebyshev
+// iteration with 2x2 block-Jacobi scaling, defined identically in the HIP
+// library.
+//
+// Parity pins (tests/test_oracle_*.py): closed-form edge coefficients and
+// control volumes of the structured meshes, sum of control volumes = |Omega|,
+// the per-cell moment identity sum_e c_e e e^T = |T| I, Hermitian KEO, K*1 = 0
+// at mu = 0, flux of the constant field through each triangle, gauge
+// covariance, finite-difference J and dF/dp, symmetry of J, <i psi, F(psi)> = 0,
+// dense brute-force assembly, MINRES on the (-1,2,-1) matrix against a dense
+// solve.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+struct V3 {
+  double x, y, z;
+};
+inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 scale(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline double norm(V3 a) { return std::sqrt(dot(a, a)); }
+
+// Solve A x = b (m x m, row-major) by Gaussian elimination with complete
+// pivoting -- the method of Eigen's fullPivLu used at src/mesh_tri.cpp:147
+// and src/mesh_tetra.cpp:155.
+void solve_full_pivot(int m, double* A, double* b, double* x) {
+  int colperm[6];
+  for (int i = 0; i < m; ++i) colperm[i] = i;
+  for (int k = 0; k < m; ++k) {
+    int pr = k, pc = k;
+    double best = -1.0;
+    for (int i = k; i < m; ++i)
+      for (int j = k; j < m; ++j)
+        if (std::fabs(A[i * m + j]) > best) {
+          best = std::fabs(A[i * m + j]);
+          pr = i;
+          pc = j;
+        }
+    if (pr != k) {
+      for (int j = 0; j < m; ++j) std::swap(A[k * m + j], A[pr * m + j]);
+      std::swap(b[k], b[pr]);
+    }
+    if (pc != k) {
+      for (int i = 0; i < m; ++i) std::swap(A[i * m + k], A[i * m + pc]);
+      std::swap(colperm[k], colperm[pc]);
+    }
+    const double piv = A[k * m + k];
+    for (int i = k + 1; i < m; ++i) {
+      const double f = A[i * m + k] / piv;
+      A[i * m + k] = 0.0;
+      for (int j = k + 1; j < m; ++j) A[i * m + j] -= f * A[k * m + j];
+      b[i] -= f * b[k];
+    }
+  }
+  double y[6];
+  for (int k = m - 1; k >= 0; --k) {
+    double s = b[k];
+    for (int j = k + 1; j < m; ++j) s -= A[k * m + j] * y[j];
+    y[k] = s / A[k * m + k];
+  }
+  for (int k = 0; k < m; ++k) x[colperm[k]] = y[k];
+}
+
+// src/mesh_tri.cpp:105-151 and src/mesh_tetra.cpp:105-160: the edge
+// coefficients c with |T| <u,u> = sum_i c_i <u,e_i>^2 for u in span(e).
+void edge_coefficients(int ne, const V3* e, double* c) {
+  double vol;
+  if (ne == 3) {
+    vol = 0.5 * norm(cross(e[0], e[1]));
+  } else {
+    // src/mesh_tetra.cpp:120-128,351-373: triple product of edges 0,1,2,
+    // falling back to edges 0,1,3 when 0,1,2 are coplanar.
+    auto tet_vol = [](V3 a, V3 b, V3 cc, bool* ok) {
+      const double al = dot(a, cross(b, cc));
+      *ok = !(std::fabs(al) / norm(a) / norm(b) / norm(cc) < 1.0e-5);
+      return std::fabs(al) / 6.0;
+    };
+    bool ok;
+    vol = tet_vol(e[0], e[1], e[2], &ok);
+    if (!ok) vol = tet_vol(e[0], e[1], e[3], &ok);
+  }
+  double A[36], rhs[6];
+  for (int i = 0; i < ne; ++i) {
+    const double al = dot(e[i], e[i]);
+    rhs[i] = vol * al;
+    A[i * ne + i] = al * al;
+    for (int j = i + 1; j < ne; ++j) {
+      const double v = dot(e[i], e[j]) * dot(e[j], e[i]);
+      A[i * ne + j] = v;
+      A[j * ne + i] = v;
+    }
+  }
+  solve_full_pivot(ne, A, rhs, c);
+}
+
+// src/mesh.cpp:903-930.
+V3 triangle_circumcenter(V3 n0, V3 n1, V3 n2) {
+  const V3 a = sub(n0, n1), b = sub(n1, n2), c = sub(n2, n0);
+  const V3 axb = cross(a, b);
+  const double omega = 2.0 * dot(axb, axb);
+  const double al = -dot(b, b) * dot(a, c) / omega;
+  const double be = -dot(c, c) * dot(b, a) / omega;
+  const double ga = -dot(a, a) * dot(c, b) / omega;
+  return add(add(scale(al, n0), scale(be, n1)), scale(ga, n2));
+}
+
+// src/mesh_tetra.cpp:375-415.
+V3 tet_circumcenter(const V3* v) {
+  const V3 r0 = sub(v[1], v[0]), r1 = sub(v[2], v[0]), r2 = sub(v[3], v[0]);
+  const double omega = 2.0 * dot(r0, cross(r1, r2));
+  const double al = dot(r0, r0) / omega, be = dot(r1, r1) / omega, ga = dot(r2, r2) / omega;
+  return add(add(add(v[0], scale(al, cross(r1, r2))), scale(be, cross(r2, r0))),
+             scale(ga, cross(r0, r1)));
+}
+
+// src/mesh.cpp:1003-1027.
+double covolume2d(V3 cc, V3 x0, V3 x1, V3 other0) {
+  const V3 mp = scale(0.5, add(x0, x1));
+  const double coedge = norm(sub(mp, cc));
+  const V3 cell_normal = cross(sub(other0, x0), sub(mp, x0));
+  const V3 cc_normal = cross(sub(cc, x0), sub(mp, x0));
+  return std::copysign(coedge, dot(cc_normal, cell_normal));
+}
+
+// src/mesh_tetra.cpp:273-331.
+double covolume3d(V3 cc, V3 x0, V3 x1, V3 o0, V3 o1) {
+  double cov = 0.0;
+  const V3 mp = scale(0.5, add(x0, x1));
+  const V3 ccf0 = triangle_circumcenter(x0, x1, o0);
+  const V3 ccf1 = triangle_circumcenter(x0, x1, o1);
+  const V3 gauge = cross(sub(o0, mp), sub(o1, mp));
+  const double a0 = 0.5 * norm(sub(mp, ccf0)) * norm(sub(ccf0, cc));
+  cov += std::copysign(a0, dot(cross(sub(ccf0, mp), sub(cc, mp)), gauge));
+  const double a1 = 0.5 * norm(sub(mp, ccf1)) * norm(sub(ccf1, cc));
+  cov += std::copysign(a1, dot(cross(sub(cc, mp), sub(ccf1, mp)), gauge));
+  return cov;
+}
+
+struct Mesh {
+  int dim = 2, nv = 3, ne = 3;
+  int64_t n = 0, T = 0, E = 0;
+  std::vector<V3> X;
+  std::vector<int32_t> cells;       // T * nv
+  std::vector<int32_t> edges;       // E * 2, (lower id, higher id)
+  std::vector<int32_t> cell_edges;  // T * ne, ascending edge id
+  std::vector<double> len, covol;   // E
+  std::vector<double> cv;           // n
+};
+
+// MOAB canonical local edges (tri: 01 12 20; tet: 01 12 20 03 13 23).
+const int kTriEdges[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+const int kTetEdges[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+
+void build_mesh(Mesh& m) {
+  const int(*le)[2] = m.nv == 3 ? kTriEdges : kTetEdges;
+  // Edge ids by first appearance (cells in file order, local edges in MOAB
+  // canonical order); endpoints sorted (src/mesh.cpp:670-677).
+  std::unordered_map<uint64_t, int32_t> id;
+  id.reserve(size_t(m.T) * m.ne / 2 + 16);
+  m.cell_edges.assign(size_t(m.T) * m.ne, 0);
+  m.edges.clear();
+  for (int64_t t = 0; t < m.T; ++t) {
+    for (int l = 0; l < m.ne; ++l) {
+      int32_t a = m.cells[t * m.nv + le[l][0]], b = m.cells[t * m.nv + le[l][1]];
+      if (a > b) std::swap(a, b);
+      const uint64_t key = uint64_t(a) * uint64_t(m.n) + uint64_t(b);
+      auto it = id.find(key);
+      int32_t e;
+      if (it == id.end()) {
+        e = int32_t(m.edges.size() / 2);
+        id.emplace(key, e);
+        m.edges.push_back(a);
+        m.edges.push_back(b);
+      } else {
+        e = it->second;
+      }
+      m.cell_edges[t * m.ne + l] = e;
+    }
+    // cell->edges sorted by edge handle (src/mesh.cpp:650-663).
+    std::sort(m.cell_edges.begin() + t * m.ne, m.cell_edges.begin() + (t + 1) * m.ne);
+  }
+  m.E = int64_t(m.edges.size() / 2);
+
+  // Edge data (src/mesh_tri.cpp:44-103, src/mesh_tetra.cpp:41-103).
+  std::vector<V3> ev(m.E);
+  m.len.assign(m.E, 0.0);
+  m.covol.assign(m.E, 0.0);
+  for (int64_t k = 0; k < m.E; ++k) {
+    ev[k] = sub(m.X[m.edges[2 * k]], m.X[m.edges[2 * k + 1]]);
+    m.len[k] = norm(ev[k]);
+  }
+  for (int64_t t = 0; t < m.T; ++t) {
+    V3 e[6];
+    double c[6];
+    for (int l = 0; l < m.ne; ++l) e[l] = ev[m.cell_edges[t * m.ne + l]];
+    edge_coefficients(m.ne, e, c);
+    for (int l = 0; l < m.ne; ++l) {
+      const int32_t k = m.cell_edges[t * m.ne + l];
+      m.covol[k] += c[l] * m.len[k];
+    }
+  }
+
+  // Control volumes.
+  m.cv.assign(m.n, 0.0);
+  for (int64_t t = 0; t < m.T; ++t) {
+    const int32_t* conn = &m.cells[t * m.nv];
+    V3 x[4];
+    for (int i = 0; i < m.nv; ++i) x[i] = m.X[conn[i]];
+    if (m.nv == 3) {
+      // src/mesh.cpp:929-978 (compute_triangle_splitting_).
+      const V3 cc = triangle_circumcenter(x[0], x[1], x[2]);
+      for (int e0 = 0; e0 < 3; ++e0)
+        for (int e1 = e0 + 1; e1 < 3; ++e1) {
+          const int other = 3 - e0 - e1;
+          const double el = norm(sub(x[e1], x[e0]));
+          const double cov = covolume2d(cc, x[e0], x[e1], x[other]);
+          const double pyr = 0.5 * el * cov / 2;
+          m.cv[conn[e0]] += pyr;
+          m.cv[conn[e1]] += pyr;
+        }
+    } else {
+      // src/mesh_tetra.cpp:194-271.
+      const V3 cc = tet_circumcenter(x);
+      for (int e0 = 0; e0 < 4; ++e0)
+        for (int e1 = e0 + 1; e1 < 4; ++e1) {
+          int other[2], no = 0;
+          for (int q = 0; q < 4; ++q)
+            if (q != e0 && q != e1) other[no++] = q;
+          const double el = norm(sub(x[e1], x[e0]));
+          const double cov = covolume3d(cc, x[e0], x[e1], x[other[0]], x[other[1]]);
+          const double pyr = 0.5 * el * cov / 3;
+          m.cv[conn[e0]] += pyr;
+          m.cv[conn[e1]] += pyr;
+        }
+    }
+  }
+}
+
+// Real CRS matrix over the complex graph (2n x 2n), the storage the
+// reference's Tpetra::CrsMatrix uses (src/mesh.cpp:787-881).
+struct Crs {
+  int64_t nrows = 0;
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> col;
+  std::vector<double> val;
+  void sum_into(int32_t r, int32_t c, double v) {
+    auto b = col.begin() + ptr[r], e = col.begin() + ptr[r + 1];
+    auto it = std::lower_bound(b, e, c);
+    val[it - col.begin()] += v;
+  }
+  void apply(const double* x, double* y) const {
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < nrows; ++r) {
+      double s = 0.0;
+      for (int64_t q = ptr[r]; q < ptr[r + 1]; ++q) s += val[q] * x[col[q]];
+      y[r] = s;
+    }
+  }
+};
+
+void build_complex_graph(const Mesh& m, Crs& K) {
+  std::vector<std::vector<int32_t>> nb(m.n);
+  for (int64_t k = 0; k < m.E; ++k) {
+    nb[m.edges[2 * k]].push_back(m.edges[2 * k + 1]);
+    nb[m.edges[2 * k + 1]].push_back(m.edges[2 * k]);
+  }
+  K.nrows = 2 * m.n;
+  K.ptr.assign(K.nrows + 1, 0);
+  for (int64_t v = 0; v < m.n; ++v) {
+    nb[v].push_back(int32_t(v));
+    std::sort(nb[v].begin(), nb[v].end());
+    K.ptr[2 * v + 1] = K.ptr[2 * v] + 2 * int64_t(nb[v].size());
+    K.ptr[2 * v + 2] = K.ptr[2 * v + 1] + 2 * int64_t(nb[v].size());
+  }
+  K.col.resize(K.ptr[K.nrows]);
+  K.val.assign(K.ptr[K.nrows], 0.0);
+  for (int64_t v = 0; v < m.n; ++v)
+    for (int c = 0; c < 2; ++c) {
+      int64_t q = K.ptr[2 * v + c];
+      for (int32_t w : nb[v]) {
+        K.col[q++] = 2 * w;
+        K.col[q++] = 2 * w + 1;
+      }
+    }
+}
+
+struct Model {
+  Mesh* mesh = nullptr;
+  std::vector<double> thickness, V, dVdp, A;  // n, n, n, 3n
+  std::vector<double> atilde, alpha;           // E: a_e / mu, alpha_e
+  std::string deriv_param = "mu";
+  std::string v_param;  // parameter name V depends on ("" = none)
+  double g = 1.0, mu = 0.0;
+  Crs K, dK, P;
+  std::vector<double> diag0, diag1b;  // Jacobian diagonals
+};
+
+// src/parameter_matrix_keo.cpp:74-184 (dkeo=false) and
+// src/parameter_matrix_dkeo_dp.cpp:60-155 (dkeo=true).
+void fill_keo(Model& md, Crs& M, bool dkeo) {
+  const Mesh& m = *md.mesh;
+  std::fill(M.val.begin(), M.val.end(), 0.0);
+  for (int64_t k = 0; k < m.E; ++k) {
+    const double a_int = md.mu * md.atilde[k];
+    const double s = std::sin(a_int), c = std::cos(a_int);
+    double v[3];
+    if (!dkeo) {
+      v[0] = -c * md.alpha[k];
+      v[1] = -s * md.alpha[k];
+      v[2] = md.alpha[k];
+    } else {
+      const double dAdP = (md.deriv_param == "mu") ? md.atilde[k] : 0.0;
+      v[0] = dAdP * s * md.alpha[k];
+      v[1] = -dAdP * c * md.alpha[k];
+      v[2] = 0.0;
+    }
+    const double vals[4][4] = {{v[2], 0.0, v[0], v[1]},
+                               {0.0, v[2], -v[1], v[0]},
+                               {v[0], -v[1], v[2], 0.0},
+                               {v[1], v[0], 0.0, v[2]}};
+    const int32_t i0 = m.edges[2 * k], i1 = m.edges[2 * k + 1];
+    const int32_t idx[4] = {2 * i0, 2 * i0 + 1, 2 * i1, 2 * i1 + 1};
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) M.sum_into(idx[i], idx[j], vals[i][j]);
+  }
+}
+
+void set_params(Model& md, double g, double mu) {
+  md.g = g;
+  md.mu = mu;
+  fill_keo(md, md.K, false);
+  fill_keo(md, md.dK, true);
+}
+
+double v_at(const Model& md, int64_t k) { return md.V[k]; }
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------- mesh ----
+void* orc_mesh_new(const double* coords, int64_t n, const int32_t* cells, int64_t ncells,
+                   int nv) {
+  auto* m = new Mesh;
+  m->nv = nv;
+  m->ne = nv == 3 ? 3 : 6;
+  m->dim = nv == 3 ? 2 : 3;
+  m->n = n;
+  m->T = ncells;
+  m->X.resize(n);
+  for (int64_t i = 0; i < n; ++i) m->X[i] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
+  m->cells.assign(cells, cells + ncells * nv);
+  build_mesh(*m);
+  return m;
+}
+void orc_mesh_free(void* p) { delete static_cast<Mesh*>(p); }
+int64_t orc_mesh_num_edges(void* p) { return static_cast<Mesh*>(p)->E; }
+void orc_mesh_edges(void* p, int32_t* out) {
+  auto* m = static_cast<Mesh*>(p);
+  std::memcpy(out, m->edges.data(), sizeof(int32_t) * m->edges.size());
+}
+void orc_mesh_cell_edges(void* p, int32_t* out) {
+  auto* m = static_cast<Mesh*>(p);
+  std::memcpy(out, m->cell_edges.data(), sizeof(int32_t) * m->cell_edges.size());
+}
+void orc_mesh_edge_data(void* p, double* len, double* covol) {
+  auto* m = static_cast<Mesh*>(p);
+  std::memcpy(len, m->len.data(), sizeof(double) * m->E);
+  std::memcpy(covol, m->covol.data(), sizeof(double) * m->E);
+}
+void orc_mesh_control_volumes(void* p, double* cv) {
+  auto* m = static_cast<Mesh*>(p);
+  std::memcpy(cv, m->cv.data(), sizeof(double) * m->n);
+}
+// Per-cell edge coefficients c_{T,e} in cell_edges order (for the moment test).
+void orc_cell_edge_coefficients(void* p, double* out) {
+  auto* m = static_cast<Mesh*>(p);
+  for (int64_t t = 0; t < m->T; ++t) {
+    V3 e[6];
+    for (int l = 0; l < m->ne; ++l) {
+      const int32_t k = m->cell_edges[t * m->ne + l];
+      e[l] = sub(m->X[m->edges[2 * k]], m->X[m->edges[2 * k + 1]]);
+    }
+    edge_coefficients(m->ne, e, out + t * m->ne);
+  }
+}
+
+// --------------------------------------------------------------- model ----
+// thickness, V, dVdp: n values; A: n x 3. v_param: the parameter V depends
+// on ("" for none); deriv_param: the DkeoDP parameter
+// (src/model_evaluator_nls.cpp:74-76).
+void* orc_model_new(void* mesh, const double* thickness, const double* V, const double* dVdp,
+                    const double* A, const char* deriv_param) {
+  auto* md = new Model;
+  md->mesh = static_cast<Mesh*>(mesh);
+  const Mesh& m = *md->mesh;
+  md->thickness.assign(thickness, thickness + m.n);
+  md->V.assign(V, V + m.n);
+  md->dVdp.assign(dVdp, dVdp + m.n);
+  md->A.assign(A, A + 3 * m.n);
+  md->deriv_param = deriv_param;
+  // src/vector_field_explicit_values.cpp:13-50.
+  md->atilde.resize(m.E);
+  md->alpha.resize(m.E);
+  for (int64_t k = 0; k < m.E; ++k) {
+    const int32_t i0 = m.edges[2 * k], i1 = m.edges[2 * k + 1];
+    const V3 A0{A[3 * i0], A[3 * i0 + 1], A[3 * i0 + 2]};
+    const V3 A1{A[3 * i1], A[3 * i1 + 1], A[3 * i1 + 2]};
+    md->atilde[k] = dot(scale(0.5, add(A0, A1)), sub(m.X[i0], m.X[i1]));
+    // src/parameter_matrix_keo.cpp:224-227.
+    md->alpha[k] = m.covol[k] / m.len[k] * 0.5 * (thickness[i0] + thickness[i1]);
+  }
+  build_complex_graph(m, md->K);
+  md->dK = md->K;
+  md->P = md->K;
+  md->diag0.assign(2 * m.n, 0.0);
+  md->diag1b.assign(m.n, 0.0);
+  return md;
+}
+void orc_model_free(void* p) { delete static_cast<Model*>(p); }
+void orc_model_edge_fields(void* p, double* atilde, double* alpha) {
+  auto* md = static_cast<Model*>(p);
+  std::memcpy(atilde, md->atilde.data(), sizeof(double) * md->atilde.size());
+  std::memcpy(alpha, md->alpha.data(), sizeof(double) * md->alpha.size());
+}
+void orc_set_params(void* p, double g, double mu) { set_params(*static_cast<Model*>(p), g, mu); }
+
+int64_t orc_keo_nnz(void* p) { return static_cast<Model*>(p)->K.ptr.back(); }
+// which: 0 = K, 1 = dK/dp, 2 = keo_regularized P.
+void orc_crs_get(void* p, int which, int64_t* ptr, int32_t* col, double* val) {
+  auto* md = static_cast<Model*>(p);
+  const Crs& M = which == 0 ? md->K : which == 1 ? md->dK : md->P;
+  std::memcpy(ptr, M.ptr.data(), sizeof(int64_t) * M.ptr.size());
+  std::memcpy(col, M.col.data(), sizeof(int32_t) * M.col.size());
+  std::memcpy(val, M.val.data(), sizeof(double) * M.val.size());
+}
+
+void orc_keo_apply(void* p, const double* x, double* y) { static_cast<Model*>(p)->K.apply(x, y); }
+void orc_dkeo_apply(void* p, const double* x, double* y) {
+  static_cast<Model*>(p)->dK.apply(x, y);
+}
+
+// src/model_evaluator_nls.cpp:527-628.
+void orc_compute_f(void* p, const double* x, double* f) {
+  auto* md = static_cast<Model*>(p);
+  const Mesh& m = *md->mesh;
+  md->K.apply(x, f);
+  for (int64_t k = 0; k < m.n; ++k) {
+    const double a = m.cv[k] * md->thickness[k] *
+                     (v_at(*md, k) + md->g * (x[2 * k] * x[2 * k] + x[2 * k + 1] * x[2 * k + 1]));
+    f[2 * k] += a * x[2 * k];
+    f[2 * k + 1] += a * x[2 * k + 1];
+  }
+}
+
+// src/model_evaluator_nls.cpp:630-695. param: "g" or any other name.
+void orc_compute_dfdp(void* p, const double* x, const char* param, double* f) {
+  auto* md = static_cast<Model*>(p);
+  const Mesh& m = *md->mesh;
+  md->dK.apply(x, f);
+  const bool is_g = std::string(param) == "g";
+  const bool v_dep = !md->v_param.empty() && md->v_param == param;
+  for (int64_t k = 0; k < m.n; ++k) {
+    double a;
+    if (is_g)
+      a = m.cv[k] * md->thickness[k] * (x[2 * k] * x[2 * k] + x[2 * k + 1] * x[2 * k + 1]);
+    else
+      a = m.cv[k] * md->thickness[k] * (v_dep ? md->dVdp[k] : 0.0);
+    f[2 * k] += a * x[2 * k];
+    f[2 * k + 1] += a * x[2 * k + 1];
+  }
+}
+void orc_set_v_param(void* p, const char* name) { static_cast<Model*>(p)->v_param = name; }
+
+// src/jacobian_operator.cpp:143-200.
+void orc_jac_rebuild(void* p, const double* x) {
+  auto* md = static_cast<Model*>(p);
+  const Mesh& m = *md->mesh;
+  const double g = md->g;
+  for (int64_t k = 0; k < m.n; ++k) {
+    const double ct = m.cv[k] * md->thickness[k];
+    const double al = ct * (v_at(*md, k) + g * 2.0 * (x[2 * k] * x[2 * k] + x[2 * k + 1] * x[2 * k + 1]));
+    const double rx2 = g * ct * (x[2 * k] * x[2 * k] - x[2 * k + 1] * x[2 * k + 1]);
+    md->diag0[2 * k] = al + rx2;
+    md->diag0[2 * k + 1] = al - rx2;
+    md->diag1b[k] = g * ct * (2.0 * x[2 * k] * x[2 * k + 1]);
+  }
+}
+// src/jacobian_operator.cpp:38-108.
+void orc_jac_apply(void* p, const double* x, double* y) {
+  auto* md = static_cast<Model*>(p);
+  const int64_t n = md->mesh->n;
+  md->K.apply(x, y);
+  for (int64_t k = 0; k < n; ++k) {
+    y[2 * k] += md->diag0[2 * k] * x[2 * k] + md->diag1b[k] * x[2 * k + 1];
+    y[2 * k + 1] += md->diag1b[k] * x[2 * k] + md->diag0[2 * k + 1] * x[2 * k + 1];
+  }
+}
+
+// src/keo_regularized.cpp:181-264: P = K + (if g > 0) the 2x2 blocks
+// [[alpha+gamma, beta], [beta, alpha-gamma]].
+void orc_prec_rebuild(void* p, const double* x) {
+  auto* md = static_cast<Model*>(p);
+  const Mesh& m = *md->mesh;
+  md->P.val = md->K.val;
+  const double g = md->g;
+  if (g > 0.0) {
+    for (int64_t k = 0; k < m.n; ++k) {
+      const double ct = m.cv[k] * md->thickness[k];
+      const double al = g * ct * 2.0 * (x[2 * k] * x[2 * k] + x[2 * k + 1] * x[2 * k + 1]);
+      const double be = g * ct * (2.0 * x[2 * k] * x[2 * k + 1]);
+      const double ga = g * ct * (x[2 * k] * x[2 * k] - x[2 * k + 1] * x[2 * k + 1]);
+      md->P.sum_into(int32_t(2 * k), int32_t(2 * k), al + ga);
+      md->P.sum_into(int32_t(2 * k), int32_t(2 * k + 1), be);
+      md->P.sum_into(int32_t(2 * k + 1), int32_t(2 * k), be);
+      md->P.sum_into(int32_t(2 * k + 1), int32_t(2 * k + 1), al - ga);
+    }
+  }
+}
+void orc_prec_op_apply(void* p, const double* x, double* y) {
+  static_cast<Model*>(p)->P.apply(x, y);
+}
+
+}  // extern "C"
+
+namespace {
+
+// 2x2 diagonal blocks of P: (a, b, c) for [[a, b], [b, c]].
+void prec_blocks(const Model& md, std::vector<double>& blk) {
+  const Crs& P = md.P;
+  const int64_t n = md.mesh->n;
+  blk.assign(3 * n, 0.0);
+  for (int64_t k = 0; k < n; ++k) {
+    for (int64_t q = P.ptr[2 * k]; q < P.ptr[2 * k + 1]; ++q) {
+      if (P.col[q] == 2 * k) blk[3 * k] = P.val[q];
+      if (P.col[q] == 2 * k + 1) blk[3 * k + 1] = P.val[q];
+    }
+    for (int64_t q = P.ptr[2 * k + 1]; q < P.ptr[2 * k + 2]; ++q)
+      if (P.col[q] == 2 * k + 1) blk[3 * k + 2] = P.val[q];
+  }
+}
+
+// Upper bound of the spectrum of D^{-1} P (D = 2x2 block diagonal of P):
+// lambda <= 1 + max_k (sum_{j != k} |K_kj|) / lambda_min(D_k), which is <= 2
+// whenever every edge weight is nonnegative.
+double cheb_upper_bound(const Model& md, const std::vector<double>& blk) {
+  const Crs& P = md.P;
+  const int64_t n = md.mesh->n;
+  double best = 0.0;
+  for (int64_t k = 0; k < n; ++k) {
+    double off = 0.0;
+    for (int64_t q = P.ptr[2 * k]; q < P.ptr[2 * k + 1]; q += 2) {
+      const int32_t w = P.col[q] / 2;
+      if (w == k) continue;
+      off += std::hypot(P.val[q], P.val[q + 1]);
+    }
+    const double a = blk[3 * k], b = blk[3 * k + 1], c = blk[3 * k + 2];
+    const double lmin = 0.5 * (a + c) - std::hypot(0.5 * (a - c), b);
+    best = std::max(best, off / lmin);
+  }
+  return 1.0 + best;
+}
+
+// Chebyshev iteration of `iters` steps (iters-1 products with P) for P z = r
+// on the interval [hi/ratio, hi] of D^{-1}P, from z = 0.
+void cheb_apply(const Model& md, const std::vector<double>& blk, int iters, double lo, double hi,
+                const double* r, double* z) {
+  const int64_t n = md.mesh->n;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+  const double sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  std::vector<double> d(2 * n), res(2 * n);
+  auto dinv = [&](int64_t k, double x0, double x1, double* o0, double* o1) {
+    const double a = blk[3 * k], b = blk[3 * k + 1], c = blk[3 * k + 2];
+    const double det = a * c - b * b;
+    *o0 = (c * x0 - b * x1) / det;
+    *o1 = (a * x1 - b * x0) / det;
+  };
+  for (int64_t k = 0; k < n; ++k) {
+    double u0, u1;
+    dinv(k, r[2 * k], r[2 * k + 1], &u0, &u1);
+    d[2 * k] = u0 / theta;
+    d[2 * k + 1] = u1 / theta;
+    z[2 * k] = d[2 * k];
+    z[2 * k + 1] = d[2 * k + 1];
+  }
+  for (int it = 1; it < iters; ++it) {
+    md.P.apply(z, res.data());
+    const double rho_new = 1.0 / (2.0 * sigma - rho);
+    const double c1 = rho_new * rho, c2 = 2.0 * rho_new / delta;
+    for (int64_t k = 0; k < n; ++k) {
+      double u0, u1;
+      dinv(k, r[2 * k] - res[2 * k], r[2 * k + 1] - res[2 * k + 1], &u0, &u1);
+      d[2 * k] = c1 * d[2 * k] + c2 * u0;
+      d[2 * k + 1] = c1 * d[2 * k + 1] + c2 * u1;
+      z[2 * k] += d[2 * k];
+      z[2 * k + 1] += d[2 * k + 1];
+    }
+    rho = rho_new;
+  }
+}
+
+struct PrecCfg {
+  int iters;  // 0: no preconditioner
+  double ratio;
+};
+
+double ddot(int64_t m, const double* a, const double* b) {
+  double s = 0.0;
+  for (int64_t i = 0; i < m; ++i) s += a[i] * b[i];
+  return s;
+}
+
+// Preconditioned MINRES (Paige & Saunders 1975), zero initial guess.
+// Stops when the recurred residual norm ||r||_{M^-1} falls to rtol * ||b||_{M^-1}.
+// op(x, y): y = A x; prec(r, z): z = M^{-1} r. Returns the iteration count.
+template <class Op, class Prec>
+int minres(int64_t m, Op op, Prec prec, const double* b, double* x, double rtol, int maxit,
+           double* relres_out, double* hist) {
+  std::vector<double> r1(b, b + m), r2(b, b + m), y(m), v(m), w(m, 0.0), w1(m), w2(m, 0.0);
+  std::fill(x, x + m, 0.0);
+  prec(r1.data(), y.data());
+  double beta1 = ddot(m, r1.data(), y.data());
+  if (beta1 <= 0.0) {
+    *relres_out = 0.0;
+    return 0;
+  }
+  beta1 = std::sqrt(beta1);
+  double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1;
+  double cs = -1.0, sn = 0.0;
+  int it = 0;
+  double relres = 1.0;
+  while (it < maxit) {
+    ++it;
+    const double s = 1.0 / beta;
+    for (int64_t i = 0; i < m; ++i) v[i] = s * y[i];
+    op(v.data(), y.data());
+    if (it >= 2)
+      for (int64_t i = 0; i < m; ++i) y[i] -= (beta / oldb) * r1[i];
+    const double alfa = ddot(m, v.data(), y.data());
+    for (int64_t i = 0; i < m; ++i) y[i] -= (alfa / beta) * r2[i];
+    r1.swap(r2);
+    for (int64_t i = 0; i < m; ++i) r2[i] = y[i];
+    prec(r2.data(), y.data());
+    oldb = beta;
+    beta = std::sqrt(std::max(ddot(m, r2.data(), y.data()), 0.0));
+    const double oldeps = epsln;
+    const double delta = cs * dbar + sn * alfa;
+    const double gbar = sn * dbar - cs * alfa;
+    epsln = sn * beta;
+    dbar = -cs * beta;
+    double gamma = std::hypot(gbar, beta);
+    gamma = std::max(gamma, 1e-300);
+    cs = gbar / gamma;
+    sn = beta / gamma;
+    const double phi = cs * phibar;
+    phibar = sn * phibar;
+    const double denom = 1.0 / gamma;
+    for (int64_t i = 0; i < m; ++i) {
+      const double wnew = (v[i] - oldeps * w1[i] - delta * w2[i]) * denom;
+      w1[i] = w2[i];
+      w2[i] = wnew;
+      x[i] += phi * wnew;
+    }
+    relres = phibar / beta1;
+    if (hist) hist[it - 1] = relres;
+    if (relres <= rtol || beta == 0.0) break;
+  }
+  *relres_out = relres;
+  return it;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Chebyshev-Jacobi approximate inverse of keo_regularized: z = M^{-1} r.
+// hi <= 0 selects the automatic bound cheb_upper_bound(); lo = hi/ratio.
+void orc_cheb_apply(void* p, int iters, double ratio, double hi, const double* r, double* z) {
+  auto* md = static_cast<Model*>(p);
+  std::vector<double> blk;
+  prec_blocks(*md, blk);
+  if (hi <= 0.0) hi = cheb_upper_bound(*md, blk);
+  cheb_apply(*md, blk, iters, hi / ratio, hi, r, z);
+}
+double orc_cheb_bound(void* p) {
+  auto* md = static_cast<Model*>(p);
+  std::vector<double> blk;
+  prec_blocks(*md, blk);
+  return cheb_upper_bound(*md, blk);
+}
+
+// MINRES on the Jacobian (current orc_jac_rebuild state), preconditioned by
+// the Chebyshev-Jacobi inverse of P (current orc_prec_rebuild state) when
+// cheb_iters > 0.
+int orc_minres_jac(void* p, const double* b, double* x, double rtol, int maxit, int cheb_iters,
+                   double ratio, double* relres, double* hist) {
+  auto* md = static_cast<Model*>(p);
+  const int64_t m = 2 * md->mesh->n;
+  std::vector<double> blk;
+  double hi = 0.0;
+  if (cheb_iters > 0) {
+    prec_blocks(*md, blk);
+    hi = cheb_upper_bound(*md, blk);
+  }
+  auto op = [&](const double* xx, double* yy) { orc_jac_apply(p, xx, yy); };
+  auto prec = [&](const double* r, double* z) {
+    if (cheb_iters > 0)
+      cheb_apply(*md, blk, cheb_iters, hi / ratio, hi, r, z);
+    else
+      std::memcpy(z, r, sizeof(double) * m);
+  };
+  return minres(m, op, prec, b, x, rtol, maxit, relres, hist);
+}
+
+// MINRES on a user CRS matrix (unpreconditioned): for the Poisson check.
+int orc_minres_crs(int64_t m, const int64_t* ptr, const int32_t* col, const double* val,
+                   const double* b, double* x, double rtol, int maxit, double* relres) {
+  auto op = [&](const double* xx, double* yy) {
+    for (int64_t r = 0; r < m; ++r) {
+      double s = 0.0;
+      for (int64_t q = ptr[r]; q < ptr[r + 1]; ++q) s += val[q] * xx[col[q]];
+      yy[r] = s;
+    }
+  };
+  auto prec = [&](const double* r, double* z) { std::memcpy(z, r, sizeof(double) * m); };
+  return minres(m, op, prec, b, x, rtol, maxit, relres, nullptr);
+}
+
+// Full-step Newton: psi <- psi + delta, J(psi) delta = -F(psi) by MINRES
+// with relative tolerance minres_rtol, until ||F(psi)||_2 <= tol.
+// normf_hist[i] = ||F|| before step i; minres_hist[i] = MINRES iterations of
+// step i. Returns the number of Newton steps taken, or -1 if maxit was hit.
+int orc_newton(void* p, double* psi, double g, double mu, double tol, int maxit,
+               double minres_rtol, int minres_maxit, int cheb_iters, double ratio,
+               double* normf_hist, int* minres_hist) {
+  auto* md = static_cast<Model*>(p);
+  const int64_t m = 2 * md->mesh->n;
+  set_params(*md, g, mu);
+  std::vector<double> f(m), rhs(m), dx(m);
+  for (int it = 0;; ++it) {
+    orc_compute_f(p, psi, f.data());
+    const double nf = std::sqrt(ddot(m, f.data(), f.data()));
+    if (normf_hist) normf_hist[it] = nf;
+    if (nf <= tol) return it;
+    if (it >= maxit) return -1;
+    orc_jac_rebuild(p, psi);
+    orc_prec_rebuild(p, psi);
+    for (int64_t i = 0; i < m; ++i) rhs[i] = -f[i];
+    double rr;
+    const int k = orc_minres_jac(p, rhs.data(), dx.data(), minres_rtol, minres_maxit, cheb_iters,
+                                 ratio, &rr, nullptr);
+    if (minres_hist) minres_hist[it] = k;
+    for (int64_t i = 0; i < m; ++i) psi[i] += dx[i];
+  }
+}
+
+}  // extern "C"
