@@ -1,0 +1,327 @@
+"""Benchmark: Newton-MINRES solves/sec at ||F||_2 <= 1e-10 on the 96^3 Kuhn cube.
+
+Workload (one "step" = one complete Newton-MINRES solve): cube [-5,5]^3,
+96^3 cells split into 6 Kuhn tetrahedra each (912,673 vertices, 5,308,416
+tets, 6,276,384 edges), A = mu*0.5*(-y, x, 0) with mu = 0.4, V = -1, g = 1,
+thickness 1, psi_0 seeded random (|psi| ~ U(0.5,1), arg ~ U(0, 2 pi), seed 0),
+MINRES preconditioned by keo_regularized (Chebyshev-Jacobi). The state is on
+the device before the timed region starts; each step restores psi_0 and
+solves.
+
+Variants: "plain" = FP64 Newton-MINRES (MINRES rtol 1e-10); "mixed" = FP32
+inner MINRES (values, blocks and Krylov vectors in FP32, inner products and
+scalars in FP64) inside FP64 Newton (F, update) with an Eisenstat-Walker
+forcing term. ``value`` is the solves/sec of the fastest variant that reaches
+||F|| <= 1e-10; both are reported.
+
+Roofline: the FP64 Jacobian apply J(psi) phi, timed with HIP events on the
+stream it is launched on; algorithmic bytes per launch 40E + 68n (complex CSR
+storing both directions: 16 B value + 4 B column per off-diagonal entry, 4 B
+row pointer and 8 B diagonal per row, phi / y / psi 16 B and c*t 8 B per
+vertex) against 8 TB/s.
+
+CPU baseline: the oracle (a C++ restatement of the reference operators and
+solver, OpenMP) on a bounded sample of the same workload on this host: the
+cost of F, the Jacobian rebuild and MINRES iterations is measured and
+combined with the GPU solve's own Newton/MINRES iteration counts.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=2)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--ncell", type=int, default=96)
+    p.add_argument("--mu", type=float, default=0.4)
+    p.add_argument("--cheb", type=int, default=6)
+    p.add_argument("--ratio", type=float, default=300.0)
+    p.add_argument("--minres-maxit", type=int, default=2000)
+    p.add_argument("--spmv-reps", type=int, default=50)
+    p.add_argument("--variants", default="plain,mixed")
+    p.add_argument("--roofline-only", action="store_true",
+                   help="only the J.phi timing loop and a copy kernel (for profilers)")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--verbose", action="store_true")
+    return p.parse_args()
+
+
+def log(args, *a):
+    if args.verbose:
+        print(*a, file=sys.stderr, flush=True)
+
+
+def _profiled_traffic(ncell):
+    """HBM bytes per J.phi launch from the committed rocprofv3 PMC summary of
+    this workload (FETCH_SIZE + WRITE_SIZE, calibrated), if there is one."""
+    path = os.path.join(ROOT, "profiles", f"jphi_fp64_kuhn{ncell}.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":")[1])), os.path.relpath(path, ROOT)
+    return None, None
+
+
+def cpu_baseline(coords, cells, A, mu, psi0, phi, gpu_stats, cheb, ratio):
+    """Time the oracle on this host; extrapolate one solve from the GPU counts."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    om = O.Model(O.Mesh(coords, cells), A)
+    om.set_params(1.0, mu)
+    setup = time.perf_counter() - t0
+
+    def med(fn, reps=5):
+        ts = []
+        for _ in range(reps):
+            s = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - s)
+        return float(np.median(ts))
+
+    t_f = med(lambda: om.f(psi0))
+    t_k = med(lambda: om.keo(phi))
+
+    def rebuild():
+        om.jac_rebuild(psi0)
+        om.prec_rebuild(psi0)
+
+    t_rb = med(rebuild, 3)
+    t_j = med(lambda: om.jac(phi))
+    b = -om.f(psi0)
+    k_sample = 20
+    s = time.perf_counter()
+    _, k, _ = om.minres(b, rtol=1e-30, maxit=k_sample, cheb_iters=cheb, ratio=ratio)
+    t_it = (time.perf_counter() - s) / max(k, 1)
+    steps = gpu_stats["newton_iters"]
+    its = gpu_stats["minres_iters_total"]
+    t_solve = (steps + 1) * t_f + steps * t_rb + its * t_it
+    sample = (f"oracle on {threads} threads: F {t_f*1e3:.1f} ms, K.phi {t_k*1e3:.1f} ms, "
+              f"J.phi {t_j*1e3:.1f} ms, rebuild {t_rb*1e3:.1f} ms, MINRES iteration "
+              f"{t_it*1e3:.1f} ms (mean of {k} with Chebyshev {cheb}); one solve = "
+              f"{steps + 1} F + {steps} rebuilds + {its} MINRES iterations, the plain GPU "
+              f"solve's counts; setup {setup:.1f} s")
+    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": sample, "jphi_ms": t_j * 1e3, "seconds_per_solve": t_solve}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is design-only in this build.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import nosh_amd as nosh
+    from nosh_amd import meshgen as mg
+
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    nc = args.ncell
+    coords, cells = mg.kuhn_cube(nc)
+    n = coords.shape[0]
+    A = mg.magnetic_potential(coords)
+    mu, g = args.mu, 1.0
+    params = {"g": g, "mu": mu}
+    psi0 = mg.random_state(n, 0)
+    phi = mg.random_state(n, 1)
+
+    t0 = time.perf_counter()
+    mesh = nosh.Mesh(coords, cells)
+    t_mesh = time.perf_counter() - t0
+    E = mesh.num_edges
+    log(args, f"mesh n={n} E={E} build {t_mesh:.2f}s")
+    mvp = nosh.VectorFieldExplicitValues(mesh, A, mu)
+    V = nosh.ScalarFieldConstant(mesh, -1.0)
+    t = nosh.ScalarFieldConstant(mesh, 1.0)
+    model = nosh.Nls(mesh, mvp, V, g, t)
+
+    # ---- roofline: FP64 J(psi) phi on the library's launch stream.
+    J = model.create_W_op()
+    J.rebuild(params, psi0)
+    stream = torch.cuda.Stream(device=dev)
+    x_ref = torch.from_numpy(phi).to(dev)
+    x = torch.empty_like(x_ref)
+    y = torch.empty_like(x_ref)
+    mesh.to_internal(x_ref.data_ptr(), x.data_ptr(), stream.cuda_stream)
+    with torch.cuda.stream(stream):
+        for _ in range(5):
+            J.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(args.spmv_reps):
+            J.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+        ev1.record(stream)
+    ev1.synchronize()
+    t_j = ev0.elapsed_time(ev1) / 1e3 / args.spmv_reps
+    bytes_canon = 40 * E + 68 * n
+    ach = bytes_canon / t_j / 1e9
+    # Cold-cache variant: reading 1 GiB between launches replaces the 256 MB
+    # Infinity Cache and the L2s with clean lines, so every launch streams its
+    # operands from HBM (a write-based flush would leave dirty lines whose
+    # write-back the next launch would pay for).
+    flush = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+    cold = []
+    with torch.cuda.stream(stream):
+        for _ in range(10):
+            fsum = flush.sum()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            J.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+            e1.record(stream)
+            cold.append((e0, e1))
+    torch.cuda.synchronize()
+    t_cold = float(np.median([a.elapsed_time(b) for a, b in cold])) / 1e3
+    del flush, fsum
+    layout = mesh.layout()
+    bytes_layout = 20 * layout["nnz_pad"] + 4 * (layout["nslices"] + 1) + 56 * n
+    # Copy-kernel reference of the same size class (stream bandwidth).
+    cbuf = torch.empty(bytes_canon // 8, dtype=torch.float64, device=dev)
+    cdst = torch.empty_like(cbuf)
+    with torch.cuda.stream(stream):
+        for _ in range(3):
+            cdst.copy_(cbuf)
+        ev0.record(stream)
+        for _ in range(20):
+            cdst.copy_(cbuf)
+        ev1.record(stream)
+    ev1.synchronize()
+    t_copy = ev0.elapsed_time(ev1) / 1e3 / 20
+    copy_gbs = 2 * cbuf.numel() * 8 / t_copy / 1e9
+    del cbuf, cdst
+    # FP32 J apply (inner solver's operator).
+    x32 = x.float()
+    y32 = torch.empty_like(x32)
+    with torch.cuda.stream(stream):
+        J.apply_internal_fp32(x32.data_ptr(), y32.data_ptr(), stream.cuda_stream)
+        ev0.record(stream)
+        for _ in range(args.spmv_reps):
+            J.apply_internal_fp32(x32.data_ptr(), y32.data_ptr(), stream.cuda_stream)
+        ev1.record(stream)
+    ev1.synchronize()
+    t_j32 = ev0.elapsed_time(ev1) / 1e3 / args.spmv_reps  # includes the block conversion
+    traffic, traffic_src = _profiled_traffic(nc)
+    roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": traffic_src,
+                "kernel": "k_block_apply<double> (J.phi FP64)",
+                "us_per_launch": round(t_j * 1e6, 2), "bytes_per_launch": bytes_canon,
+                "cold_us_per_launch": round(t_cold * 1e6, 2),
+                "cold_achieved": round(bytes_canon / t_cold / 1e9, 1),
+                "cold_frac": round(bytes_canon / t_cold / 1e9 / HBM_PEAK_GBS, 4),
+                "copy_kernel_GBs": round(copy_gbs, 1),
+                "layout_bytes_per_launch": bytes_layout,
+                "layout_note": "sliced ELLPACK (64-row slices) of the off-diagonal entries: "
+                               "16 B value + 4 B column per padded slot, 4 B per slice, "
+                               "24 B 2x2 block + 16 B x + 16 B y per vertex",
+                "nnz_pad": layout["nnz_pad"]}
+    log(args, f"J.phi {t_j*1e6:.1f} us -> {ach:.0f} GB/s ({ach/HBM_PEAK_GBS:.1%}); "
+              f"copy {copy_gbs:.0f} GB/s; fp32 J.phi {t_j32*1e6:.1f} us")
+    if args.roofline_only:
+        print(json.dumps({"roofline": roofline, "jphi_fp32_us": t_j32 * 1e6}))
+        return
+
+    # ---- solves.
+    psi_dev0 = torch.from_numpy(psi0).to(dev)
+    psi = torch.empty_like(psi_dev0)
+    results = {}
+    sols = {}
+    for var in [v for v in args.variants.split(",") if v]:
+        kw = dict(tol=1e-10, max_iters=40, minres_rtol=1e-10,
+                  minres_max_iters=args.minres_maxit, cheb_iters=args.cheb,
+                  cheb_ratio=args.ratio)
+        if var == "mixed":
+            kw.update(precision=1, forcing=1, forcing_min=1e-6, forcing_max=0.1)
+        opts = nosh.default_newton_options(**kw)
+        stats = None
+        for w in range(args.warmup):
+            psi.copy_(psi_dev0)
+            torch.cuda.synchronize()
+            stats = nosh.newton_solve_device(model, psi.data_ptr(), params, opts,
+                                             stream.cuda_stream)
+            log(args, var, "warmup", stats["newton_iters"], stats["minres_iters_total"],
+                f"{stats['seconds']:.3f}s")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        all_stats = []
+        for k in range(args.steps):
+            with torch.cuda.stream(stream):
+                psi.copy_(psi_dev0)
+            stats = nosh.newton_solve_device(model, psi.data_ptr(), params, opts,
+                                             stream.cuda_stream)
+            all_stats.append(stats)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        ok = all(s["converged"] and s["final_norm_f"] <= 1e-10 for s in all_stats)
+        sols[var] = psi.cpu().numpy()
+        results[var] = {"solves_per_s": args.steps / dt, "s_per_solve": dt / args.steps,
+                        "converged": ok, "final_norm_f": all_stats[-1]["final_norm_f"],
+                        "newton_iters": all_stats[-1]["newton_iters"],
+                        "minres_iters": all_stats[-1]["minres_iters_total"],
+                        "minres_history": all_stats[-1]["minres_history"]}
+        log(args, var, results[var])
+    if "plain" in sols:
+        ref = sols["plain"]
+        for var, s in sols.items():
+            d = np.linalg.norm(s - ref)
+            results[var]["abs_diff_vs_fp64"] = float(d)
+            results[var]["rel_diff_vs_fp64"] = float(d / max(np.linalg.norm(ref), 1e-300))
+            cvz = s[0::2] ** 2 + s[1::2] ** 2
+            results[var]["mean_abs_psi2"] = float(cvz.mean())
+
+    best = max((v for v in results if results[v]["converged"]),
+               key=lambda v: results[v]["solves_per_s"], default=None)
+    value = results[best]["solves_per_s"] if best else 0.0
+    cpu = None
+    if not args.no_cpu and "plain" in results:
+        cpu = cpu_baseline(coords, cells, A, mu, psi0, phi,
+                           {"newton_iters": results["plain"]["newton_iters"],
+                            "minres_iters_total": results["plain"]["minres_iters"]},
+                           args.cheb, args.ratio)
+    out = {
+        "metric": "Newton-MINRES solves/sec at ||F||<=1e-10 vs plain FP64 GPU; SpMV HBM GB/s",
+        "value": round(value, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 / value, 2) if value else None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if best == "plain" else "f64/f32 (FP32 inner MINRES, FP64 Newton)",
+        "data": "synthetic (seeded SplitMix64 psi_0; structured Kuhn mesh)",
+        "config": {"workload": f"Kuhn-tet cube {nc}^3 Newton-MINRES from seeded random psi",
+                   "n_vertices": n, "n_edges": E, "n_tets": int(cells.shape[0]), "mu": mu,
+                   "g": g, "V": -1.0, "preconditioner": f"Chebyshev-Jacobi({args.cheb}, "
+                   f"ratio {args.ratio})", "best_variant": best},
+        "variants": results,
+        "speedup_vs_plain_fp64": (round(value / results["plain"]["solves_per_s"], 3)
+                                  if "plain" in results and value else None),
+        "roofline": roofline,
+        "jphi_fp32_us": round(t_j32 * 1e6, 2),
+        "mesh_build_s": round(t_mesh, 3),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -87,6 +87,9 @@ int nosh_mesh_edge_data(nosh_mesh_t mesh, double* length, double* covolume);
 int nosh_mesh_control_volumes(nosh_mesh_t mesh, double* control_volumes);
 /* Internal vertex order: perm[i] = reference vertex stored at internal slot i. */
 int nosh_mesh_permutation(nosh_mesh_t mesh, int32_t* perm);
+/* Internal sparse layout: padded sliced-ELLPACK slots, slices of 64 rows,
+ * widest row. */
+int nosh_mesh_layout(nosh_mesh_t mesh, int64_t* nnz_pad, int64_t* nslices, int* max_width);
 /* Device vectors between reference order and the internal order (stream
  * may be NULL). Callers that keep state on the device in internal order use
  * the *_apply_internal entry points and skip the permutation per apply. */
@@ -164,7 +167,7 @@ int nosh_jac_destroy(nosh_jac_t jac);
  * scaling here: `iters` iterations (iters-1 products with P) on the
  * interval [upper/ratio, upper] of D^{-1}P; upper <= 0 selects the bound
  * 1 + max_k sum_{j!=k}|K_kj| / lambda_min(D_k) (<= 2 for nonnegative edge
- * weights). Defaults: iters 4, ratio 30, upper 0. */
+ * weights). Defaults: iters 6, ratio 300, upper 0. */
 int nosh_prec_create(nosh_mesh_t mesh, nosh_sfield_t thickness, nosh_vfield_t mvp,
                      nosh_prec_t* out);
 int nosh_prec_set_chebyshev(nosh_prec_t prec, int iters, double ratio, double upper);
@@ -219,8 +222,8 @@ typedef struct {
   int max_iters;        /* Newton steps (default 30) */
   double minres_rtol;   /* MINRES relative tolerance in the M^{-1} norm (1e-10) */
   int minres_max_iters; /* (default 1000) */
-  int cheb_iters;       /* Chebyshev iterations of the preconditioner, 0 = none (4) */
-  double cheb_ratio;    /* interval ratio (30) */
+  int cheb_iters;       /* Chebyshev iterations of the preconditioner, 0 = none (6) */
+  double cheb_ratio;    /* interval ratio (300) */
   int precision;        /* 0 = FP64 throughout; 1 = FP32 inner MINRES / FP64 outer */
   int forcing;          /* 0 = constant minres_rtol; 1 = Eisenstat-Walker choice 2 */
   double forcing_min;   /* lower bound of the forcing term (mixed: 1e-6) */

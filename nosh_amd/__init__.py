@@ -108,6 +108,7 @@ def _load():
         "nosh_mesh_edge_data": (i32, [vp, vp, vp]),
         "nosh_mesh_control_volumes": (i32, [vp, vp]),
         "nosh_mesh_permutation": (i32, [vp, vp]),
+        "nosh_mesh_layout": (i32, [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i32)]),
         "nosh_mesh_to_internal": (i32, [vp, vp, vp, vp]),
         "nosh_mesh_to_reference": (i32, [vp, vp, vp, vp]),
         "nosh_vfield_explicit_values_create": (i32, [vp, vp, d, pvp]),
@@ -262,6 +263,20 @@ class Mesh(_Handle):
         out = np.empty(self.num_vertices)
         _check(_lib.nosh_mesh_control_volumes(self._h, _ptr(out)))
         return out
+
+    def layout(self) -> dict:
+        """Internal sliced-ELLPACK layout sizes."""
+        nnz, ns, w = C.c_int64(), C.c_int64(), C.c_int()
+        _check(_lib.nosh_mesh_layout(self._h, C.byref(nnz), C.byref(ns), C.byref(w)))
+        return {"nnz_pad": nnz.value, "nslices": ns.value, "max_width": w.value}
+
+    def to_internal(self, x_ref_ptr: int, x_int_ptr: int, stream: int = 0):
+        _check(_lib.nosh_mesh_to_internal(self._h, C.c_void_p(x_ref_ptr), C.c_void_p(x_int_ptr),
+                                          C.c_void_p(stream)))
+
+    def to_reference(self, x_int_ptr: int, x_ref_ptr: int, stream: int = 0):
+        _check(_lib.nosh_mesh_to_reference(self._h, C.c_void_p(x_int_ptr), C.c_void_p(x_ref_ptr),
+                                           C.c_void_p(stream)))
 
     def permutation(self) -> np.ndarray:
         out = np.empty(self.num_vertices, dtype=np.int32)
@@ -530,3 +545,14 @@ def minres_solve(jac: JacobianOperator, prec: Optional[KeoRegularized], b, rtol:
     _check(_lib.nosh_minres_solve(jac._h, None if prec is None else prec._h, _ptr(b), _ptr(x),
                                   rtol, max_iters, C.byref(it), C.byref(rr)))
     return x, it.value, rr.value
+
+
+def newton_solve_device(model: Nls, psi_ptr: int, params: Dict[str, float],
+                        options: Optional[NewtonOptions] = None, stream: int = 0, **kw) -> dict:
+    """Newton-MINRES on a device vector (2n doubles, reference order) in place."""
+    o = options if options is not None else default_newton_options(**kw)
+    st = NewtonStats()
+    p = _Params(params)
+    _check(_lib.nosh_newton_solve_device(model._h, C.c_void_p(psi_ptr), *p.args, C.byref(o),
+                                         C.byref(st), C.c_void_p(stream)))
+    return st.as_dict()

@@ -130,8 +130,8 @@ struct nosh_prec_s {
   nosh_sfield_s* thickness;
   std::unique_ptr<nosh_keo_s> keo;
   DBuf<double> ct, blk, psi, bound;
-  int iters = 4;
-  double ratio = 30.0, upper = 0.0;
+  int iters = 6;
+  double ratio = 300.0, upper = 0.0;
   bool built = false;
   MinresWork<double> work;
   Staging st;
@@ -357,6 +357,15 @@ int nosh_mesh_permutation(nosh_mesh_t mesh, int32_t* perm) {
     NOSH_CHECK(mesh && perm, kInvalid, "null argument");
     mesh->m->perm.download(perm, mesh->m->n, default_stream());
     NOSH_HIP(hipStreamSynchronize(default_stream()));
+  });
+}
+
+int nosh_mesh_layout(nosh_mesh_t mesh, int64_t* nnz_pad, int64_t* nslices, int* max_width) {
+  return guard([&] {
+    NOSH_CHECK(mesh, kInvalid, "null mesh");
+    if (nnz_pad) *nnz_pad = mesh->m->nnz_pad;
+    if (nslices) *nslices = mesh->m->nslices;
+    if (max_width) *max_width = mesh->m->max_width;
   });
 }
 
@@ -874,8 +883,8 @@ int nosh_newton_default_options(nosh_newton_options* o) {
     o->max_iters = 30;
     o->minres_rtol = 1e-10;
     o->minres_max_iters = 1000;
-    o->cheb_iters = 4;
-    o->cheb_ratio = 30.0;
+    o->cheb_iters = 6;
+    o->cheb_ratio = 300.0;
     o->precision = 0;
     o->forcing = 0;
     o->forcing_min = 1e-6;

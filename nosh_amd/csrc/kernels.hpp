@@ -80,43 +80,35 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
   __syncthreads();
 }
 
-// Grid-wide deterministic sums: every block stores its partial sums, the
-// last block to finish adds all partials in block order and returns true
-// (totals in wave 0). partials: NV * gridDim.x doubles; counter starts at 0
-// and is reset by the last block.
+// Phase one of a grid-wide deterministic sum: each block stores its NV
+// partial sums at partials[q * gridDim.x + blockIdx.x]. Phase two is a
+// separate one-block kernel (finalize_sum) that adds them in block order;
+// the kernel boundary makes the partials visible across XCDs without a
+// per-block device-scope fence (which on gfx950 writes back the L2).
 template <int NV>
-__device__ __forceinline__ bool grid_sum(double (&v)[NV], double* __restrict__ partials,
-                                         unsigned* __restrict__ counter) {
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partials) {
   __shared__ double lds[16 * NV];
-  __shared__ int last;
   block_sum<NV>(v, lds);
-  const unsigned nb = gridDim.x;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) partials[q * nb + blockIdx.x] = v[q];
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    last = (prev == nb - 1) ? 1 : 0;
+    for (int q = 0; q < NV; ++q) partials[q * gridDim.x + blockIdx.x] = v[q];
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
+}
+
+// Sums nb partials per value in fixed order; totals valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void finalize_sum(double (&v)[NV], const double* __restrict__ partials,
+                                             int nb) {
+  __shared__ double lds[16 * NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
     double s = 0.0;
-    for (unsigned i = threadIdx.x; i < nb; i += blockDim.x)
-      s += __builtin_nontemporal_load(partials + q * nb + i);
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partials[q * nb + i];
     v[q] = s;
   }
   block_sum<NV>(v, lds);
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
 }
 
-// Workspace for grid reductions (device memory).
-struct RedWs {
-  double* partials = nullptr;  // >= 4 * max grid
-  unsigned* counter = nullptr; // one per concurrent reduction slot
-};
+constexpr int kFinalizeThreads = 256;
 
 }  // namespace nosh

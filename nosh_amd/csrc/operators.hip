@@ -133,8 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_compute_f(
     int64_t n, const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ scol,
     const double2* __restrict__ val, const double* __restrict__ diag,
     const double* __restrict__ ct, const double* __restrict__ V, double g,
-    const double2* __restrict__ x, double2* __restrict__ f, double* __restrict__ nrm2,
-    double* __restrict__ partials, unsigned* __restrict__ counter) {
+    const double2* __restrict__ x, double2* __restrict__ f, double* __restrict__ partials) {
   const int64_t row = xcd_block(true) * kBlock + threadIdx.x;
   double s2[1] = {0.0};
   if (row < n) {
@@ -146,7 +145,14 @@ __global__ __launch_bounds__(kBlock) void k_compute_f(
     f[row] = r;
     s2[0] = r.x * r.x + r.y * r.y;
   }
-  if (nrm2 && grid_sum<1>(s2, partials, counter) && threadIdx.x == 0) *nrm2 = s2[0];
+  if (partials) block_partials<1>(s2, partials);
+}
+
+__global__ void k_finalize_store(const double* __restrict__ partials, int nb,
+                                 double* __restrict__ out) {
+  double v[1];
+  finalize_sum<1>(v, partials, nb);
+  if (threadIdx.x == 0) *out = v[0];
 }
 
 // dF/dp column: dK x + ct |x|^2 x (parameter "g") or ct dV/dp x.
@@ -205,9 +211,15 @@ __global__ __launch_bounds__(kBlock) void k_rebuild_blocks(
     const double lmin = 0.5 * (a0 + a1) - hypot(0.5 * (a0 - a1), b);
     ratio = lmin > 0.0 ? rowabs[row] / lmin : 1.0;
   }
+  __shared__ double wmax[kBlock / 64];
   ratio = wave_max(ratio);
-  if ((threadIdx.x & 63) == 0 && bound)
-    atomicMax(bound, static_cast<unsigned long long>(__double_as_longlong(ratio)));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = ratio;
+  __syncthreads();
+  if (threadIdx.x == 0 && bound) {
+    double m = wmax[0];
+    for (int w = 1; w < kBlock / 64; ++w) m = fmax(m, wmax[w]);
+    atomicMax(bound, static_cast<unsigned long long>(__double_as_longlong(m)));
+  }
 }
 
 template <class T>
@@ -294,20 +306,12 @@ void launch_dfdp(const MeshCache& m, const KeoValues& kv, const double* ct, cons
 // Reduction scratch for k_compute_f (one per process; calls on the library
 // stream are serialised).
 namespace {
-struct FScratch {
-  DBuf<double> partials;
-  DBuf<unsigned> counter;
-};
-FScratch& fscratch(int64_t nblocks, hipStream_t s) {
+DBuf<double>& fscratch(int64_t nblocks) {
   // Never freed: device memory must not be released from static destructors
   // that may run after the HIP runtime has shut down.
-  static FScratch& fs = *new FScratch;
-  if (fs.partials.n < size_t(nblocks)) fs.partials.alloc(nblocks);
-  if (fs.counter.n == 0) {
-    fs.counter.alloc(1);
-    fs.counter.zero(s);
-  }
-  return fs;
+  static DBuf<double>& partials = *new DBuf<double>;
+  if (partials.n < size_t(nblocks)) partials.alloc(nblocks);
+  return partials;
 }
 }  // namespace
 
@@ -315,11 +319,11 @@ void launch_compute_f(const MeshCache& m, const KeoValues& kv, const double* ct,
                       const double* V, double g, const double* x, double* f, double* nrm2,
                       hipStream_t s) {
   const unsigned nb = grid_for(m.n, kBlock);
-  FScratch& fs = fscratch(nb, s);
+  double* partials = nrm2 ? fscratch(nb).p : nullptr;
   k_compute_f<<<nb, kBlock, 0, s>>>(m.n, m.slice_ptr.p, m.scol.p, kv.val.p, kv.diag.p, ct, V, g,
                                     reinterpret_cast<const double2*>(x),
-                                    reinterpret_cast<double2*>(f), nrm2, fs.partials.p,
-                                    fs.counter.p);
+                                    reinterpret_cast<double2*>(f), partials);
+  if (nrm2) k_finalize_store<<<1, kFinalizeThreads, 0, s>>>(partials, int(nb), nrm2);
 }
 
 void launch_rebuild_blocks(const MeshCache& m, const KeoValues& kv, const double* ct,

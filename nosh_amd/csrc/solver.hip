@@ -11,8 +11,8 @@
 //   B  r_new = u - (alpha/beta) r2 and the first Chebyshev step on r_new;
 //   C  the remaining Chebyshev steps, each one product with P; the last one
 //      also sums beta^2 = <r_new, z> and advances the Givens rotation.
-// Reductions are deterministic (partials per block, summed in block order by
-// the last block, which also evaluates the scalar recurrences). The host only
+// Reductions are deterministic: partial sums per block, then a one-block
+// kernel adds them in block order and evaluates the scalar recurrences. The host only
 // polls the `done` flag every few iterations, so iterations issued after
 // convergence are no-ops. Buffer rotations (r1/r2, w) are indexed by the
 // device iteration counter, so every iteration launches identical kernels.
@@ -124,7 +124,6 @@ struct ChebArgs {
   const V2<T>* u;      // mode 2: J v - (beta/oldb) r1
   MinresScalars* S;
   double* partials;
-  unsigned* counter;
 };
 
 // Chebyshev step 0 (z = D^{-1} r / theta), producing r first in mode 2.
@@ -164,12 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb_first(ChebArgs<T> a, int mode, 
     s2[0] = double(rv.x) * double(z.x) + double(rv.y) * double(z.y);
   }
   if (mode == 0 || a.iters > 1) return;
-  if (grid_sum<1>(s2, a.partials, a.counter) && threadIdx.x == 0) {
-    if (mode == 1)
-      init_update(S, s2[0]);
-    else
-      givens_update(S, s2[0]);
-  }
+  block_partials<1>(s2, a.partials);
 }
 
 // Chebyshev step j >= 1: z_out = z_in + d, d = c1 d + c2 D^{-1}(r - P z_in).
@@ -205,11 +199,31 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(ChebArgs<T> a, int mode, i
     s2[0] = double(rv.x) * double(z.x) + double(rv.y) * double(z.y);
   }
   if (mode == 0 || !last) return;
-  if (grid_sum<1>(s2, a.partials, a.counter) && threadIdx.x == 0) {
+  block_partials<1>(s2, a.partials);
+}
+
+// beta^2 = sum of the partials of <r, z>: MINRES initialisation (mode 1) or
+// the Givens step of the current iteration (mode 2).
+__global__ void k_fin_beta(MinresScalars* S, const double* __restrict__ partials, int nb,
+                           int mode) {
+  if (mode == 2 && S->done) return;
+  double v[1];
+  finalize_sum<1>(v, partials, nb);
+  if (threadIdx.x == 0) {
     if (mode == 1)
-      init_update(S, s2[0]);
+      init_update(S, v[0]);
     else
-      givens_update(S, s2[0]);
+      givens_update(S, v[0]);
+  }
+}
+
+__global__ void k_fin_alfa(MinresScalars* S, const double* __restrict__ partials, int nb) {
+  if (S->done) return;
+  double v[1];
+  finalize_sum<1>(v, partials, nb);
+  if (threadIdx.x == 0) {
+    S->alfa = v[0];
+    S->pending = 0;
   }
 }
 
@@ -231,7 +245,6 @@ struct MinresArgs {
   double2* x;
   MinresScalars* S;
   double* partials;
-  unsigned* counter;
 };
 
 template <class T>
@@ -291,10 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_minres_a(MinresArgs<T> a) {
     a.u[row] = uu;
     s[0] = double(vn.x) * double(uu.x) + double(vn.y) * double(uu.y);
   }
-  if (grid_sum<1>(s, a.partials, a.counter) && threadIdx.x == 0) {
-    S->alfa = s[0];
-    S->pending = 0;
-  }
+  block_partials<1>(s, a.partials);
 }
 
 // Applies the last pending w/x update, then psi += x when psi != nullptr.
@@ -358,8 +368,6 @@ void MinresWork<T>::ensure(int64_t n_) {
   x.alloc(n);
   const int64_t nb = grid_for(n, kBlock);
   partials.alloc(4 * nb + 64);
-  counter.alloc(4);
-  counter.zero(default_stream());
   if (!S.p) S.alloc(1);
   if (!Sh) NOSH_HIP(hipHostMalloc(reinterpret_cast<void**>(&Sh), sizeof(MinresScalars)));
   NOSH_HIP(hipStreamSynchronize(default_stream()));
@@ -389,7 +397,6 @@ static ChebArgs<T> cheb_args(const MeshCache& m, const PrecView<T>& P, MinresWor
   a.u = w.u.p;
   a.S = w.S.p;
   a.partials = w.partials.p;
-  a.counter = w.counter.p;
   return a;
 }
 
@@ -406,6 +413,7 @@ static void cheb_chain(const MeshCache& m, ChebArgs<T> a, int mode, V2<T>* out, 
     k_cheb_step<T><<<nb, kBlock, 0, s>>>(a, mode, j, zb[cur], zb[1 - cur]);
     cur = 1 - cur;
   }
+  if (mode != 0) k_fin_beta<<<1, kFinalizeThreads, 0, s>>>(a.S, a.partials, int(nb), mode);
 }
 
 template <class T>
@@ -444,11 +452,11 @@ int minres_solve(const MeshCache& m, const JacView<T>& J, const PrecView<T>& P,
   ma.x = w.x.p;
   ma.S = w.S.p;
   ma.partials = w.partials.p;
-  ma.counter = w.counter.p;
   int issued = 0;
   for (;;) {
     for (int q = 0; q < poll && issued < maxit; ++q, ++issued) {
       k_minres_a<T><<<nb, kBlock, 0, s>>>(ma);
+      k_fin_alfa<<<1, kFinalizeThreads, 0, s>>>(w.S.p, w.partials.p, int(nb));
       cheb_chain<T>(m, ca, 2, w.Y.p, w.Z2.p, s);
     }
     NOSH_HIP(hipMemcpyAsync(w.Sh, w.S.p, sizeof(MinresScalars), hipMemcpyDeviceToHost, s));

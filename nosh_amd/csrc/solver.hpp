@@ -20,7 +20,6 @@ struct MinresWork {
   DBuf<V2<T>> Y, Z2, v, u, R0, R1, D, W0, W1, W2;
   DBuf<double2> x;
   DBuf<double> partials;
-  DBuf<unsigned> counter;
   DBuf<MinresScalars> S;
   MinresScalars* Sh = nullptr;  // pinned host mirror
   void ensure(int64_t n);

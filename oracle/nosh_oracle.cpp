@@ -526,6 +526,7 @@ void orc_jac_apply(void* p, const double* x, double* y) {
   auto* md = static_cast<Model*>(p);
   const int64_t n = md->mesh->n;
   md->K.apply(x, y);
+#pragma omp parallel for schedule(static)
   for (int64_t k = 0; k < n; ++k) {
     y[2 * k] += md->diag0[2 * k] * x[2 * k] + md->diag1b[k] * x[2 * k + 1];
     y[2 * k + 1] += md->diag1b[k] * x[2 * k] + md->diag0[2 * k + 1] * x[2 * k + 1];
@@ -611,6 +612,7 @@ void cheb_apply(const Model& md, const std::vector<double>& blk, int iters, doub
     *o0 = (c * x0 - b * x1) / det;
     *o1 = (a * x1 - b * x0) / det;
   };
+#pragma omp parallel for schedule(static)
   for (int64_t k = 0; k < n; ++k) {
     double u0, u1;
     dinv(k, r[2 * k], r[2 * k + 1], &u0, &u1);
@@ -623,6 +625,7 @@ void cheb_apply(const Model& md, const std::vector<double>& blk, int iters, doub
     md.P.apply(z, res.data());
     const double rho_new = 1.0 / (2.0 * sigma - rho);
     const double c1 = rho_new * rho, c2 = 2.0 * rho_new / delta;
+#pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < n; ++k) {
       double u0, u1;
       dinv(k, r[2 * k] - res[2 * k], r[2 * k + 1] - res[2 * k + 1], &u0, &u1);
@@ -642,6 +645,7 @@ struct PrecCfg {
 
 double ddot(int64_t m, const double* a, const double* b) {
   double s = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
   for (int64_t i = 0; i < m; ++i) s += a[i] * b[i];
   return s;
 }
@@ -668,13 +672,18 @@ int minres(int64_t m, Op op, Prec prec, const double* b, double* x, double rtol,
   while (it < maxit) {
     ++it;
     const double s = 1.0 / beta;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < m; ++i) v[i] = s * y[i];
     op(v.data(), y.data());
-    if (it >= 2)
+    if (it >= 2) {
+#pragma omp parallel for schedule(static)
       for (int64_t i = 0; i < m; ++i) y[i] -= (beta / oldb) * r1[i];
+    }
     const double alfa = ddot(m, v.data(), y.data());
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < m; ++i) y[i] -= (alfa / beta) * r2[i];
     r1.swap(r2);
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < m; ++i) r2[i] = y[i];
     prec(r2.data(), y.data());
     oldb = beta;
@@ -691,6 +700,7 @@ int minres(int64_t m, Op op, Prec prec, const double* b, double* x, double rtol,
     const double phi = cs * phibar;
     phibar = sn * phibar;
     const double denom = 1.0 / gamma;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < m; ++i) {
       const double wnew = (v[i] - oldeps * w1[i] - delta * w2[i]) * denom;
       w1[i] = w2[i];

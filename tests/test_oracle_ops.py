@@ -1,0 +1,294 @@
+"""Pins of the oracle's operators and solvers: K, F, dF/dp, J, P, Chebyshev,
+MINRES and Newton, each against an independent statement of what it must be.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from nosh_amd import meshgen as mg
+from oracle import oracle as O
+
+
+def cplx(x):
+    return x[0::2] + 1j * x[1::2]
+
+
+def real(z):
+    out = np.empty(2 * z.size)
+    out[0::2] = z.real
+    out[1::2] = z.imag
+    return out
+
+
+@pytest.fixture(scope="module")
+def sq():
+    coords, cells = mg.square_tri(8)
+    m = O.Mesh(coords, cells)
+    A = mg.magnetic_potential(coords)
+    return coords, cells, m, A
+
+
+def dense_keo_brute(m, coords, A, mu, thickness=None):
+    """K as a complex n x n matrix from the edge list, written independently:
+    K_ii = sum alpha, K_lo,hi = -alpha e^{-i a}, K_hi,lo = -alpha e^{+i a}."""
+    n = m.n
+    t = np.ones(n) if thickness is None else thickness
+    ln, cov = m.edge_data()
+    K = np.zeros((n, n), complex)
+    for (lo, hi), l, c in zip(m.edges(), ln, cov):
+        al = c / l * 0.5 * (t[lo] + t[hi])
+        a = mu * np.dot(0.5 * (A[lo] + A[hi]), coords[lo] - coords[hi])
+        K[lo, lo] += al
+        K[hi, hi] += al
+        K[lo, hi] -= al * np.exp(-1j * a)
+        K[hi, lo] -= al * np.exp(1j * a)
+    return K
+
+
+def test_keo_dense_brute_force(sq):
+    coords, cells, m, A = sq
+    md = O.Model(m, A)
+    mu = 0.37
+    md.set_params(1.0, mu)
+    K = dense_keo_brute(m, coords, A, mu)
+    x = mg.random_state(m.n, 4)
+    np.testing.assert_allclose(cplx(md.keo(x)), K @ cplx(x), rtol=0, atol=1e-13)
+    # Hermitian, and the real 2n x 2n form is symmetric.
+    assert np.allclose(K, K.conj().T)
+    D = md.dense(0)
+    np.testing.assert_allclose(D, D.T, atol=1e-14)
+
+
+def test_keo_reference_identities(sq):
+    """test/keo.cpp:78-113: with psi = 1 (x = [1,0,1,0,...]) x^T K x =
+    sum_e 2 alpha (1 - cos a); the all-ones R^{2n} vector gives twice that;
+    the Hermitian cross term [0,1,...]^T K [1,0,...] vanishes; K 1 = 0 at mu = 0."""
+    coords, cells, m, A = sq
+    md = O.Model(m, A)
+    mu = 0.2
+    md.set_params(1.0, mu)
+    at, al = md.edge_fields()
+    u = np.zeros(2 * m.n)
+    u[0::2] = 1.0
+    v = np.zeros(2 * m.n)
+    v[1::2] = 1.0
+    expect = np.sum(2 * al * (1 - np.cos(mu * at)))
+    assert abs(u @ md.keo(u) - expect) < 1e-12
+    ones = np.ones(2 * m.n)
+    assert abs(ones @ md.keo(ones) - 2 * expect) < 1e-12
+    assert abs(v @ md.keo(u)) < 1e-13
+    md.set_params(1.0, 0.0)
+    assert np.max(np.abs(md.keo(u))) < 1e-14
+
+
+def test_flux_through_triangles(sq):
+    """Midpoint rule is exact for the linear A = (-y, x)/2, so the oriented
+    sum of a_e around every counter-clockwise triangle equals mu * B * |T|
+    with B = curl A = 1 (a_e = -mu int_lo^hi A.dl)."""
+    coords, cells, m, A = sq
+    md = O.Model(m, A)
+    at, _ = md.edge_fields()
+    mu = 0.3
+    e = m.edges()
+    idx = {(a, b): k for k, (a, b) in enumerate(e)}
+    for c in cells:
+        s = 0.0
+        for j in range(3):
+            p, q = c[j], c[(j + 1) % 3]
+            k = idx[(min(p, q), max(p, q))]
+            s += -mu * at[k] if p < q else mu * at[k]
+        x = coords[c]
+        area = 0.5 * ((x[1, 0] - x[0, 0]) * (x[2, 1] - x[0, 1]) -
+                      (x[1, 1] - x[0, 1]) * (x[2, 0] - x[0, 0]))
+        assert area > 0
+        assert abs(s - mu * area) < 1e-13
+
+
+def test_gauge_covariance():
+    """A = grad(chi) with chi = x: K_A = U K_0 U^H, U = diag(exp(-i mu chi)),
+    and F_A(U psi) = U F_0(psi)."""
+    coords, cells = mg.kuhn_cube(3)
+    m = O.Mesh(coords, cells)
+    Agrad = np.zeros_like(coords)
+    Agrad[:, 0] = 1.0
+    mu = 0.7
+    mA = O.Model(m, Agrad)
+    mA.set_params(1.0, mu)
+    m0 = O.Model(m, np.zeros_like(coords))
+    m0.set_params(1.0, mu)
+    U = np.exp(-1j * mu * coords[:, 0])
+    psi = mg.random_state(m.n, 2)
+    z = cplx(psi)
+    np.testing.assert_allclose(cplx(mA.keo(real(U * z))), U * cplx(m0.keo(psi)), atol=1e-12)
+    np.testing.assert_allclose(cplx(mA.f(real(U * z))), U * cplx(m0.f(psi)), atol=1e-12)
+
+
+def _fd5(fun, h):
+    """Five-point central difference: exact for polynomials of degree <= 4."""
+    return (8 * (fun(h) - fun(-h)) - (fun(2 * h) - fun(-2 * h))) / (12 * h)
+
+
+@pytest.mark.parametrize("gen", [lambda: mg.square_tri(8), lambda: mg.kuhn_cube(3, jitter=0.2)])
+def test_jacobian_is_derivative_of_f(gen):
+    """F is a cubic polynomial in psi, so the 5-point stencil gives J phi
+    up to rounding; J is symmetric; P = J - c t V; <i psi, F(psi)> = 0."""
+    coords, cells = gen()
+    m = O.Mesh(coords, cells)
+    md = O.Model(m, mg.magnetic_potential(coords))
+    md.set_params(1.0, 0.4)
+    psi = mg.random_state(m.n, 0)
+    phi = mg.random_state(m.n, 1)
+    md.jac_rebuild(psi)
+    jp = md.jac(phi)
+    fd = _fd5(lambda h: md.f(psi + h * phi), 1e-2)
+    assert rel_err(jp, fd) < 1e-12
+    u = mg.random_state(m.n, 5)
+    assert abs(u @ md.jac(phi) - phi @ md.jac(u)) < 1e-12 * np.linalg.norm(u) * np.linalg.norm(jp)
+    md.prec_rebuild(psi)
+    cv = m.control_volumes()
+    cvv = np.repeat(cv, 2)
+    assert rel_err(md.prec_op(phi), jp + cvv * phi) < 1e-14  # V = -1, t = 1
+    f = md.f(psi)
+    gauge = np.sum(psi[0::2] * f[1::2] - psi[1::2] * f[0::2])
+    assert abs(gauge) < 1e-12 * np.linalg.norm(psi) * np.linalg.norm(f)
+
+
+def test_dfdp_columns():
+    """dF/dmu against a 5-point stencil in mu (deriv_parameter "mu"); dF/dg
+    exact (F is linear in g) with deriv_parameter "g"; and the reference's
+    column-"g" quirk: with deriv_parameter "mu" the g column also carries
+    dK/dmu psi (src/model_evaluator_nls.cpp:640-641,665-674)."""
+    coords, cells = mg.square_tri(8)
+    m = O.Mesh(coords, cells)
+    A = mg.magnetic_potential(coords)
+    psi = mg.random_state(m.n, 0)
+    mu, g = 0.4, 1.3
+
+    def f_at(gg, mm):
+        md = O.Model(m, A)
+        md.set_params(gg, mm)
+        return md.f(psi)
+
+    md = O.Model(m, A, deriv_param="mu")
+    md.set_params(g, mu)
+    d_mu = md.dfdp(psi, "mu")
+    assert rel_err(d_mu, _fd5(lambda h: f_at(g, mu + h), 1e-3)) < 1e-9
+    dk = md.dkeo(psi)
+    md_g = O.Model(m, A, deriv_param="g")
+    md_g.set_params(g, mu)
+    d_g = md_g.dfdp(psi, "g")
+    assert rel_err(d_g, f_at(g + 1.0, mu) - f_at(g, mu)) < 1e-12
+    assert rel_err(md.dfdp(psi, "g"), d_g + dk) < 1e-12
+
+
+def test_reference_jac_identities():
+    """test/jac.cpp structure at psi = 1, V = -1, g = 1: for s = [1,0,...],
+    s^T J s = s^T K s + 2 sum(cv); for s = [0,1,...], s^T J s = s^T K s."""
+    coords, cells = mg.square_tri(8)
+    m = O.Mesh(coords, cells)
+    md = O.Model(m, mg.magnetic_potential(coords))
+    md.set_params(1.0, 0.01)
+    psi = np.zeros(2 * m.n)
+    psi[0::2] = 1.0
+    md.jac_rebuild(psi)
+    cv = m.control_volumes()
+    s0 = psi.copy()
+    s1 = np.zeros(2 * m.n)
+    s1[1::2] = 1.0
+    assert abs(s0 @ md.jac(s0) - (s0 @ md.keo(s0) + 2 * cv.sum())) < 1e-11
+    assert abs(s1 @ md.jac(s1) - s1 @ md.keo(s1)) < 1e-11
+    # With psi = 1, F = K psi (test/compute_f.cpp: V = -1 cancels g |psi|^2).
+    assert rel_err(md.f(psi), md.keo(psi)) < 1e-14
+
+
+def test_chebyshev_matches_polynomial():
+    """z = [I - R_k(D^-1 P)] P^-1 r, R_k the Chebyshev residual polynomial
+    T_k((theta - lambda)/delta) / T_k(theta/delta), via the eigendecomposition
+    of D^-1/2 P D^-1/2 (D = 2x2 block diagonal of P)."""
+    coords, cells = mg.square_tri(4)
+    m = O.Mesh(coords, cells)
+    md = O.Model(m, mg.magnetic_potential(coords))
+    md.set_params(1.0, 0.3)
+    psi = mg.random_state(m.n, 0)
+    md.prec_rebuild(psi)
+    P = md.dense(2)
+    n2 = P.shape[0]
+    D = np.zeros_like(P)
+    for k in range(m.n):
+        D[2 * k:2 * k + 2, 2 * k:2 * k + 2] = P[2 * k:2 * k + 2, 2 * k:2 * k + 2]
+    w, Q = np.linalg.eigh(D)
+    Dm12 = Q @ np.diag(w ** -0.5) @ Q.T
+    S = Dm12 @ P @ Dm12
+    lam, V = np.linalg.eigh(S)
+    hi = md.cheb_bound()
+    assert lam.max() <= hi * (1 + 1e-12)  # the bound is an upper bound
+    r = mg.random_state(m.n, 3)
+    for iters, ratio in ((1, 30.0), (3, 30.0), (6, 300.0)):
+        lo = hi / ratio
+        theta, delta = (hi + lo) / 2, (hi - lo) / 2
+        Tk = np.polynomial.chebyshev.Chebyshev.basis(iters)
+        R = Tk((theta - lam) / delta) / Tk(theta / delta)
+        pol = V @ np.diag((1 - R) / lam) @ V.T
+        expect = Dm12 @ pol @ Dm12 @ r
+        got = md.cheb(r, iters, ratio)
+        assert rel_err(got, expect) < 1e-10, iters
+        assert n2 == r.size
+
+
+def test_minres_poisson_against_dense_solve():
+    """MINRES on the (-1, 2, -1) matrix of test/linear-solve-test/linop-tester.cpp
+    against numpy's dense solve."""
+    n = 200
+    ptr = [0]
+    col, val = [], []
+    for i in range(n):
+        for j, v in ((i - 1, -1.0), (i, 2.0), (i + 1, -1.0)):
+            if 0 <= j < n:
+                col.append(j)
+                val.append(v)
+        ptr.append(len(col))
+    A = np.zeros((n, n))
+    for i in range(n):
+        for q in range(ptr[i], ptr[i + 1]):
+            A[i, col[q]] = val[q]
+    b = np.sin(np.arange(n) * 0.37) + 0.1
+    x, k, rr = O.minres_crs(np.array(ptr), np.array(col), np.array(val), b, rtol=1e-12,
+                            maxit=2000)
+    xd = np.linalg.solve(A, b)
+    assert rel_err(x, xd) < 1e-9
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-11
+    assert k <= n + 5  # exact arithmetic terminates in <= n steps
+
+
+def test_minres_on_jacobian_residual_estimate():
+    """The recurred residual (M^-1 norm) tracks the true one; with the identity
+    preconditioner it is the 2-norm."""
+    coords, cells = mg.square_tri(8)
+    m = O.Mesh(coords, cells)
+    md = O.Model(m, mg.magnetic_potential(coords))
+    md.set_params(1.0, 0.1)
+    psi = mg.random_state(m.n, 0)
+    md.jac_rebuild(psi)
+    md.prec_rebuild(psi)
+    b = -md.f(psi)
+    x, k, rr = md.minres(b, rtol=1e-8, maxit=2000, cheb_iters=0)
+    true = np.linalg.norm(b - md.jac(x)) / np.linalg.norm(b)
+    assert rr <= 1e-8 and abs(true - rr) < 1e-9
+    x2, k2, rr2 = md.minres(b, rtol=1e-8, maxit=2000, cheb_iters=4, ratio=30.0)
+    assert k2 < k and np.linalg.norm(b - md.jac(x2)) / np.linalg.norm(b) < 1e-6
+
+
+def test_newton_converges_quadratically():
+    coords, cells = mg.square_tri(16)
+    m = O.Mesh(coords, cells)
+    md = O.Model(m, mg.magnetic_potential(coords))
+    psi, it, nf, mh = md.newton(mg.random_state(m.n, 0), 1.0, 0.1, tol=1e-10, maxit=40,
+                                minres_rtol=1e-12, cheb_iters=4, ratio=30.0)
+    assert it > 0 and nf[-1] <= 1e-10
+    # Terminal phase: ||F_{k+1}|| <= C ||F_k||^2 over the last three steps
+    # (history ... 2.1e-3, 1.3e-5, 8.9e-9, 3.9e-15: C ~ 3..60).
+    tail = nf[-4:]
+    assert tail[0] < 1e-2
+    for a, b in zip(tail[:-1], tail[1:]):
+        assert b <= 100.0 * a * a
+    assert np.linalg.norm(md.f(psi)) <= 1e-10
