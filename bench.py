@@ -57,6 +57,12 @@ def parse():
                    help="only the J.phi timing loop and a copy kernel (for profilers)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--verbose", action="store_true")
+    p.add_argument("--workload", default="kuhn96", choices=["kuhn96", "batched"],
+                   help="kuhn96: the headline single-instance workload; batched: 128 uneven "
+                        "instances on jittered 16^3/24^3/32^3/48^3 cubes")
+    p.add_argument("--instances", type=int, default=128)
+    p.add_argument("--plain-sample", type=int, default=8,
+                   help="batched: instances solved one by one with plain FP64 for the ratio")
     return p.parse_args()
 
 
@@ -120,12 +126,111 @@ def cpu_baseline(coords, cells, A, mu, psi0, phi, gpu_stats, cheb, ratio):
             "sample": sample, "jphi_ms": t_j * 1e3, "seconds_per_solve": t_solve}
 
 
+def batched_workload(args):
+    """128 instances drawn round-robin over jittered Kuhn cubes 16^3, 24^3, 32^3,
+    48^3 (interior vertices moved by U(-0.2h, 0.2h), seed 2); mu ~ U(0,1),
+    g ~ U(0.5,2) from the SplitMix64 stream of seed 2 (two draws per
+    instance); psi_0 of instance i from seed 1000 + i; V = -1. One step = the
+    whole batch solved to ||F|| <= 1e-10 (FP32 inner MINRES, FP64 outer)."""
+    print("batched workload: importing torch", file=sys.stderr, flush=True)
+    import torch
+    import nosh_amd as nosh
+    from nosh_amd import meshgen as mg
+    print("batched workload: building meshes", file=sys.stderr, flush=True)
+    sizes = [16, 24, 32, 48]
+    meshes = [mg.kuhn_cube(s, jitter=0.2, seed=2) for s in sizes]
+    models = []
+    for c, t in meshes:
+        mesh = nosh.Mesh(c, t)
+        mvp = nosh.VectorFieldExplicitValues(mesh, mg.magnetic_potential(c), 0.0)
+        models.append(nosh.Nls(mesh, mvp, nosh.ScalarFieldConstant(mesh, -1.0), 1.0,
+                               nosh.ScalarFieldConstant(mesh, 1.0)))
+    ninst = args.instances
+    owner = [i % 4 for i in range(ninst)]
+    counts = [owner.count(k) for k in range(4)]
+    u = mg.splitmix64_uniform(2, 2 * ninst).reshape(ninst, 2)
+    # Group-major order: instance i sits in group owner[i].
+    order = sorted(range(ninst), key=lambda i: (owner[i], i))
+    mu = np.array([u[i, 0] for i in order])
+    g = np.array([0.5 + 1.5 * u[i, 1] for i in order])
+    psi0 = np.concatenate([mg.random_state(meshes[owner[i]][0].shape[0], 1000 + i)
+                           for i in order])
+    batch = nosh.Batch([(models[k], counts[k]) for k in range(4)], mu, g)
+    kw = dict(tol=1e-10, max_iters=80, minres_rtol=1e-10, minres_max_iters=4000,
+              cheb_iters=args.cheb, cheb_ratio=args.ratio, forcing=1, forcing_min=1e-6,
+              forcing_max=0.1)
+    def say(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    tuned = batch.autotune(psi0, use_cache=False, **kw)
+    t_tune = time.perf_counter() - t0
+    say(f"autotune {t_tune:.2f}s -> block_threads {tuned[0]}, mode {tuned[1]}")
+    dev = torch.device("cuda:0")
+    x0 = torch.from_numpy(psi0).to(dev)
+    x = torch.empty_like(x0)
+    for w in range(args.warmup):
+        x.copy_(x0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        st = batch.solve_device(x.data_ptr(), **kw)
+        say(f"warmup {w}: {time.perf_counter() - t1:.2f}s, converged "
+            f"{sum(s['converged'] for s in st)}/{ninst}, newton max "
+            f"{max(s['newton_iters'] for s in st)}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        x.copy_(x0)
+        st = batch.solve_device(x.data_ptr(), **kw)
+        say(f"step {k}: {time.perf_counter() - t0:.2f}s cumulative")
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    conv = sum(s["converged"] for s in st)
+    # Plain FP64 single-instance solves of a sample, same options otherwise.
+    sample = [order.index(i) for i in range(min(args.plain_sample, ninst))]
+    t_plain, worst = 0.0, 0.0
+    for j in sample:
+        k = owner[order[j]]
+        p0 = batch.split(psi0)[j]
+        t1 = time.perf_counter()
+        p, s = nosh.newton_solve(models[k], p0, {"g": g[j], "mu": mu[j]}, tol=1e-10,
+                                 max_iters=80, minres_rtol=1e-10, minres_max_iters=4000,
+                                 cheb_iters=args.cheb, cheb_ratio=args.ratio)
+        t_plain += time.perf_counter() - t1
+        worst = max(worst, s["final_norm_f"])
+        say(f"plain instance {j} (mesh {sizes[k]}^3): {time.perf_counter() - t1:.2f}s, "
+            f"newton {s['newton_iters']}, minres {s['minres_iters_total']}")
+    plain_rate = len(sample) / t_plain
+    rate = ninst / dt
+    out = {"metric": "batched Newton-MINRES solves/sec at ||F||<=1e-10 (128 uneven instances)",
+           "value": round(rate, 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64/f32 (FP32 inner MINRES, FP64 Newton)",
+           "data": "synthetic (jittered Kuhn meshes, seeded psi_0, mu, g)",
+           "config": {"workload": "batched uneven solves", "instances": ninst,
+                      "meshes": [f"{s}^3 jittered" for s in sizes], "counts": counts,
+                      "autotuned": {"block_threads": tuned[0],
+                                    "mode": ["hipGraph", "persistent"][tuned[1]]},
+                      "autotune_s": round(t_tune, 2)},
+           "converged": conv,
+           "newton_iters_max": max(s["newton_iters"] for s in st),
+           "minres_iters_max": max(s["minres_iters"] for s in st),
+           "plain_fp64_sequential_solves_per_s": round(plain_rate, 3),
+           "plain_sample": len(sample), "plain_worst_norm_f": worst,
+           "speedup_vs_plain_fp64": round(rate / plain_rate, 2)}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is design-only in this build.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        batched_workload(args)
+        return
     import torch
     import nosh_amd as nosh
     from nosh_amd import meshgen as mg

@@ -256,6 +256,152 @@ int nosh_newton_solve_device(nosh_nls_t nls, double* psi, int nparams,
 int nosh_minres_solve(nosh_jac_t jac, nosh_prec_t prec, const double* b, double* x,
                       double rtol, int max_iters, int* iters, double* relres);
 
+/* ---------------------------------------------------------- batches ------
+ * No counterpart in the reference (which solves one instance per process):
+ * many Newton-MINRES solves on meshes of uneven size advanced in lockstep,
+ * one kernel launch per phase for all of them. Group k is the model
+ * models[k] (mesh, A, V, thickness) with counts[k] (1..64) instances; mu and
+ * g hold one value per instance, group-major. The inner MINRES is FP32
+ * (vectors, blocks, on-the-fly KEO values) with FP64 scalars, F and update. */
+typedef struct nosh_batch_s* nosh_batch_t;
+
+typedef struct {
+  int newton_iters;
+  int minres_iters;
+  int converged;
+  double final_norm_f;
+} nosh_batch_instance_stats;
+
+int nosh_batch_create(int ngroups, const nosh_nls_t* models, const int* counts, const double* mu,
+                      const double* g, nosh_batch_t* out);
+/* block_threads: threads per block (multiple of 64 and of the lane count);
+ * mode 0: each MINRES iteration is a captured hipGraph replayed graph_iters
+ * at a time, mode 1: one persistent cooperative kernel per MINRES solve;
+ * poll: iterations between host checks of the done flags (mode 0). */
+int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int mode, int graph_iters,
+                          int poll);
+/* Times every (block_threads, mode) candidate on one Newton step from psi and
+ * keeps the fastest; with use_cache the choice is read from / appended to
+ * $NOSH_AUTOTUNE_CACHE (default ~/.cache/nosh_amd_autotune.tsv). */
+int nosh_batch_autotune(nosh_batch_t batch, const double* psi, const nosh_newton_options* opts,
+                        int use_cache, int* best_threads, int* best_mode);
+/* psi: all instances concatenated (each 2 n_k values, reference order);
+ * overwritten by the solutions. stats: one record per instance (may be NULL). */
+int nosh_batch_solve(nosh_batch_t batch, double* psi, const nosh_newton_options* opts,
+                     nosh_batch_instance_stats* stats);
+int nosh_batch_solve_device(nosh_batch_t batch, double* psi, const nosh_newton_options* opts,
+                            nosh_batch_instance_stats* stats, void* stream);
+int nosh_batch_destroy(nosh_batch_t batch);
+
+/* ------------------------------------------------------ MLP surrogate ----
+ * No counterpart in the reference: the per-vertex network that seeds Newton
+ * during continuation. x (n_in) -> [Linear(width) + GELU] x depth ->
+ * Linear(n_out); GELU in its tanh form. BF16 weights and activations,
+ * FP32 accumulation (MFMA), FP32 master weights, Adam. Parameters: all
+ * weights (layer by layer, each [out][in] row-major) then all biases. */
+typedef struct nosh_mlp_s* nosh_mlp_t;
+
+int nosh_mlp_create(int n_in, int width, int depth, int n_out, uint64_t seed, nosh_mlp_t* out);
+int nosh_mlp_num_params(nosh_mlp_t mlp, int64_t* nparams, int64_t* nweights);
+int nosh_mlp_get_params(nosh_mlp_t mlp, float* params);
+int nosh_mlp_set_params(nosh_mlp_t mlp, const float* params);
+/* X: M x n_in, Y: M x n_out (host). */
+int nosh_mlp_forward(nosh_mlp_t mlp, const float* X, int64_t M, float* Y);
+int nosh_mlp_forward_device(nosh_mlp_t mlp, const float* X, int64_t M, float* Y, void* stream);
+/* Mean-squared-error loss over the M x n_out outputs against T and its
+ * gradient (nparams values, parameter layout). grad may be NULL. */
+int nosh_mlp_gradients(nosh_mlp_t mlp, const float* X, const float* T, int64_t M, float* grad,
+                       double* loss);
+/* One Adam step (lr; beta 0.9 / 0.999, eps 1e-8) on device minibatch X, T. */
+int nosh_mlp_train_step_device(nosh_mlp_t mlp, const float* X, const float* T, int64_t M,
+                               float lr, double* loss, void* stream);
+int nosh_mlp_destroy(nosh_mlp_t mlp);
+
+/* ------------------------------------------------ energy functionals -----
+ * The functionals nls::gibbs_energy and nls::inner_product compute but do
+ * not return (src/model_evaluator_nls.cpp:697-770; they return 0.0): the
+ * intended values -sum_k c_k |psi_k|^4 / sum_k c_k and
+ * sum_k c_k Re(conj(phi_k) psi_k) / sum_k c_k. */
+int nosh_nls_gibbs_energy(nosh_nls_t nls, const double* psi, double* energy);
+int nosh_nls_inner_product(nosh_nls_t nls, const double* phi, const double* psi, double* value);
+
+/* ------------------------------------------------------ continuation -----
+ * Natural-parameter continuation, the in-repo stand-in for the LOCA stepper
+ * the reference drives through mikado (examples/conf.xml:35-75): the
+ * parameter goes from start to stop in `steps` equal steps; each step
+ * predicts (previous solution, tangent J t = -dF/dp, or secant) and corrects
+ * with Newton-MINRES; a failed corrector halves the step, up to
+ * max_halvings times (LOCA's adaptive step with a minimum size). */
+typedef struct {
+  const char* parameter; /* "mu" or "g" (default "mu") */
+  double start, stop;
+  int steps;
+  int predictor;    /* 0 previous solution, 1 tangent (default), 2 secant */
+  int max_halvings; /* default 4 */
+  nosh_newton_options newton;
+} nosh_continuation_options;
+
+typedef struct {
+  double param;       /* parameter value of this accepted grid point */
+  int newton_iters;   /* Newton steps of the corrector (sum over sub-steps) */
+  int minres_iters;   /* MINRES iterations incl. the tangent solves */
+  int halvings;       /* sub-steps needed */
+  int converged;
+  double norm_f;      /* ||F(psi)||_2 at the accepted point */
+  double gibbs;       /* Gibbs energy (see nosh_nls_gibbs_energy) */
+  double seconds;     /* wall time of this step (predictor + corrector) */
+  int predictor_used; /* 0 previous, 1 tangent, 2 secant, 3 surrogate */
+  int fallback;       /* 1 if the surrogate guess was rejected for the tangent */
+  double pred_error;  /* ||psi_guess - psi_solution||_2 of the accepted guess */
+  double tan_error;   /* ||psi_tangent - psi_solution||_2 (surrogate runs) */
+} nosh_continuation_step;
+
+int nosh_continuation_default_options(nosh_continuation_options* opts);
+/* psi: 2n host values, the initial state in, the last accepted solution out.
+ * record: steps + 1 entries (entry 0 = the solve at `start`). The other
+ * parameters (e.g. "g") are fixed at the given values. Returns
+ * NOSH_ERR_NOT_CONVERGED if a step failed after max_halvings (record and
+ * steps_done describe what was accepted). */
+int nosh_continuation_run(nosh_nls_t nls, double* psi, int nparams, const char* const* names,
+                          const double* values, const nosh_continuation_options* opts,
+                          nosh_continuation_step* record, int* steps_done);
+
+/* Surrogate-seeded continuation (no counterpart in the reference): the
+ * tangent run above, plus a per-vertex MLP that predicts the second-order
+ * correction (psi_{k+1} - psi_k - h t_k) / h^2 from (x, y[, z], Re/Im psi,
+ * h Re/Im t, mu). It trains on the run's own first train_after accepted
+ * steps (train_steps Adam steps on random vertex minibatches), then seeds
+ * Newton with psi_k + h t_k + h^2 MLP(...); the guess is rejected for the
+ * plain tangent predictor when Newton needs more than newton_budget steps
+ * or the converged Gibbs energy leaves the linear extrapolation of the last
+ * two steps by more than max(energy_tol, 10 |E_k - E_{k-1}|). */
+typedef struct {
+  int width, depth;     /* 256, 4 */
+  int train_after;      /* 40 */
+  int train_steps;      /* 300 */
+  int batch;            /* vertices per minibatch, 16384 */
+  float lr;             /* 1e-3 */
+  int newton_budget;    /* 4 */
+  double energy_tol;    /* 1e-8 */
+  uint64_t seed;
+} nosh_surrogate_options;
+
+typedef struct {
+  int trained;
+  double train_seconds;
+  double final_loss;
+  int steps_seeded;     /* steps that used the surrogate guess */
+  int fallbacks;
+} nosh_surrogate_stats;
+
+int nosh_surrogate_default_options(nosh_surrogate_options* opts);
+int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int nparams,
+                                    const char* const* names, const double* values,
+                                    const nosh_continuation_options* opts,
+                                    const nosh_surrogate_options* sopts,
+                                    nosh_continuation_step* record, int* steps_done,
+                                    nosh_surrogate_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

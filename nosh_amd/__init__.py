@@ -556,3 +556,299 @@ def newton_solve_device(model: Nls, psi_ptr: int, params: Dict[str, float],
     _check(_lib.nosh_newton_solve_device(model._h, C.c_void_p(psi_ptr), *p.args, C.byref(o),
                                          C.byref(st), C.c_void_p(stream)))
     return st.as_dict()
+
+
+class ContinuationOptions(C.Structure):
+    _fields_ = [("parameter", C.c_char_p), ("start", C.c_double), ("stop", C.c_double),
+                ("steps", C.c_int), ("predictor", C.c_int), ("max_halvings", C.c_int),
+                ("newton", NewtonOptions)]
+
+
+class ContinuationStep(C.Structure):
+    _fields_ = [("param", C.c_double), ("newton_iters", C.c_int), ("minres_iters", C.c_int),
+                ("halvings", C.c_int), ("converged", C.c_int), ("norm_f", C.c_double),
+                ("gibbs", C.c_double), ("seconds", C.c_double),
+                ("predictor_used", C.c_int), ("fallback", C.c_int),
+                ("pred_error", C.c_double), ("tan_error", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _load_more():
+    lib = _load()
+    if getattr(lib, "_nosh_more", False):
+        return lib
+    vp, i32, d = C.c_void_p, C.c_int, C.c_double
+    cpp = C.POINTER(C.c_char_p)
+    sig = {
+        "nosh_nls_gibbs_energy": (i32, [vp, vp, C.POINTER(d)]),
+        "nosh_nls_inner_product": (i32, [vp, vp, vp, C.POINTER(d)]),
+        "nosh_continuation_default_options": (i32, [C.POINTER(ContinuationOptions)]),
+        "nosh_continuation_run": (i32, [vp, vp, i32, cpp, vp, C.POINTER(ContinuationOptions),
+                                        C.POINTER(ContinuationStep), C.POINTER(i32)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    lib._nosh_more = True
+    return lib
+
+
+def gibbs_energy(model: Nls, psi) -> float:
+    """-sum c |psi|^4 / sum c (the value nls::gibbs_energy computes)."""
+    lib = _load_more()
+    x = _vec(psi, 2 * model.mesh.num_vertices)
+    v = C.c_double()
+    _check(lib.nosh_nls_gibbs_energy(model._h, _ptr(x), C.byref(v)))
+    return v.value
+
+
+def inner_product(model: Nls, phi, psi) -> float:
+    """sum c Re(conj(phi) psi) / sum c (the value nls::inner_product computes)."""
+    lib = _load_more()
+    m = 2 * model.mesh.num_vertices
+    a, b = _vec(phi, m), _vec(psi, m)
+    v = C.c_double()
+    _check(lib.nosh_nls_inner_product(model._h, _ptr(a), _ptr(b), C.byref(v)))
+    return v.value
+
+
+def continuation(model: Nls, psi0, params: Dict[str, float], parameter: str = "mu",
+                 start: float = 0.0, stop: float = 1.0, steps: int = 10, predictor: int = 1,
+                 max_halvings: int = 4, newton: Optional[NewtonOptions] = None, **newton_kw):
+    """Natural-parameter continuation; returns (final psi, list of step records).
+
+    Raises NoshError(NOT_CONVERGED) only through the records: a failed step
+    ends the run early and is reported with converged=0.
+    """
+    lib = _load_more()
+    o = ContinuationOptions()
+    _check(lib.nosh_continuation_default_options(C.byref(o)))
+    pname = parameter.encode()
+    o.parameter, o.start, o.stop, o.steps = pname, start, stop, steps
+    o.predictor, o.max_halvings = predictor, max_halvings
+    o.newton = newton if newton is not None else default_newton_options(**newton_kw)
+    psi = np.array(_vec(psi0, 2 * model.mesh.num_vertices), copy=True)
+    rec = (ContinuationStep * (steps + 1))()
+    done = C.c_int()
+    p = _Params(params)
+    rc = lib.nosh_continuation_run(model._h, _ptr(psi), *p.args, C.byref(o), rec, C.byref(done))
+    if rc not in (_OK, _NOTCONV):
+        _check(rc)
+    out = [rec[i].as_dict() for i in range(done.value)]
+    if rc == _NOTCONV and done.value <= steps:
+        out.append(rec[done.value].as_dict())
+    return psi, out
+
+
+class BatchInstanceStats(C.Structure):
+    _fields_ = [("newton_iters", C.c_int), ("minres_iters", C.c_int), ("converged", C.c_int),
+                ("final_norm_f", C.c_double)]
+
+
+def _load_batch():
+    lib = _load_more()
+    if getattr(lib, "_nosh_batch", False):
+        return lib
+    vp, i32 = C.c_void_p, C.c_int
+    sig = {
+        "nosh_batch_create": (i32, [i32, vp, vp, vp, vp, C.POINTER(vp)]),
+        "nosh_batch_set_tuning": (i32, [vp, i32, i32, i32, i32]),
+        "nosh_batch_autotune": (i32, [vp, vp, C.POINTER(NewtonOptions), i32, C.POINTER(i32),
+                                      C.POINTER(i32)]),
+        "nosh_batch_solve": (i32, [vp, vp, C.POINTER(NewtonOptions),
+                                   C.POINTER(BatchInstanceStats)]),
+        "nosh_batch_solve_device": (i32, [vp, vp, C.POINTER(NewtonOptions),
+                                          C.POINTER(BatchInstanceStats), vp]),
+        "nosh_batch_destroy": (i32, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    lib._nosh_batch = True
+    return lib
+
+
+class Batch(_Handle):
+    """Many Newton-MINRES solves on meshes of uneven size, advanced together.
+
+    groups: list of (model, count); mu, g: one value per instance, group-major.
+    """
+    _destroy = "nosh_batch_destroy"
+
+    def __init__(self, groups, mu, g):
+        super().__init__()
+        lib = _load_batch()
+        self.models = [m for m, _ in groups]
+        self.counts = [int(c) for _, c in groups]
+        self.sizes = [2 * m.mesh.num_vertices for m in self.models]
+        self.ninst = sum(self.counts)
+        self.mu = np.ascontiguousarray(mu, dtype=np.float64)
+        self.g = np.ascontiguousarray(g, dtype=np.float64)
+        if self.mu.size != self.ninst or self.g.size != self.ninst:
+            raise ValueError("mu and g need one value per instance")
+        hs = (C.c_void_p * len(groups))(*[m._h for m in self.models])
+        cnt = np.array(self.counts, dtype=np.int32)
+        _check(lib.nosh_batch_create(len(groups), hs, _ptr(cnt), _ptr(self.mu), _ptr(self.g),
+                                     C.byref(self._h)))
+
+    @property
+    def total_length(self) -> int:
+        return sum(c * s for c, s in zip(self.counts, self.sizes))
+
+    def split(self, psi):
+        """Instance vectors of a concatenated state."""
+        out, off = [], 0
+        for c, s in zip(self.counts, self.sizes):
+            for _ in range(c):
+                out.append(psi[off:off + s])
+                off += s
+        return out
+
+    def set_tuning(self, block_threads=256, mode=0, graph_iters=4, poll=16):
+        _check(_lib.nosh_batch_set_tuning(self._h, block_threads, mode, graph_iters, poll))
+
+    def autotune(self, psi, options: Optional[NewtonOptions] = None, use_cache=True, **kw):
+        o = options if options is not None else default_newton_options(**kw)
+        x = _vec(psi, self.total_length)
+        t, m = C.c_int(), C.c_int()
+        _check(_lib.nosh_batch_autotune(self._h, _ptr(x), C.byref(o), 1 if use_cache else 0,
+                                        C.byref(t), C.byref(m)))
+        return t.value, m.value
+
+    def solve(self, psi0, options: Optional[NewtonOptions] = None, **kw):
+        o = options if options is not None else default_newton_options(**kw)
+        psi = np.array(_vec(psi0, self.total_length), copy=True)
+        st = (BatchInstanceStats * self.ninst)()
+        _check(_lib.nosh_batch_solve(self._h, _ptr(psi), C.byref(o), st))
+        return psi, [{k: getattr(st[i], k) for k, _ in BatchInstanceStats._fields_}
+                     for i in range(self.ninst)]
+
+    def solve_device(self, psi_ptr: int, options: Optional[NewtonOptions] = None, stream=0, **kw):
+        o = options if options is not None else default_newton_options(**kw)
+        st = (BatchInstanceStats * self.ninst)()
+        _check(_lib.nosh_batch_solve_device(self._h, C.c_void_p(psi_ptr), C.byref(o), st,
+                                            C.c_void_p(stream)))
+        return [{k: getattr(st[i], k) for k, _ in BatchInstanceStats._fields_}
+                for i in range(self.ninst)]
+
+
+def _load_mlp():
+    lib = _load_batch()
+    if getattr(lib, "_nosh_mlp", False):
+        return lib
+    vp, i32, i64 = C.c_void_p, C.c_int, C.c_int64
+    sig = {
+        "nosh_mlp_create": (i32, [i32, i32, i32, i32, C.c_uint64, C.POINTER(vp)]),
+        "nosh_mlp_num_params": (i32, [vp, C.POINTER(i64), C.POINTER(i64)]),
+        "nosh_mlp_get_params": (i32, [vp, vp]),
+        "nosh_mlp_set_params": (i32, [vp, vp]),
+        "nosh_mlp_forward": (i32, [vp, vp, i64, vp]),
+        "nosh_mlp_forward_device": (i32, [vp, vp, i64, vp, vp]),
+        "nosh_mlp_gradients": (i32, [vp, vp, vp, i64, vp, C.POINTER(C.c_double)]),
+        "nosh_mlp_train_step_device": (i32, [vp, vp, vp, i64, C.c_float,
+                                             C.POINTER(C.c_double), vp]),
+        "nosh_mlp_destroy": (i32, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    lib._nosh_mlp = True
+    return lib
+
+
+class Mlp(_Handle):
+    """Per-vertex MLP (BF16 MFMA kernels, FP32 accumulation and masters)."""
+    _destroy = "nosh_mlp_destroy"
+
+    def __init__(self, n_in=7, width=256, depth=4, n_out=2, seed=0):
+        super().__init__()
+        lib = _load_mlp()
+        _check(lib.nosh_mlp_create(n_in, width, depth, n_out, seed, C.byref(self._h)))
+        self.n_in, self.n_out = n_in, n_out
+        self.dims = [n_in] + [width] * depth + [n_out]
+        npar, nw = C.c_int64(), C.c_int64()
+        _check(lib.nosh_mlp_num_params(self._h, C.byref(npar), C.byref(nw)))
+        self.nparams, self.nweights = npar.value, nw.value
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.nparams, np.float32)
+        _check(_lib.nosh_mlp_get_params(self._h, _ptr(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(_lib.nosh_mlp_set_params(self._h, _ptr(p)))
+
+    def forward(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.empty((X.shape[0], self.n_out), np.float32)
+        _check(_lib.nosh_mlp_forward(self._h, _ptr(X), X.shape[0], _ptr(Y)))
+        return Y
+
+    def gradients(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.empty(self.nparams, np.float32)
+        loss = C.c_double()
+        _check(_lib.nosh_mlp_gradients(self._h, _ptr(X), _ptr(T), X.shape[0], _ptr(g),
+                                       C.byref(loss)))
+        return loss.value, g
+
+    def train_step_device(self, x_ptr: int, t_ptr: int, M: int, lr: float, stream: int = 0):
+        loss = C.c_double()
+        _check(_lib.nosh_mlp_train_step_device(self._h, C.c_void_p(x_ptr), C.c_void_p(t_ptr),
+                                               M, lr, C.byref(loss), C.c_void_p(stream)))
+        return loss.value
+
+
+class SurrogateOptions(C.Structure):
+    _fields_ = [("width", C.c_int), ("depth", C.c_int), ("train_after", C.c_int),
+                ("train_steps", C.c_int), ("batch", C.c_int), ("lr", C.c_float),
+                ("newton_budget", C.c_int), ("energy_tol", C.c_double), ("seed", C.c_uint64)]
+
+
+class SurrogateStats(C.Structure):
+    _fields_ = [("trained", C.c_int), ("train_seconds", C.c_double), ("final_loss", C.c_double),
+                ("steps_seeded", C.c_int), ("fallbacks", C.c_int)]
+
+
+def continuation_surrogate(model: Nls, psi0, params: Dict[str, float], parameter: str = "mu",
+                           start: float = 0.0, stop: float = 1.0, steps: int = 10,
+                           newton: Optional[NewtonOptions] = None, surrogate: Optional[dict] = None,
+                           **newton_kw):
+    """Surrogate-seeded continuation; returns (psi, step records, surrogate stats)."""
+    lib = _load_mlp()
+    if not getattr(lib, "_nosh_sur", False):
+        lib.nosh_surrogate_default_options.restype = C.c_int
+        lib.nosh_surrogate_default_options.argtypes = [C.POINTER(SurrogateOptions)]
+        lib.nosh_continuation_run_surrogate.restype = C.c_int
+        lib.nosh_continuation_run_surrogate.argtypes = [
+            C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.c_void_p,
+            C.POINTER(ContinuationOptions), C.POINTER(SurrogateOptions),
+            C.POINTER(ContinuationStep), C.POINTER(C.c_int), C.POINTER(SurrogateStats)]
+        lib._nosh_sur = True
+    o = ContinuationOptions()
+    _check(lib.nosh_continuation_default_options(C.byref(o)))
+    pname = parameter.encode()
+    o.parameter, o.start, o.stop, o.steps, o.predictor = pname, start, stop, steps, 1
+    o.newton = newton if newton is not None else default_newton_options(**newton_kw)
+    so = SurrogateOptions()
+    _check(lib.nosh_surrogate_default_options(C.byref(so)))
+    for k, v in (surrogate or {}).items():
+        setattr(so, k, v)
+    psi = np.array(_vec(psi0, 2 * model.mesh.num_vertices), copy=True)
+    rec = (ContinuationStep * (steps + 1))()
+    done = C.c_int()
+    st = SurrogateStats()
+    p = _Params(params)
+    rc = lib.nosh_continuation_run_surrogate(model._h, _ptr(psi), *p.args, C.byref(o),
+                                             C.byref(so), rec, C.byref(done), C.byref(st))
+    if rc not in (_OK, _NOTCONV):
+        _check(rc)
+    out = [rec[i].as_dict() for i in range(done.value)]
+    return psi, out, {k: getattr(st, k) for k, _ in SurrogateStats._fields_}

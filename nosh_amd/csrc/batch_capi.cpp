@@ -1,0 +1,410 @@
+// batch_capi.cpp -- C ABI of the batched solver and its autotuner.
+//
+// A batch groups instances by model (mesh + fields); each instance has its
+// own mu, g and state. Newton runs in lockstep over the batch: one F
+// evaluation, one block rebuild and one MINRES (FP32 inner, FP64 scalars and
+// update) per step for all instances still above the tolerance.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "batched.hpp"
+#include "handles.hpp"
+
+using namespace nosh;
+
+struct nosh_batch_s {
+  Batch b;
+  std::vector<nosh_nls_s*> models;
+  std::vector<int> counts;
+  int64_t total_len = 0;  // sum over instances of 2 n
+  DBuf<double> stage;
+};
+
+namespace {
+
+template <class F>
+int bguard(F&& f) {
+  try {
+    f();
+    set_last_error("");
+    return NOSH_OK;
+  } catch (const nosh::Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return NOSH_ERR_INVALID;
+  }
+}
+
+int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+void alloc_group(GroupHost& G) {
+  const int64_t m = G.mesh->n * G.dev.bp;
+  G.psi.alloc(m);
+  G.f.alloc(m);
+  G.x.alloc(m);
+  G.jblk.alloc(3 * m);
+  G.pblk.alloc(3 * m);
+  for (auto* v : {&G.Y, &G.Z2, &G.v, &G.u, &G.R0, &G.R1, &G.D, &G.W0, &G.W1, &G.W2}) {
+    v->alloc(m);
+    v->zero(default_stream());
+  }
+  BGroup& d = G.dev;
+  d.psi = G.psi.p;
+  d.f = G.f.p;
+  d.x = G.x.p;
+  d.jblk = G.jblk.p;
+  d.pblk = G.pblk.p;
+  d.Y = G.Y.p;
+  d.Z2 = G.Z2.p;
+  d.v = G.v.p;
+  d.u = G.u.p;
+  d.R0 = G.R0.p;
+  d.R1 = G.R1.p;
+  d.D = G.D.p;
+  d.W0 = G.W0.p;
+  d.W1 = G.W1.p;
+  d.W2 = G.W2.p;
+}
+
+// Instance vectors (reference order, concatenated) <-> interleaved state.
+void scatter_all(nosh_batch_s* h, const double* src_dev, hipStream_t s) {
+  int64_t off = 0;
+  for (auto& G : h->b.ghost) {
+    for (int b = 0; b < G.B; ++b) {
+      launch_scatter_lane(G.mesh->n, G.dev.bp, b, G.mesh->perm.p, src_dev + off,
+                          reinterpret_cast<double*>(G.psi.p), s);
+      off += 2 * G.mesh->n;
+    }
+  }
+}
+
+void gather_all(nosh_batch_s* h, double* dst_dev, hipStream_t s) {
+  int64_t off = 0;
+  for (auto& G : h->b.ghost) {
+    for (int b = 0; b < G.B; ++b) {
+      launch_gather_lane(G.mesh->n, G.dev.bp, b, G.mesh->perm.p,
+                         reinterpret_cast<const double*>(G.psi.p), dst_dev + off, s);
+      off += 2 * G.mesh->n;
+    }
+  }
+}
+
+void newton_batch(nosh_batch_s* h, const nosh_newton_options& o,
+                  nosh_batch_instance_stats* st, hipStream_t s) {
+  Batch& B = h->b;
+  const int ni = B.args.ninst;
+  B.args.iters = o.cheb_iters;
+  B.args.ratio = o.cheb_ratio;
+  B.args.upper = 0.0;
+  B.args.z0 = std::max(o.cheb_iters - 1, 0) % 2;
+  B.drop_graph();
+  std::vector<double> nf, nf_prev(ni, -1.0), eta_prev(ni, o.forcing_max), rt(ni);
+  std::vector<int> done(ni, 0), newton(ni, 0), mres(ni, 0);
+  std::vector<double> final_nf(ni, 0.0);
+  for (int it = 0;; ++it) {
+    B.compute_norms(s, nf);
+    int nactive = 0;
+    for (int i = 0; i < ni; ++i) {
+      if (done[i]) {
+        B.active_h[i] = 0;
+        continue;
+      }
+      final_nf[i] = nf[i];
+      if (!(nf[i] > o.tol) || !std::isfinite(nf[i]) || it >= o.max_iters) {
+        done[i] = 1;
+        B.active_h[i] = 0;
+        continue;
+      }
+      double eta = o.minres_rtol;
+      if (o.forcing == 1) {
+        if (nf_prev[i] < 0.0) {
+          eta = o.forcing_max;
+        } else {
+          eta = 0.9 * (nf[i] / nf_prev[i]) * (nf[i] / nf_prev[i]);
+          const double safe = 0.9 * eta_prev[i] * eta_prev[i];
+          if (safe > 0.1) eta = std::max(eta, safe);
+        }
+        eta = std::min(eta, o.forcing_max);
+        eta = std::max(eta, 0.5 * o.tol / nf[i]);
+        eta = std::max(eta, o.forcing_min);
+      }
+      rt[i] = eta;
+      eta_prev[i] = eta;
+      nf_prev[i] = nf[i];
+      B.active_h[i] = 1;
+      newton[i] = it + 1;
+      ++nactive;
+    }
+    if (nactive == 0) break;
+    NOSH_HIP(hipMemcpyAsync(B.active.p, B.active_h.data(), sizeof(int) * ni,
+                            hipMemcpyHostToDevice, s));
+    NOSH_HIP(hipMemcpyAsync(B.rtol.p, rt.data(), sizeof(double) * ni, hipMemcpyHostToDevice, s));
+    B.minres(s, o.minres_max_iters);
+    for (int i = 0; i < ni; ++i)
+      if (B.active_h[i]) mres[i] += B.Sh[i].it;
+  }
+  if (st)
+    for (int i = 0; i < ni; ++i) {
+      st[i].newton_iters = newton[i];
+      st[i].minres_iters = mres[i];
+      st[i].final_norm_f = final_nf[i];
+      st[i].converged = (final_nf[i] <= o.tol) ? 1 : 0;
+    }
+}
+
+std::string tune_key(nosh_batch_s* h, const nosh_newton_options& o) {
+  hipDeviceProp_t prop{};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipGetDeviceProperties(&prop, dev);
+  std::ostringstream k;
+  k << prop.gcnArchName << "|cheb" << o.cheb_iters;
+  for (auto& G : h->b.ghost) k << "|" << G.mesh->n << "x" << G.dev.bp;
+  return k.str();
+}
+
+std::string tune_path() {
+  if (const char* p = std::getenv("NOSH_AUTOTUNE_CACHE")) return p;
+  const char* home = std::getenv("HOME");
+  return std::string(home ? home : "/tmp") + "/.cache/nosh_amd_autotune.tsv";
+}
+
+bool tune_lookup(const std::string& key, int* threads, int* mode) {
+  std::ifstream f(tune_path());
+  std::string k;
+  int t, m;
+  while (f >> k >> t >> m)
+    if (k == key) {
+      *threads = t;
+      *mode = m;
+      return true;
+    }
+  return false;
+}
+
+void tune_store(const std::string& key, int threads, int mode) {
+  std::ofstream f(tune_path(), std::ios::app);
+  if (f) f << key << " " << threads << " " << mode << "\n";
+}
+
+}  // namespace
+
+extern "C" {
+
+int nosh_batch_create(int ngroups, const nosh_nls_t* models, const int* counts, const double* mu,
+                      const double* g, nosh_batch_t* out) {
+  return bguard([&] {
+    NOSH_CHECK(ngroups >= 1 && ngroups <= 16 && models && counts && mu && g && out, kInvalid,
+               "invalid batch description");
+    auto h = std::make_unique<nosh_batch_s>();
+    Batch& B = h->b;
+    hipStream_t s = default_stream();
+    int inst = 0;
+    for (int k = 0; k < ngroups; ++k) {
+      NOSH_CHECK(models[k] && counts[k] >= 1 && counts[k] <= 64, kInvalid,
+                 "each group needs a model and 1..64 instances");
+      nosh_nls_s* md = models[k];
+      h->models.push_back(md);
+      h->counts.push_back(counts[k]);
+      GroupHost G;
+      G.mesh = md->mesh->m.get();
+      G.B = counts[k];
+      MeshCache& m = *G.mesh;
+      // alpha_e with the model's thickness (keo::build_alpha_cache_).
+      std::map<std::string, double> p0{{"mu", mu[inst]}, {"g", g[inst]}};
+      md->keo->set_parameters(p0, s);
+      G.a32.alloc(m.nnz_pad);
+      G.t32.alloc(m.nnz_pad);
+      G.a64.alloc(m.nnz_pad);
+      G.t64.alloc(m.nnz_pad);
+      G.diag.alloc(m.n);
+      G.rowabs.alloc(m.n);
+      launch_slot_fields(m, md->keo->alpha.p, md->mvp->vf.atilde.p, G.a32.p, G.t32.p, G.a64.p,
+                         G.t64.p, G.diag.p, G.rowabs.p, s);
+      G.ct.alloc(m.n);
+      compute_ct(m, md->thickness, G.ct, p0, s);
+      G.V.alloc(m.n);
+      NOSH_HIP(hipMemcpyAsync(G.V.p, md->V->sf.get_v(p0, s), sizeof(double) * m.n,
+                              hipMemcpyDeviceToDevice, s));
+      BGroup& d = G.dev;
+      d.n = m.n;
+      d.B = counts[k];
+      d.bp = next_pow2(counts[k]);
+      d.inst0 = inst;
+      d.slice_ptr = m.slice_ptr.p;
+      d.scol = m.scol.p;
+      d.a32 = G.a32.p;
+      d.t32 = G.t32.p;
+      d.a64 = G.a64.p;
+      d.t64 = G.t64.p;
+      d.diag = G.diag.p;
+      d.rowabs = G.rowabs.p;
+      d.ct = G.ct.p;
+      d.V = G.V.p;
+      alloc_group(G);
+      h->total_len += int64_t(counts[k]) * 2 * m.n;
+      inst += counts[k];
+      B.ghost.push_back(std::move(G));
+    }
+    B.args.ngroups = ngroups;
+    B.args.ninst = inst;
+    B.mu.alloc(inst);
+    B.g.alloc(inst);
+    B.mu.upload(mu, inst, s);
+    B.g.upload(g, inst, s);
+    B.bound.alloc(inst);
+    B.rtol.alloc(inst);
+    B.norms.alloc(inst);
+    B.active.alloc(inst);
+    B.S.alloc(inst);
+    B.Sh.resize(inst);
+    B.active_h.assign(inst, 0);
+    B.args.S = B.S.p;
+    B.args.mu = B.mu.p;
+    B.args.g = B.g.p;
+    B.args.bound = B.bound.p;
+    B.args.rtol = B.rtol.p;
+    B.args.active = B.active.p;
+    B.relayout(256);
+    NOSH_HIP(hipStreamSynchronize(s));
+    *out = h.release();
+  });
+}
+
+int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int mode, int graph_iters,
+                          int poll) {
+  return bguard([&] {
+    NOSH_CHECK(batch, kInvalid, "null batch");
+    int maxbp = 1;
+    for (auto& G : batch->b.ghost) maxbp = std::max(maxbp, G.dev.bp);
+    NOSH_CHECK(block_threads >= 64 && block_threads <= 1024 && block_threads % 64 == 0 &&
+                   block_threads % maxbp == 0,
+               kInvalid, "block_threads must be a multiple of 64 and of the lane count, <= 1024");
+    NOSH_CHECK(mode == 0 || mode == 1, kInvalid, "mode must be 0 (graph) or 1 (persistent)");
+    NOSH_CHECK(graph_iters >= 1 && poll >= graph_iters, kInvalid, "bad graph_iters / poll");
+    batch->b.graph_iters = graph_iters;
+    batch->b.poll = poll;
+    if (block_threads != batch->b.block_threads) batch->b.relayout(block_threads);
+    batch->b.drop_graph();
+    batch->b.mode = 0;
+    NOSH_CHECK(mode == 0 || batch->b.args.total_blocks <= nosh::kPersistentMaxBlocks,
+               kUnsupported,
+               "persistent mode needs a grid of at most 64 workgroups at this block size; "
+               "use mode 0 (hipGraph)");
+    batch->b.mode = mode;
+  });
+}
+
+int nosh_batch_solve_device(nosh_batch_t batch, double* psi, const nosh_newton_options* opts,
+                            nosh_batch_instance_stats* stats, void* stream) {
+  return bguard([&] {
+    NOSH_CHECK(batch && psi, kInvalid, "null argument");
+    nosh_newton_options o;
+    if (opts)
+      o = *opts;
+    else
+      nosh_newton_default_options(&o);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : default_stream();
+    scatter_all(batch, psi, s);
+    newton_batch(batch, o, stats, s);
+    gather_all(batch, psi, s);
+    NOSH_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int nosh_batch_solve(nosh_batch_t batch, double* psi, const nosh_newton_options* opts,
+                     nosh_batch_instance_stats* stats) {
+  return bguard([&] {
+    NOSH_CHECK(batch && psi, kInvalid, "null argument");
+    hipStream_t s = default_stream();
+    if (batch->stage.n != size_t(batch->total_len)) batch->stage.alloc(batch->total_len);
+    batch->stage.upload(psi, batch->total_len, s);
+    const int rc = nosh_batch_solve_device(batch, batch->stage.p, opts, stats, s);
+    if (rc != NOSH_OK) throw nosh::Error(rc, nosh_last_error());
+    batch->stage.download(psi, batch->total_len, s);
+    NOSH_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int nosh_batch_autotune(nosh_batch_t batch, const double* psi, const nosh_newton_options* opts,
+                        int use_cache, int* best_threads, int* best_mode) {
+  return bguard([&] {
+    NOSH_CHECK(batch && psi, kInvalid, "null argument");
+    nosh_newton_options o;
+    if (opts)
+      o = *opts;
+    else
+      nosh_newton_default_options(&o);
+    const std::string key = tune_key(batch, o);
+    int bt = 256, bm = 0;
+    if (use_cache && tune_lookup(key, &bt, &bm)) {
+      nosh_batch_set_tuning(batch, bt, bm, batch->b.graph_iters, batch->b.poll);
+    } else {
+      // Time one Newton step (F, rebuild, MINRES capped at 40 iterations)
+      // for every candidate layout and iteration mode.
+      int maxbp = 1;
+      for (auto& G : batch->b.ghost) maxbp = std::max(maxbp, G.dev.bp);
+      hipStream_t s = default_stream();
+      if (batch->stage.n != size_t(batch->total_len)) batch->stage.alloc(batch->total_len);
+      nosh_newton_options t = o;
+      t.max_iters = 1;
+      t.minres_max_iters = 40;
+      t.forcing = 0;
+      t.minres_rtol = 1e-30;
+      double best = 1e30;
+      for (int threads : {128, 256, 512, 1024}) {
+        if (threads % maxbp || threads < maxbp) continue;
+        for (int mode : {0, 1}) {
+          if (nosh_batch_set_tuning(batch, threads, mode, batch->b.graph_iters,
+                                    batch->b.poll) != NOSH_OK)
+            continue;
+          double tmin = 1e30;
+          for (int rep = 0; rep < 2; ++rep) {
+            batch->stage.upload(psi, batch->total_len, s);
+            NOSH_HIP(hipStreamSynchronize(s));
+            const auto t0 = std::chrono::steady_clock::now();
+            scatter_all(batch, batch->stage.p, s);
+            newton_batch(batch, t, nullptr, s);
+            NOSH_HIP(hipStreamSynchronize(s));
+            tmin = std::min(tmin, std::chrono::duration<double>(
+                                      std::chrono::steady_clock::now() - t0).count());
+          }
+          if (std::getenv("NOSH_AUTOTUNE_VERBOSE"))
+            std::fprintf(stderr, "autotune: block_threads %d mode %d: %.4f s\n", threads, mode,
+                         tmin);
+          if (tmin < best) {
+            best = tmin;
+            bt = threads;
+            bm = mode;
+          }
+        }
+      }
+      nosh_batch_set_tuning(batch, bt, bm, batch->b.graph_iters, batch->b.poll);
+      if (use_cache) tune_store(key, bt, bm);
+    }
+    if (best_threads) *best_threads = bt;
+    if (best_mode) *best_mode = bm;
+  });
+}
+
+int nosh_batch_destroy(nosh_batch_t batch) {
+  return bguard([&] { delete batch; });
+}
+
+}  // extern "C"

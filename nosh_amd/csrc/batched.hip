@@ -1,0 +1,545 @@
+// batched.hip -- many Newton-MINRES solves, on meshes of uneven size, in one
+// launch per phase.
+//
+// An instance is (mesh, mu, g, psi). Instances on the same mesh form a group;
+// one kernel launch covers every instance of every group. Within a group the
+// vectors are instance-interleaved ([row][lane], lane = instance, padded to a
+// power of two bp), so the gather of x[col] for one row is one contiguous
+// read of bp complex values. The KEO is never stored per instance: each padded
+// slot keeps (alpha, a~_dir) once per mesh (a~_dir = +a~ for the lower
+// endpoint's row, -a~ for the higher one's) and the value
+//   K_ij = -alpha e^{-i mu_b a~_dir}
+// is rebuilt in registers (FP32 __sincosf in the inner solver, FP64 sincos
+// for F), trading ~20x fewer matrix bytes for cheap ALU work.
+//
+// Precision: F(psi), the Newton update and all MINRES scalars are FP64; the
+// Krylov vectors, blocks and inner SpMVs are FP32 (the FP32-inner / FP64-outer
+// scheme). MINRES runs in lockstep over the batch with per-instance scalars
+// and done flags; one iteration is either a captured hipGraph (replayed) or
+// a persistent cooperative kernel with grid-wide barriers between phases.
+//
+// Reference operators: keo::refill_ (src/parameter_matrix_keo.cpp:74-184),
+// nls::compute_f_ (src/model_evaluator_nls.cpp:527-628), jacobian_operator
+// (src/jacobian_operator.cpp:143-200), keo_regularized::rebuild
+// (src/keo_regularized.cpp:181-264).
+
+#include <hip/hip_cooperative_groups.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <type_traits>
+
+#include "batched.hpp"
+#include "kernels.hpp"
+#include "minres_device.hpp"
+
+namespace cg = cooperative_groups;
+
+namespace nosh {
+namespace {
+
+struct Lane {
+  const BGroup* G;
+  int64_t row;
+  int b;
+  int inst;
+  int64_t idx;  // row * bp + b
+  bool valid;
+};
+
+__device__ __forceinline__ int group_of(const BatchArgs& a, int64_t vb) {
+  int gi = 0;
+  while (gi + 1 < a.ngroups && vb >= a.groups[gi + 1].block0) ++gi;
+  return gi;
+}
+
+__device__ __forceinline__ Lane lane_ctx(const BatchArgs& a, int64_t vb) {
+  Lane L;
+  const int gi = group_of(a, vb);
+  L.G = &a.groups[gi];
+  const int bp = L.G->bp;
+  L.b = threadIdx.x & (bp - 1);
+  L.row = (vb - L.G->block0) * (blockDim.x / bp) + threadIdx.x / bp;
+  L.inst = L.G->inst0 + L.b;
+  L.idx = L.row * bp + L.b;
+  L.valid = L.row < L.G->n && L.b < L.G->B;
+  return L;
+}
+
+// Per-instance partial sums of one (virtual) block: rows of the block are
+// reduced in order for every lane b.
+__device__ __forceinline__ void lane_partials(const BatchArgs& a, const Lane& L, int64_t vb,
+                                              double v) {
+  __shared__ double lds[1024];
+  lds[threadIdx.x] = L.valid ? v : 0.0;
+  __syncthreads();
+  const int bp = L.G->bp;
+  if (threadIdx.x < bp && threadIdx.x < L.G->B) {
+    double s = 0.0;
+    for (int r = 0; r < int(blockDim.x) / bp; ++r) s += lds[r * bp + threadIdx.x];
+    a.partials[int64_t(L.G->inst0 + threadIdx.x) * a.maxblocks + (vb - L.G->block0)] = s;
+  }
+  __syncthreads();
+}
+
+template <class T>
+struct Vt;
+template <>
+struct Vt<float> {
+  using v2 = float2;
+};
+template <>
+struct Vt<double> {
+  using v2 = double2;
+};
+
+__device__ __forceinline__ void sc(float x, float* s, float* c) { __sincosf(x, s, c); }
+__device__ __forceinline__ void sc(double x, double* s, double* c) { sincos(x, s, c); }
+
+// Off-diagonal product of one (row, instance) with values rebuilt in registers.
+template <class T>
+__device__ __forceinline__ typename Vt<T>::v2 brow(const BGroup& G, int64_t row, int b, T mu,
+                                                    const typename Vt<T>::v2* __restrict__ x) {
+  const int32_t s = int32_t(row >> 6);
+  const int32_t beg = G.slice_ptr[s], end = G.slice_ptr[s + 1];
+  const T* al;
+  const T* at;
+  if constexpr (std::is_same<T, float>::value) {
+    al = G.a32;
+    at = G.t32;
+  } else {
+    al = G.a64;
+    at = G.t64;
+  }
+  T ar = T(0), ai = T(0);
+  const int bp = G.bp;
+#pragma unroll 2
+  for (int32_t j = beg + int32_t(row & 63); j < end; j += kSliceH) {
+    const int32_t c = G.scol[j];
+    const T a = al[j];
+    T sn, cs;
+    sc(mu * at[j], &sn, &cs);
+    const T vr = -a * cs, vi = a * sn;
+    const typename Vt<T>::v2 xv = x[int64_t(c) * bp + b];
+    ar += vr * xv.x - vi * xv.y;
+    ai += vr * xv.y + vi * xv.x;
+  }
+  typename Vt<T>::v2 r;
+  r.x = ar;
+  r.y = ai;
+  return r;
+}
+
+// ---------------------------------------------------------------- phases
+// Every phase is a device function over one virtual block, used both by the
+// per-phase kernels (graph mode) and by the persistent kernel.
+
+__device__ void ph_compute_f(const BatchArgs& a, int64_t vb) {
+  const Lane L = lane_ctx(a, vb);
+  double s2 = 0.0;
+  if (L.valid) {
+    const BGroup& G = *L.G;
+    const double mu = a.mu[L.inst], g = a.g[L.inst];
+    const double2 acc = brow<double>(G, L.row, L.b, mu, G.psi);
+    const double2 xi = G.psi[L.idx];
+    const double d = G.diag[L.row] + G.ct[L.row] * (G.V[L.row] + g * (xi.x * xi.x + xi.y * xi.y));
+    const double2 r = make_double2(acc.x + d * xi.x, acc.y + d * xi.y);
+    G.f[L.idx] = r;
+    s2 = r.x * r.x + r.y * r.y;
+  }
+  lane_partials(a, L, vb, s2);
+}
+
+// J and P blocks (FP32, including diag K) and the per-instance Chebyshev bound.
+__device__ void ph_rebuild(const BatchArgs& a, int64_t vb) {
+  const Lane L = lane_ctx(a, vb);
+  __shared__ double rmax[1024];
+  rmax[threadIdx.x] = 0.0;
+  if (L.valid) {
+  const BGroup& G = *L.G;
+  const int64_t nb3 = G.n * G.bp;
+  const double g = a.g[L.inst];
+  const double2 p = G.psi[L.idx];
+  const double c = G.ct[L.row], d = G.diag[L.row];
+  const double r2 = p.x * p.x + p.y * p.y;
+  const double al = c * (G.V[L.row] + g * 2.0 * r2);
+  const double rx2 = g * c * (p.x * p.x - p.y * p.y);
+  const double d1b = g * c * (2.0 * p.x * p.y);
+  G.jblk[L.idx] = float(d + al + rx2);
+  G.jblk[nb3 + L.idx] = float(d + al - rx2);
+  G.jblk[2 * nb3 + L.idx] = float(d1b);
+  double a0 = d, a1 = d, bb = 0.0;
+  if (g > 0.0) {
+    const double pa = g * c * 2.0 * r2;
+    a0 = d + pa + rx2;
+    a1 = d + pa - rx2;
+    bb = d1b;
+  }
+  G.pblk[L.idx] = float(a0);
+  G.pblk[nb3 + L.idx] = float(a1);
+  G.pblk[2 * nb3 + L.idx] = float(bb);
+  const double lmin = 0.5 * (a0 + a1) - hypot(0.5 * (a0 - a1), bb);
+  rmax[threadIdx.x] = lmin > 0.0 ? G.rowabs[L.row] / lmin : 1.0;
+  }
+  __syncthreads();
+  // One atomic per (block, instance): the max over the block's rows.
+  const int bp = L.G->bp;
+  if (threadIdx.x < bp && threadIdx.x < L.G->B) {
+    double m = 0.0;
+    for (int r = 0; r < int(blockDim.x) / bp; ++r) m = fmax(m, rmax[r * bp + threadIdx.x]);
+    atomicMax(reinterpret_cast<unsigned long long*>(a.bound) + L.G->inst0 + threadIdx.x,
+              static_cast<unsigned long long>(__double_as_longlong(m)));
+  }
+}
+
+__device__ void ph_setup(const BatchArgs& a, int64_t vb) {
+  const Lane L = lane_ctx(a, vb);
+  if (!L.valid) return;
+  const BGroup& G = *L.G;
+  const double2 fv = G.f[L.idx];
+  const float2 r = make_float2(float(-fv.x), float(-fv.y));
+  G.R0[L.idx] = r;
+  G.R1[L.idx] = r;
+  G.W0[L.idx] = make_float2(0.f, 0.f);
+  G.W2[L.idx] = make_float2(0.f, 0.f);
+  G.x[L.idx] = make_double2(0.0, 0.0);
+}
+
+__device__ __forceinline__ double hi_of(const BatchArgs& a, int inst) {
+  return a.upper > 0.0 ? a.upper : 1.0 + a.bound[inst];
+}
+
+__device__ __forceinline__ float2* rbuf(const BGroup& G, int k) { return (k & 1) ? G.R1 : G.R0; }
+__device__ __forceinline__ float2* wbuf(const BGroup& G, int k) {
+  const int q = ((k % 3) + 3) % 3;
+  return q == 0 ? G.W0 : (q == 1 ? G.W1 : G.W2);
+}
+__device__ __forceinline__ float2* zbuf(const BGroup& G, int k) { return (k & 1) ? G.Z2 : G.Y; }
+
+// Chebyshev step 0. mode 1: r = R0 (MINRES start); mode 2: r_new = u - (alfa/beta) r2.
+__device__ void ph_cheb_first(const BatchArgs& a, int64_t vb, int mode) {
+  const Lane L = lane_ctx(a, vb);
+  double s2 = 0.0;
+  const MinresScalars* S = L.valid ? &a.S[L.inst] : nullptr;
+  if (L.valid && !S->done) {
+    const BGroup& G = *L.G;
+    const int it = mode == 2 ? S->it : 0;
+    float2* r = rbuf(G, it);
+    float2 rv;
+    if (mode == 2) {
+      const float2* r2 = rbuf(G, it + 1);
+      const float c = float(S->alfa / S->beta);
+      const float2 uu = G.u[L.idx], r2v = r2[L.idx];
+      rv = make_float2(uu.x - c * r2v.x, uu.y - c * r2v.y);
+      r[L.idx] = rv;
+    } else {
+      rv = r[L.idx];
+    }
+    const int64_t nb3 = G.n * G.bp;
+    float2 z;
+    if (a.iters == 0) {
+      z = rv;
+    } else {
+      const double hi = hi_of(a, L.inst);
+      const float th = float(0.5 * (hi + hi / a.ratio));
+      float o0, o1;
+      dinv<float>(G.pblk[L.idx], G.pblk[nb3 + L.idx], G.pblk[2 * nb3 + L.idx], rv.x, rv.y, &o0,
+                  &o1);
+      z = make_float2(o0 / th, o1 / th);
+      if (a.iters > 1) G.D[L.idx] = z;
+    }
+    zbuf(G, a.z0)[L.idx] = z;
+    s2 = double(rv.x) * double(z.x) + double(rv.y) * double(z.y);
+  }
+  if (a.iters <= 1) lane_partials(a, L, vb, s2);
+}
+
+__device__ void ph_cheb_step(const BatchArgs& a, int64_t vb, int mode, int j) {
+  const Lane L = lane_ctx(a, vb);
+  double s2 = 0.0;
+  const bool last = (j == a.iters - 1);
+  const MinresScalars* S = L.valid ? &a.S[L.inst] : nullptr;
+  if (L.valid && !S->done) {
+    const BGroup& G = *L.G;
+    const int it = mode == 2 ? S->it : 0;
+    const float2* r = rbuf(G, it);
+    const float2* zin = zbuf(G, a.z0 + j - 1);
+    float2* zout = zbuf(G, a.z0 + j);
+    const Cheb c = cheb_coef(hi_of(a, L.inst), a.ratio, j);
+    const float2 acc = brow<float>(G, L.row, L.b, float(a.mu[L.inst]), zin);
+    const float2 zi = zin[L.idx];
+    const int64_t nb3 = G.n * G.bp;
+    const float b0 = G.pblk[L.idx], b1 = G.pblk[nb3 + L.idx], b2 = G.pblk[2 * nb3 + L.idx];
+    const float2 rv = r[L.idx];
+    const float res0 = rv.x - (acc.x + b0 * zi.x + b2 * zi.y);
+    const float res1 = rv.y - (acc.y + b2 * zi.x + b1 * zi.y);
+    float u0, u1;
+    dinv<float>(b0, b1, b2, res0, res1, &u0, &u1);
+    float2 d = G.D[L.idx];
+    d.x = float(c.c1) * d.x + float(c.c2) * u0;
+    d.y = float(c.c1) * d.y + float(c.c2) * u1;
+    const float2 z = make_float2(zi.x + d.x, zi.y + d.y);
+    if (!last) G.D[L.idx] = d;
+    zout[L.idx] = z;
+    s2 = double(rv.x) * double(z.x) + double(rv.y) * double(z.y);
+  }
+  if (last) lane_partials(a, L, vb, s2);
+}
+
+__device__ __forceinline__ void w_update(const BGroup& G, const MinresScalars* S, int k,
+                                         int64_t idx, float2 vk) {
+  float2* wk = wbuf(G, k);
+  const float2 w1 = wbuf(G, k - 2)[idx], w2 = wbuf(G, k - 1)[idx];
+  const float oe = float(S->p_oldeps), de = float(S->p_delta), dn = float(S->p_denom);
+  const float2 w = make_float2((vk.x - oe * w1.x - de * w2.x) * dn,
+                               (vk.y - oe * w1.y - de * w2.y) * dn);
+  wk[idx] = w;
+  double2 xv = G.x[idx];
+  xv.x += S->p_phi * double(w.x);
+  xv.y += S->p_phi * double(w.y);
+  G.x[idx] = xv;
+}
+
+// v = y / beta, u = J v - (beta/beta_old) r1, previous w/x update, <v,u> partials.
+__device__ void ph_minres_a(const BatchArgs& a, int64_t vb) {
+  const Lane L = lane_ctx(a, vb);
+  double s = 0.0;
+  const MinresScalars* S = L.valid ? &a.S[L.inst] : nullptr;
+  if (L.valid && !S->done) {
+    const BGroup& G = *L.G;
+    const int it = S->it;
+    const double beta = S->beta;
+    const float2 vold = G.v[L.idx];
+    if (S->pending) w_update(G, S, it, L.idx, vold);
+    const float2* y = zbuf(G, 0);
+    const float2 acc = brow<float>(G, L.row, L.b, float(a.mu[L.inst]), y);
+    const float2 yi = y[L.idx];
+    const int64_t nb3 = G.n * G.bp;
+    const float b0 = G.jblk[L.idx], b1 = G.jblk[nb3 + L.idx], b2 = G.jblk[2 * nb3 + L.idx];
+    const float ib = float(1.0 / beta);
+    float2 uu = make_float2((acc.x + b0 * yi.x + b2 * yi.y) * ib,
+                            (acc.y + b2 * yi.x + b1 * yi.y) * ib);
+    if (it >= 1) {
+      const float2 rv = rbuf(G, it)[L.idx];
+      const float c = float(beta / S->oldb);
+      uu.x -= c * rv.x;
+      uu.y -= c * rv.y;
+    }
+    const float2 vn = make_float2(yi.x * ib, yi.y * ib);
+    G.v[L.idx] = vn;
+    G.u[L.idx] = uu;
+    s = double(vn.x) * double(uu.x) + double(vn.y) * double(uu.y);
+  }
+  lane_partials(a, L, vb, s);
+}
+
+__device__ void ph_flush(const BatchArgs& a, int64_t vb) {
+  const Lane L = lane_ctx(a, vb);
+  if (!L.valid || !a.active[L.inst]) return;
+  const BGroup& G = *L.G;
+  const MinresScalars* S = &a.S[L.inst];
+  if (S->pending) w_update(G, S, S->it, L.idx, G.v[L.idx]);
+  const double2 d = G.x[L.idx];
+  double2 p = G.psi[L.idx];
+  p.x += d.x;
+  p.y += d.y;
+  G.psi[L.idx] = p;
+}
+
+// Per-instance finalisation (block = instance): 0 norm^2 -> out, 1 init,
+// 2 alfa, 3 Givens.
+__device__ void ph_finalize(const BatchArgs& a, int inst, int what, double* out) {
+  MinresScalars* S = &a.S[inst];
+  if ((what == 2 || what == 3) && S->done) return;
+  int gi = 0;
+  while (gi + 1 < a.ngroups && inst >= a.groups[gi + 1].inst0) ++gi;
+  const int nb = int(a.groups[gi].nblocks);
+  double v[1];
+  finalize_sum<1>(v, a.partials + int64_t(inst) * a.maxblocks, nb);
+  if (threadIdx.x == 0) {
+    if (what == 0)
+      out[inst] = v[0];
+    else if (what == 1)
+      init_update(S, v[0]);
+    else if (what == 2) {
+      S->alfa = v[0];
+      S->pending = 0;
+    } else
+      givens_update(S, v[0]);
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------- kernels
+__global__ void kb_compute_f(BatchArgs a) { ph_compute_f(a, blockIdx.x); }
+__global__ void kb_rebuild(BatchArgs a) { ph_rebuild(a, blockIdx.x); }
+__global__ void kb_setup(BatchArgs a) { ph_setup(a, blockIdx.x); }
+__global__ void kb_cheb_first(BatchArgs a, int mode) { ph_cheb_first(a, blockIdx.x, mode); }
+__global__ void kb_cheb_step(BatchArgs a, int mode, int j) {
+  ph_cheb_step(a, blockIdx.x, mode, j);
+}
+__global__ void kb_minres_a(BatchArgs a) { ph_minres_a(a, blockIdx.x); }
+__global__ void kb_flush(BatchArgs a) { ph_flush(a, blockIdx.x); }
+__global__ void kb_finalize(BatchArgs a, int what, double* out) {
+  ph_finalize(a, blockIdx.x, what, out);
+}
+
+// Per-instance MINRES state reset: active instances start, others are done.
+__global__ void kb_reset(BatchArgs a, int maxit) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.ninst) return;
+  MinresScalars* S = &a.S[i];
+  S->done = a.active[i] ? 0 : 1;
+  S->it = 0;
+  S->pending = 0;
+  S->rtol = a.rtol[i];
+  S->maxit = maxit;
+  S->relres = 1.0;
+  a.bound[i] = 0.0;
+}
+
+// Persistent MINRES: all iterations of the batch in one cooperative launch;
+// grid-wide barriers separate the phases; the loop ends when every instance
+// is done or after maxit iterations, a decision every block takes from the
+// same post-barrier state.
+__global__ void kb_minres_persistent(BatchArgs a, int maxit) {
+  cg::grid_group grid = cg::this_grid();
+  const int64_t nvb = a.total_blocks;
+  for (int it = 0; it < maxit; ++it) {
+    for (int64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) ph_minres_a(a, vb);
+    grid.sync();
+    for (int inst = blockIdx.x; inst < a.ninst; inst += gridDim.x)
+      ph_finalize(a, inst, 2, nullptr);
+    grid.sync();
+    for (int64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) ph_cheb_first(a, vb, 2);
+    grid.sync();
+    for (int j = 1; j < a.iters; ++j) {
+      for (int64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) ph_cheb_step(a, vb, 2, j);
+      grid.sync();
+    }
+    for (int inst = blockIdx.x; inst < a.ninst; inst += gridDim.x)
+      ph_finalize(a, inst, 3, nullptr);
+    grid.sync();
+    int all_done = 1;
+    for (int i = 0; i < a.ninst; ++i) all_done &= a.S[i].done;
+    if (all_done) break;
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+void Batch::launch_phases_iteration(hipStream_t s) {
+  const unsigned nb = unsigned(args.total_blocks), bt = unsigned(block_threads);
+  kb_minres_a<<<nb, bt, 0, s>>>(args);
+  kb_finalize<<<args.ninst, 256, 0, s>>>(args, 2, nullptr);
+  kb_cheb_first<<<nb, bt, 0, s>>>(args, 2);
+  for (int j = 1; j < args.iters; ++j) kb_cheb_step<<<nb, bt, 0, s>>>(args, 2, j);
+  kb_finalize<<<args.ninst, 256, 0, s>>>(args, 3, nullptr);
+}
+
+void Batch::build_graph(hipStream_t s) {
+  if (graph_exec) return;
+  NOSH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  for (int q = 0; q < graph_iters; ++q) launch_phases_iteration(s);
+  hipGraph_t g = nullptr;
+  NOSH_HIP(hipStreamEndCapture(s, &g));
+  NOSH_HIP(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0));
+  NOSH_HIP(hipGraphDestroy(g));
+}
+
+void Batch::drop_graph() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  graph_exec = nullptr;
+}
+
+int Batch::persistent_grid() {
+  int per_cu = 0;
+  NOSH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void*>(kb_minres_persistent), block_threads, 0));
+  int dev = 0, cus = 0;
+  NOSH_HIP(hipGetDevice(&dev));
+  NOSH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int64_t cap = int64_t(std::max(per_cu, 1)) * cus;
+  return int(std::min<int64_t>(cap, args.total_blocks));
+}
+
+int Batch::minres(hipStream_t s, int maxit) {
+  NOSH_CHECK(mode == 0 || args.total_blocks <= kPersistentMaxBlocks, kUnsupported,
+             "persistent grid too large");
+  const unsigned nb = unsigned(args.total_blocks), bt = unsigned(block_threads);
+  kb_reset<<<grid_for(args.ninst, 128), 128, 0, s>>>(args, maxit);
+  kb_rebuild<<<nb, bt, 0, s>>>(args);
+  kb_setup<<<nb, bt, 0, s>>>(args);
+  // y = M^{-1} r0 and beta1 for every instance.
+  kb_cheb_first<<<nb, bt, 0, s>>>(args, 1);
+  for (int j = 1; j < args.iters; ++j) kb_cheb_step<<<nb, bt, 0, s>>>(args, 1, j);
+  kb_finalize<<<args.ninst, 256, 0, s>>>(args, 1, nullptr);
+  int issued = 0;
+  if (mode == 1) {
+    void* kargs[] = {&args, &maxit};
+    NOSH_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kb_minres_persistent),
+                                        dim3(persistent_grid()), dim3(bt), kargs, 0, s));
+    issued = maxit;
+  } else {
+    build_graph(s);
+    for (;;) {
+      for (int q = 0; q < poll; q += graph_iters, issued += graph_iters)
+        NOSH_HIP(hipGraphLaunch(graph_exec, s));
+      NOSH_HIP(hipMemcpyAsync(Sh.data(), args.S, sizeof(MinresScalars) * args.ninst,
+                              hipMemcpyDeviceToHost, s));
+      NOSH_HIP(hipStreamSynchronize(s));
+      bool all = true;
+      for (int i = 0; i < args.ninst; ++i) all = all && Sh[i].done;
+      if (all || issued >= maxit) break;
+    }
+  }
+  kb_flush<<<nb, bt, 0, s>>>(args);
+  NOSH_HIP(hipMemcpyAsync(Sh.data(), args.S, sizeof(MinresScalars) * args.ninst,
+                          hipMemcpyDeviceToHost, s));
+  NOSH_HIP(hipStreamSynchronize(s));
+  int total = 0;
+  for (int i = 0; i < args.ninst; ++i)
+    if (active_h[i]) total = std::max(total, Sh[i].it);
+  (void)issued;
+  return total;
+}
+
+void Batch::compute_norms(hipStream_t s, std::vector<double>& nrm) {
+  const unsigned nb = unsigned(args.total_blocks), bt = unsigned(block_threads);
+  kb_compute_f<<<nb, bt, 0, s>>>(args);
+  kb_finalize<<<args.ninst, 256, 0, s>>>(args, 0, norms.p);
+  nrm.resize(args.ninst);
+  NOSH_HIP(hipMemcpyAsync(nrm.data(), norms.p, sizeof(double) * args.ninst,
+                          hipMemcpyDeviceToHost, s));
+  NOSH_HIP(hipStreamSynchronize(s));
+  for (double& v : nrm) v = std::sqrt(v);
+}
+
+void Batch::relayout(int threads) {
+  block_threads = threads;
+  int64_t b0 = 0, maxb = 0;
+  for (auto& gh : ghost) {
+    BGroup& G = gh.dev;
+    G.block0 = b0;
+    const int rows = threads / G.bp;
+    G.nblocks = (G.n + rows - 1) / rows;
+    b0 += G.nblocks;
+    maxb = std::max(maxb, G.nblocks);
+  }
+  args.total_blocks = b0;
+  args.maxblocks = maxb;
+  partials.alloc(size_t(args.ninst) * maxb + 64);
+  args.partials = partials.p;
+  std::vector<BGroup> gv;
+  for (auto& gh : ghost) gv.push_back(gh.dev);
+  groups.alloc(gv.size());
+  NOSH_HIP(hipMemcpy(groups.p, gv.data(), sizeof(BGroup) * gv.size(), hipMemcpyHostToDevice));
+  args.groups = groups.p;
+  drop_graph();
+}
+
+Batch::~Batch() { drop_graph(); }
+
+}  // namespace nosh

@@ -1,0 +1,112 @@
+// batched.hpp -- device records and host driver of the batched solver.
+#pragma once
+
+#include <vector>
+
+#include "internal.hpp"
+#include "solver.hpp"
+
+namespace nosh {
+
+// One mesh and the instances solved on it (device-side record).
+struct BGroup {
+  int64_t n;        // rows (vertices)
+  int bp;           // lanes per row: instances padded to a power of two
+  int B;            // instances
+  int inst0;        // first global instance index
+  int64_t block0;   // first block of this group in a batched launch
+  int64_t nblocks;  // blocks of this group
+  const int32_t* slice_ptr;
+  const int32_t* scol;
+  const float* a32;   // per padded slot: alpha (0 for padding)
+  const float* t32;   //                  a~ signed by direction
+  const double* a64;
+  const double* t64;
+  const double* diag;    // per row: sum alpha
+  const double* rowabs;  // per row: sum |alpha|
+  const double* ct;      // per row: control volume * thickness
+  const double* V;       // per row: scalar potential
+  double2* psi;          // [n][bp], FP64 state
+  double2* f;            // [n][bp], F(psi)
+  double2* x;            // [n][bp], MINRES solution accumulator
+  float* jblk;           // 3 [n][bp]
+  float* pblk;           // 3 [n][bp]
+  float2 *Y, *Z2, *v, *u, *R0, *R1, *D, *W0, *W1, *W2;  // [n][bp] FP32 Krylov vectors
+};
+
+struct BatchArgs {
+  const BGroup* groups;
+  int ngroups;
+  int ninst;
+  int64_t total_blocks;
+  int64_t maxblocks;
+  MinresScalars* S;     // per instance
+  const double* mu;     // per instance
+  const double* g;      // per instance
+  double* bound;        // per instance: max ratio for the Chebyshev interval
+  const double* rtol;   // per instance MINRES tolerance
+  const int* active;    // per instance: 1 while Newton runs
+  double upper, ratio;
+  int iters;            // Chebyshev iterations
+  int z0;               // parity of the first Chebyshev output buffer
+  double* partials;     // [instance][maxblocks]
+};
+
+struct GroupHost {
+  MeshCache* mesh = nullptr;
+  int B = 0;
+  DBuf<float> a32, t32;
+  DBuf<double> a64, t64, diag, rowabs, ct, V;
+  DBuf<double2> psi, f, x;
+  DBuf<float> jblk, pblk;
+  DBuf<float2> Y, Z2, v, u, R0, R1, D, W0, W1, W2;
+  BGroup dev{};
+};
+
+// Largest persistent (cooperative) grid accepted. The persistent kernel is
+// correct whenever its whole grid is co-resident; on gfx950 that has been
+// established for grids of up to 64 workgroups, while a 3584-workgroup
+// launch (128 threads, 14 per CU by the occupancy query) never finished, for
+// a reason not yet identified. Larger grids are refused; the per-phase
+// (hipGraph) mode serves them.
+constexpr int64_t kPersistentMaxBlocks = 64;
+
+struct Batch {
+  std::vector<GroupHost> ghost;
+  DBuf<BGroup> groups;
+  BatchArgs args{};
+  DBuf<MinresScalars> S;
+  std::vector<MinresScalars> Sh;
+  DBuf<double> mu, g, bound, rtol, norms, partials;
+  DBuf<int> active;
+  std::vector<int> active_h;
+  int block_threads = 256;
+  int mode = 0;         // 0: hipGraph-captured iterations, 1: persistent cooperative kernel
+  int graph_iters = 4;  // MINRES iterations per captured graph
+  int poll = 16;        // iterations between host checks of the done flags
+  hipGraphExec_t graph_exec = nullptr;
+
+  void relayout(int threads);
+  void launch_phases_iteration(hipStream_t s);
+  void build_graph(hipStream_t s);
+  void drop_graph();
+  int persistent_grid();
+  // Rebuilds blocks at psi, solves J dx = -F for the active instances and
+  // applies psi += dx; returns the largest MINRES iteration count.
+  int minres(hipStream_t s, int maxit);
+  void compute_norms(hipStream_t s, std::vector<double>& nrm);
+  ~Batch();
+};
+
+// Per-slot (alpha, signed a~) in FP32 and FP64 plus row sums, from the edge
+// weights of the mesh (operators.hip).
+void launch_slot_fields(const MeshCache& m, const double* alpha, const double* atilde,
+                        float* a32, float* t32, double* a64, double* t64, double* diag,
+                        double* rowabs, hipStream_t s);
+// Reference-order instance vector -> lane b of an interleaved [n][bp] array, and back.
+void launch_scatter_lane(int64_t n, int bp, int b, const int32_t* perm, const double* src,
+                         double* dst, hipStream_t s);
+void launch_gather_lane(int64_t n, int bp, int b, const int32_t* perm, const double* src,
+                        double* dst, hipStream_t s);
+
+}  // namespace nosh

@@ -22,9 +22,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/nosh_amd.h"
-#include "internal.hpp"
-#include "solver.hpp"
+#include "handles.hpp"
 
 namespace nosh {
 
@@ -73,90 +71,6 @@ const double* ScalarField::get_v(const std::map<std::string, double>& p, hipStre
 
 using namespace nosh;
 
-struct nosh_mesh_s {
-  std::unique_ptr<MeshCache> m;
-};
-
-struct nosh_vfield_s {
-  nosh_mesh_s* mesh;
-  VectorField vf;
-};
-
-struct nosh_sfield_s {
-  ScalarField sf;
-  // Reference-order copy of the values at the default parameters (for alpha).
-  DBuf<double> ref;
-};
-
-// Device staging for host-pointer applies.
-struct Staging {
-  DBuf<double> a, b, c, d;
-  void ensure(int64_t m) {
-    if (a.n != size_t(m)) {
-      a.alloc(m);
-      b.alloc(m);
-      c.alloc(m);
-      d.alloc(m);
-    }
-  }
-};
-
-struct nosh_keo_s {
-  nosh_mesh_s* mesh;
-  nosh_sfield_s* thickness;
-  nosh_vfield_s* mvp;
-  bool dkeo = false;
-  std::string dparam;
-  DBuf<double> alpha;  // alpha cache (built on first fill)
-  bool alpha_valid = false;
-  KeoValues kv;
-  Staging st;
-  void set_parameters(const std::map<std::string, double>& p, hipStream_t s);
-};
-
-struct nosh_jac_s {
-  nosh_mesh_s* mesh;
-  nosh_sfield_s* V;
-  nosh_sfield_s* thickness;
-  nosh_keo_s* keo;
-  DBuf<double> ct, blk, psi;
-  DBuf<float> blk32;
-  bool built = false;
-  Staging st;
-};
-
-struct nosh_prec_s {
-  nosh_mesh_s* mesh;
-  nosh_sfield_s* thickness;
-  std::unique_ptr<nosh_keo_s> keo;
-  DBuf<double> ct, blk, psi, bound;
-  int iters = 6;
-  double ratio = 300.0, upper = 0.0;
-  bool built = false;
-  MinresWork<double> work;
-  Staging st;
-};
-
-struct nosh_nls_s {
-  nosh_mesh_s* mesh;
-  nosh_vfield_s* mvp;
-  nosh_sfield_s* V;
-  nosh_sfield_s* thickness;
-  double g;
-  std::vector<double> initial_x;
-  std::string deriv;
-  std::unique_ptr<nosh_keo_s> keo, dkeo;
-  std::vector<std::string> p_names;
-  std::vector<double> p_nominal;
-  DBuf<double> ct, x, f, nrm2;
-  Staging st;
-  // Newton work.
-  DBuf<double> jblk, pblk, bound;
-  DBuf<float> jblk32, pblk32;
-  MinresWork<double> w64;
-  MinresWork<float> w32;
-};
-
 namespace {
 
 thread_local std::string g_err;
@@ -191,14 +105,6 @@ std::map<std::string, double> to_map(int np, const char* const* names, const dou
     m[names[i]] = values[i];
   }
   return m;
-}
-
-// params.at(key) of the reference: std::out_of_range when absent.
-double need(const std::map<std::string, double>& p, const char* key) {
-  auto it = p.find(key);
-  if (it == p.end())
-    throw nosh::Error(kMissingParam, std::string("parameter '") + key + "' not found (map::at)");
-  return it->second;
 }
 
 void check_apply_mode(int mode, double alpha, double beta) {
@@ -267,13 +173,27 @@ const double* thickness_ref(nosh_sfield_s* t, hipStream_t s) {
   return t->ref.p;
 }
 
+}  // namespace
+
+namespace nosh {
+
+void set_last_error(const std::string& msg) { g_err = msg; }
+
+// params.at(key) of the reference: std::out_of_range when absent.
+double need(const std::map<std::string, double>& p, const char* key) {
+  auto it = p.find(key);
+  if (it == p.end())
+    throw nosh::Error(kMissingParam, std::string("parameter '") + key + "' not found (map::at)");
+  return it->second;
+}
+
 void compute_ct(MeshCache& m, nosh_sfield_s* t, DBuf<double>& ct,
                 const std::map<std::string, double>& p, hipStream_t s) {
   if (ct.n != size_t(m.n)) ct.alloc(m.n);
   launch_mul(m.cv_int.p, t->sf.get_v(p, s), ct.p, m.n, s);
 }
 
-}  // namespace
+}  // namespace nosh
 
 void nosh_keo_s::set_parameters(const std::map<std::string, double>& p, hipStream_t s) {
   MeshCache& m = *mesh->m;
@@ -892,7 +812,9 @@ int nosh_newton_default_options(nosh_newton_options* o) {
   });
 }
 
-static void newton_run(nosh_nls_s* nls, double* psi_int, const std::map<std::string, double>& p,
+}  // extern "C"
+
+void nosh::newton_run(nosh_nls_s* nls, double* psi_int, const std::map<std::string, double>& p,
                        const nosh_newton_options& o, nosh_newton_stats* st, hipStream_t s) {
   MeshCache& m = *nls->mesh->m;
   const int64_t n = m.n;
@@ -974,6 +896,8 @@ static void newton_run(nosh_nls_s* nls, double* psi_int, const std::map<std::str
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (st) *st = local;
 }
+
+extern "C" {
 
 int nosh_newton_solve(nosh_nls_t nls, double* psi, int np, const char* const* names,
                       const double* values, const nosh_newton_options* opts,

@@ -137,3 +137,11 @@ void launch_block_apply32(const MeshCache& m, const KeoValues& kv, const float* 
                           const float* x, float* y, hipStream_t s);
 
 }  // namespace nosh
+
+namespace nosh {
+// Control-volume weighted functionals of the model (internal order):
+// b == nullptr: -sum c |a|^4 / sum c (Gibbs energy); else sum c Re(conj(a) b) / sum c.
+// partials: >= 2 * ceil(n / 256) doubles.
+void launch_cv_weighted(const MeshCache& m, const double* a, const double* b, double* partials,
+                        double* out, hipStream_t s);
+}  // namespace nosh

@@ -1,0 +1,42 @@
+// mlp.hpp -- per-vertex MLP surrogate (mlp.hip).
+#pragma once
+
+#include <vector>
+
+#include "internal.hpp"
+
+namespace nosh {
+
+struct Mlp {
+  int nin = 0, width = 0, depth = 0, nout = 0;
+  std::vector<int> dims;           // nin, width x depth, nout
+  std::vector<int64_t> woff, boff; // per layer offsets into the parameter vector
+  int64_t nweights = 0, nparams = 0;
+  int steps = 0;
+  // Parameters: weights (layer-major, each [out][in]) then biases; FP32
+  // masters, Adam moments and gradients; BF16 copy of the weights.
+  DBuf<float> master, grad, m1, m2;
+  DBuf<__bf16> wbf;
+  // Activations of the last forward pass (capacity cap rows).
+  int64_t cap = 0;
+  std::vector<DBuf<float>> zbuf;   // pre-activations per hidden layer [M][width]
+  std::vector<DBuf<__bf16>> hbuf;  // activations per hidden layer
+  DBuf<__bf16> dzb, dza;
+  DBuf<float> y, dy, red;
+  DBuf<double> partials, loss;
+
+  void init(int nin, int width, int depth, int nout, uint64_t seed);
+  void ensure(int64_t M);
+  const float* bias(int l) const;
+  void gemm(const __bf16* A, int64_t sam, int64_t sak, const __bf16* B, int64_t sbk, int64_t sbn,
+            int64_t M, int64_t N, int64_t K, int epi, const float* bias, const float* zin,
+            float* outf, __bf16* outb, int splits, hipStream_t s);
+  void forward(const float* X, int64_t M, float* Y, hipStream_t s);
+  // Fills grad (same layout as the parameters) for the mean squared error
+  // of the outputs against T; returns the loss.
+  double gradients(const float* X, const float* T, int64_t M, hipStream_t s);
+  void adam(float lr, hipStream_t s);
+  void sync_bf16(hipStream_t s);
+};
+
+}  // namespace nosh

@@ -370,3 +370,100 @@ void launch_block_apply32(const MeshCache& m, const KeoValues& kv, const float* 
 }
 
 }  // namespace nosh
+
+namespace nosh {
+namespace {
+// Partial sums of (sum c |psi|^4, sum c) or (sum c Re(conj(phi) psi), sum c).
+__global__ void k_energy_partials(int64_t n, const double* __restrict__ cv,
+                                  const double2* __restrict__ a, const double2* __restrict__ b,
+                                  double* __restrict__ partials) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < n) {
+    const double c = cv[i];
+    const double2 x = a[i];
+    if (b) {
+      const double2 y = b[i];
+      v[0] = c * (x.x * y.x + x.y * y.y);
+    } else {
+      const double r2 = x.x * x.x + x.y * x.y;
+      v[0] = c * r2 * r2;
+    }
+    v[1] = c;
+  }
+  block_partials<2>(v, partials);
+}
+
+__global__ void k_energy_finalize(const double* __restrict__ partials, int nb, double sign,
+                                  double* __restrict__ out) {
+  double v[2];
+  finalize_sum<2>(v, partials, nb);
+  if (threadIdx.x == 0) *out = sign * v[0] / v[1];
+}
+}  // namespace
+
+void launch_cv_weighted(const MeshCache& m, const double* a, const double* b, double* partials,
+                        double* out, hipStream_t s) {
+  const unsigned nb = grid_for(m.n, kBlock);
+  k_energy_partials<<<nb, kBlock, 0, s>>>(m.n, m.cv_int.p, reinterpret_cast<const double2*>(a),
+                                          reinterpret_cast<const double2*>(b), partials);
+  k_energy_finalize<<<1, kFinalizeThreads, 0, s>>>(partials, int(nb), b ? 1.0 : -1.0, out);
+}
+
+}  // namespace nosh
+
+namespace nosh {
+namespace {
+__global__ void k_slot_fields(const int32_t* __restrict__ sedge, const double* __restrict__ alpha,
+                              const double* __restrict__ atilde, int64_t nnz,
+                              float* __restrict__ a32, float* __restrict__ t32,
+                              double* __restrict__ a64, double* __restrict__ t64) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= nnz) return;
+  const int32_t code = sedge[j];
+  double al = 0.0, at = 0.0;
+  if (code >= 0) {
+    al = alpha[code >> 1];
+    at = (code & 1) ? -atilde[code >> 1] : atilde[code >> 1];
+  }
+  a32[j] = float(al);
+  t32[j] = float(at);
+  a64[j] = al;
+  t64[j] = at;
+}
+
+__global__ void k_scatter_lane(int64_t n, int bp, int b, const int32_t* __restrict__ perm,
+                               const double2* __restrict__ src, double2* __restrict__ dst) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i * bp + b] = src[perm[i]];
+}
+
+__global__ void k_gather_lane(int64_t n, int bp, int b, const int32_t* __restrict__ perm,
+                              const double2* __restrict__ src, double2* __restrict__ dst) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[perm[i]] = src[i * bp + b];
+}
+}  // namespace
+
+void launch_slot_fields(const MeshCache& m, const double* alpha, const double* atilde,
+                        float* a32, float* t32, double* a64, double* t64, double* diag,
+                        double* rowabs, hipStream_t s) {
+  k_slot_fields<<<grid_for(m.nnz_pad, kBlock), kBlock, 0, s>>>(m.sedge.p, alpha, atilde,
+                                                               m.nnz_pad, a32, t32, a64, t64);
+  k_keo_diag<<<grid_for(m.n, kBlock), kBlock, 0, s>>>(m.slice_ptr.p, m.sedge.p, alpha, m.n, diag,
+                                                      rowabs);
+}
+
+void launch_scatter_lane(int64_t n, int bp, int b, const int32_t* perm, const double* src,
+                         double* dst, hipStream_t s) {
+  k_scatter_lane<<<grid_for(n, kBlock), kBlock, 0, s>>>(
+      n, bp, b, perm, reinterpret_cast<const double2*>(src), reinterpret_cast<double2*>(dst));
+}
+
+void launch_gather_lane(int64_t n, int bp, int b, const int32_t* perm, const double* src,
+                        double* dst, hipStream_t s) {
+  k_gather_lane<<<grid_for(n, kBlock), kBlock, 0, s>>>(
+      n, bp, b, perm, reinterpret_cast<const double2*>(src), reinterpret_cast<double2*>(dst));
+}
+
+}  // namespace nosh

@@ -1,0 +1,104 @@
+"""GPU: the batched solver (uneven meshes, per-instance mu/g, FP32 inner
+MINRES) — every instance's solution checked with the oracle."""
+import numpy as np
+import pytest
+
+from nosh_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(coords, cells):
+    import nosh_amd as nosh
+    mesh = nosh.Mesh(coords, cells)
+    mvp = nosh.VectorFieldExplicitValues(mesh, mg.magnetic_potential(coords), 0.0)
+    V = nosh.ScalarFieldConstant(mesh, -1.0)
+    t = nosh.ScalarFieldConstant(mesh, 1.0)
+    return nosh.Nls(mesh, mvp, V, 1.0, t)
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import nosh_amd as nosh
+    meshes = [mg.kuhn_cube(6, jitter=0.2, seed=2), mg.square_tri(16), mg.kuhn_cube(4)]
+    counts = [3, 5, 2]
+    models = [_model(c, t) for c, t in meshes]
+    u = mg.splitmix64_uniform(2, 2 * sum(counts)).reshape(-1, 2)
+    mu = u[:, 0]
+    g = 0.5 + 1.5 * u[:, 1]
+    # Starts near the superconducting branch (psi = 1 plus 10 % seeded noise):
+    # Newton from fully random phases on these coarse meshes is chaotic, and
+    # these tests are about the batching, not Newton's global behaviour.
+    psi0 = []
+    k = 0
+    for (c, t), cnt in zip(meshes, counts):
+        for _ in range(cnt):
+            p = 0.1 * mg.random_state(c.shape[0], 100 + k)
+            p[0::2] += 1.0
+            psi0.append(p)
+            k += 1
+    return meshes, counts, models, mu, g, np.concatenate(psi0)
+
+
+def _check_with_oracle(meshes, counts, mu, g, sols, tol=1e-9):
+    from oracle import oracle as O
+    k = 0
+    for (c, t), cnt in zip(meshes, counts):
+        om = O.Model(O.Mesh(c, t), mg.magnetic_potential(c))
+        for _ in range(cnt):
+            om.set_params(g[k], mu[k])
+            assert np.linalg.norm(om.f(sols[k])) <= tol, k
+            k += 1
+
+
+@pytest.mark.parametrize("threads", [128, 256])
+def test_batched_graph_and_persistent_agree(setup, threads):
+    import nosh_amd as nosh
+    meshes, counts, models, mu, g, psi0 = setup
+    batch = nosh.Batch(list(zip(models, counts)), mu, g)
+    # Random starts: Newton needs 20-50 steps; with inner MINRES to 1e-6 the
+    # batched counts match the single-instance solver's.
+    opts = dict(tol=1e-10, max_iters=80, minres_rtol=1e-6, minres_max_iters=4000, cheb_iters=4,
+                cheb_ratio=100.0, forcing=0)
+    out = {}
+    for mode in (0, 1):
+        print(f"[batched threads={threads} mode={mode}]", flush=True)
+        batch.set_tuning(threads, mode, 4, 16)
+        psi, st = batch.solve(psi0, **opts)
+        assert all(s["converged"] for s in st), st
+        assert max(s["final_norm_f"] for s in st) <= 1e-10
+        _check_with_oracle(meshes, counts, mu, g, batch.split(psi))
+        out[mode] = psi
+    # Same phases, same deterministic reductions: bitwise identical.
+    assert np.array_equal(out[0], out[1])
+
+
+def test_batched_autotune(setup, tmp_path, monkeypatch):
+    import nosh_amd as nosh
+    monkeypatch.setenv("NOSH_AUTOTUNE_CACHE", str(tmp_path / "tune.tsv"))
+    meshes, counts, models, mu, g, psi0 = setup
+    batch = nosh.Batch(list(zip(models, counts)), mu, g)
+    t, m = batch.autotune(psi0, cheb_iters=4, cheb_ratio=100.0)
+    assert t in (128, 256, 512, 1024) and m in (0, 1)
+    assert (tmp_path / "tune.tsv").read_text().strip().endswith(f"{t} {m}")
+    t2, m2 = batch.autotune(psi0, cheb_iters=4, cheb_ratio=100.0)  # from the cache
+    assert (t2, m2) == (t, m)
+    psi, st = batch.solve(psi0, tol=1e-10, max_iters=80, minres_rtol=1e-6, cheb_iters=4,
+                          cheb_ratio=100.0, forcing=0)
+    assert all(s["converged"] for s in st)
+    _check_with_oracle(meshes, counts, mu, g, batch.split(psi))
+
+
+def test_persistent_mode_refuses_large_grids():
+    """Persistent (cooperative) mode accepts only grids of at most 64
+    workgroups; a larger batch is refused with an error and stays in graph
+    mode, which then solves it."""
+    import nosh_amd as nosh
+    c, t = mg.kuhn_cube(12)
+    m = _model(c, t)
+    batch = nosh.Batch([(m, 8)], np.linspace(0.1, 0.5, 8), np.ones(8))
+    with pytest.raises(nosh.NoshError):
+        batch.set_tuning(128, 1, 4, 16)
+    psi0 = np.tile(np.stack([np.ones(c.shape[0]), np.zeros(c.shape[0])], 1).ravel(), 8)
+    psi, st = batch.solve(psi0, tol=1e-10, max_iters=30, minres_rtol=1e-6, forcing=0)
+    assert all(s["converged"] for s in st)

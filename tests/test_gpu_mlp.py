@@ -1,0 +1,72 @@
+"""GPU: BF16 MFMA MLP kernels against the NumPy oracle (forward, gradients),
+and training reduces the loss."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as MO
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(nin=7, width=256, depth=4, nout=2, seed=11):
+    import nosh_amd as nosh
+    return nosh.Mlp(nin, width, depth, nout, seed)
+
+
+@pytest.mark.parametrize("M", [64, 1000, 4096])
+def test_forward_matches_oracle(M):
+    net = _net()
+    rng = np.random.default_rng(1)
+    X = rng.normal(0, 1, (M, 7)).astype(np.float32)
+    y = net.forward(X)
+    p = net.get_params()
+    y_bf, _, _ = MO.forward(p, X, net.dims, bf=True)
+    err = np.linalg.norm(y - y_bf) / np.linalg.norm(y_bf)
+    print("forward rel err vs bf16-emulating oracle", err)
+    assert err < 5e-3
+    y64, _, _ = MO.forward(p, X, net.dims, bf=False)
+    assert np.linalg.norm(y - y64) / np.linalg.norm(y64) < 5e-2
+
+
+def test_gradients_match_oracle():
+    net = _net()
+    rng = np.random.default_rng(2)
+    M = 3000
+    X = rng.normal(0, 1, (M, 7)).astype(np.float32)
+    T = rng.normal(0, 1, (M, 2)).astype(np.float32)
+    loss, g = net.gradients(X, T)
+    p = net.get_params()
+    loss_ref, g_ref = MO.loss_and_grad(p, X, T, net.dims, bf=True)
+    assert abs(loss - loss_ref) / loss_ref < 5e-3
+    # Layer by layer (weights, then biases).
+    dims = net.dims
+    off = 0
+    for a, b in zip(dims[:-1], dims[1:]):
+        s = slice(off, off + a * b)
+        e = np.linalg.norm(g[s] - g_ref[s]) / np.linalg.norm(g_ref[s])
+        print("weights", a, "x", b, "rel err", e)
+        assert e < 2e-2
+        off += a * b
+    for b in dims[1:]:
+        s = slice(off, off + b)
+        e = np.linalg.norm(g[s] - g_ref[s]) / max(np.linalg.norm(g_ref[s]), 1e-30)
+        assert e < 2e-2
+        off += b
+
+
+def test_training_reduces_loss():
+    import torch
+    net = _net(seed=3)
+    rng = np.random.default_rng(4)
+    M = 8192
+    X = rng.uniform(-1, 1, (M, 7)).astype(np.float32)
+    T = np.stack([np.sin(2 * X[:, 0]) * X[:, 1], X[:, 2] * X[:, 3] - 0.5 * X[:, 4]],
+                 axis=1).astype(np.float32)
+    dev = torch.device("cuda:0")
+    Xd = torch.from_numpy(X).to(dev)
+    Td = torch.from_numpy(T).to(dev)
+    losses = [net.train_step_device(Xd.data_ptr(), Td.data_ptr(), M, 1e-3) for _ in range(300)]
+    print("loss", losses[0], "->", losses[-1])
+    assert losses[-1] < 0.1 * losses[0]
