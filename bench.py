@@ -57,7 +57,17 @@ def parse():
                    help="only the J.phi timing loop and a copy kernel (for profilers)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--verbose", action="store_true")
-    p.add_argument("--workload", default="kuhn96", choices=["kuhn96", "batched"],
+    p.add_argument("--cont-branches", type=int, default=8)
+    p.add_argument("--cont-span", type=float, default=1.0)
+    p.add_argument("--cont-steps", type=int, default=50)
+    p.add_argument("--cont-train-after", type=int, default=10)
+    p.add_argument("--cont-plain-branches", type=int, default=1,
+                   help="branches run with the plain FP64 continuation for the ratio")
+    p.add_argument("--no-cont", action="store_true")
+    p.add_argument("--cont-tuning", default="",
+                   help="block_threads,fold for the batched continuation (skips the autotuner)")
+    p.add_argument("--no-surrogate", action="store_true")
+    p.add_argument("--workload", default="kuhn96", choices=["kuhn96", "batched", "cont"],
                    help="kuhn96: the headline single-instance workload; batched: 128 uneven "
                         "instances on jittered 16^3/24^3/32^3/48^3 cubes")
     p.add_argument("--instances", type=int, default=128)
@@ -165,7 +175,7 @@ def batched_workload(args):
     t0 = time.perf_counter()
     tuned = batch.autotune(psi0, use_cache=False, **kw)
     t_tune = time.perf_counter() - t0
-    say(f"autotune {t_tune:.2f}s -> block_threads {tuned[0]}, mode {tuned[1]}")
+    say(f"autotune {t_tune:.2f}s -> block_threads {tuned[0]}, mode {tuned[1]}, fold {tuned[2]}")
     dev = torch.device("cuda:0")
     x0 = torch.from_numpy(psi0).to(dev)
     x = torch.empty_like(x0)
@@ -210,7 +220,7 @@ def batched_workload(args):
            "data": "synthetic (jittered Kuhn meshes, seeded psi_0, mu, g)",
            "config": {"workload": "batched uneven solves", "instances": ninst,
                       "meshes": [f"{s}^3 jittered" for s in sizes], "counts": counts,
-                      "autotuned": {"block_threads": tuned[0],
+                      "autotuned": {"block_threads": tuned[0], "fold": tuned[2],
                                     "mode": ["hipGraph", "persistent"][tuned[1]]},
                       "autotune_s": round(t_tune, 2)},
            "converged": conv,
@@ -222,6 +232,114 @@ def batched_workload(args):
     print(json.dumps(out))
 
 
+def _phase_align(psi, ref):
+    z = psi[0::2] + 1j * psi[1::2]
+    zr = ref[0::2] + 1j * ref[1::2]
+    w = z * np.exp(-1j * np.angle(np.vdot(zr, z)))
+    out = np.empty_like(psi)
+    out[0::2], out[1::2] = w.real, w.imag
+    return out
+
+
+def continuation_phase(args, model, n, say):
+    """mu-continuation of several (mu_0, g) branches on one mesh: the
+    batched, mixed-precision, surrogate-seeded driver against the plain FP64
+    single-instance tangent continuation on a sample of the branches.
+    Branch b: g_b = 0.6 + 0.2 b, mu_0 = 0, psi_0 = 1/sqrt(g_b) (exact at
+    mu = 0); mu runs to mu_0 + span in equal steps."""
+    import nosh_amd as nosh
+    nb, span, steps = args.cont_branches, args.cont_span, args.cont_steps
+    g = 0.6 + 0.2 * np.arange(nb)
+    mu0 = np.zeros(nb)
+    psi0 = np.zeros(2 * n * nb)
+    for b in range(nb):
+        psi0[2 * n * b:2 * n * (b + 1):2] = 1.0 / np.sqrt(g[b])
+    kw = dict(tol=1e-10, max_iters=40, minres_rtol=1e-10, minres_max_iters=4000,
+              cheb_iters=args.cheb, cheb_ratio=args.ratio, forcing=1, forcing_min=1e-6,
+              forcing_max=0.1)
+    batch = nosh.Batch([(model, nb)], mu0, g)
+    t0 = time.perf_counter()
+    if args.cont_tuning:
+        bt, fold = (int(v) for v in args.cont_tuning.split(","))
+        batch.set_tuning(bt, 0, 4, 16, fold=fold)
+        tuned = (bt, 0, fold)
+    else:
+        tuned = batch.autotune(psi0, use_cache=False, **kw)
+    say(f"continuation: autotune {time.perf_counter() - t0:.2f}s -> {tuned}")
+    sur = None if args.no_surrogate else dict(
+        width=256, depth=4, train_after=args.cont_train_after, train_steps=300, batch=16384,
+        lr=1e-3, newton_budget=6)
+    t0 = time.perf_counter()
+    psi, rec, sst = batch.continuation(psi0, span, steps, surrogate=sur, **kw)
+    t_comb = time.perf_counter() - t0
+    conv = len(rec) == steps + 1 and all(r["converged"] == nb for r in rec)
+    comb_rate = nb * steps / t_comb
+    say(f"continuation: combined {t_comb:.2f}s for {nb}x{steps} solves -> {comb_rate:.3f}/s, "
+        f"converged {conv}, surrogate {sst}")
+    finals = batch.split(psi)
+    # Plain FP64 continuation of a sample of the branches, one at a time.
+    t_plain, plain_steps, diffs, plain_rec = 0.0, 0, [], []
+    for b in range(min(args.cont_plain_branches, nb)):
+        t1 = time.perf_counter()
+        p, r1 = nosh.continuation(model, psi0[2 * n * b:2 * n * (b + 1)], {"g": g[b]}, "mu",
+                                  mu0[b], mu0[b] + span, steps, predictor=1, tol=1e-10,
+                                  max_iters=40, minres_rtol=1e-10,
+                                  minres_max_iters=args.minres_maxit, cheb_iters=args.cheb,
+                                  cheb_ratio=args.ratio)
+        dt = time.perf_counter() - t1
+        t_plain += dt
+        plain_steps += steps
+        ok = len(r1) == steps + 1 and all(r["converged"] for r in r1)
+        d = float(np.linalg.norm(_phase_align(finals[b], p) - p) / np.linalg.norm(p))
+        diffs.append(d)
+        plain_rec.append({"branch": b, "seconds": round(dt, 3), "converged": ok,
+                          "newton_iters": sum(r["newton_iters"] for r in r1),
+                          "minres_iters": sum(r["minres_iters"] for r in r1),
+                          "halvings": sum(r["halvings"] for r in r1)})
+        say(f"continuation: plain branch {b}: {dt:.2f}s, {plain_rec[-1]}, rel diff {d:.2e}")
+    plain_rate = plain_steps / t_plain if t_plain > 0 else None
+    return {
+        "solves_per_s": comb_rate, "seconds": t_comb, "converged": conv,
+        "branches": nb, "steps": steps, "span": span, "g": [round(v, 3) for v in g],
+        "autotuned": {"block_threads": tuned[0], "fold": tuned[2],
+                      "mode": ["hipGraph", "persistent"][tuned[1]]},
+        "lockstep_newton_iters": sum(r["newton_iters"] for r in rec),
+        "lockstep_minres_iters": sum(r["minres_iters"] for r in rec),
+        "max_norm_f": max(r["max_norm_f"] for r in rec),
+        "seeded_branch_steps": sum(r["seeded"] for r in rec),
+        "fallback_branch_steps": sum(r["fallbacks"] for r in rec),
+        "surrogate": sst,
+        "mean_pred_error_seeded": (float(np.mean([r["pred_error"] for r in rec if r["seeded"]]))
+                                   if any(r["seeded"] for r in rec) else None),
+        "mean_secant_error_seeded": (float(np.mean([r["sec_error"] for r in rec if r["seeded"]]))
+                                     if any(r["seeded"] for r in rec) else None),
+        "per_step_seconds": [round(r["seconds"], 4) for r in rec],
+        "per_step_newton": [r["newton_iters"] for r in rec],
+        "plain_fp64_solves_per_s": plain_rate, "plain_sample": plain_rec,
+        "rel_diff_vs_plain_fp64": diffs,
+        "speedup_vs_plain_fp64": comb_rate / plain_rate if plain_rate else None,
+    }
+
+
+def cont_workload(args):
+    import torch
+    import nosh_amd as nosh
+    from nosh_amd import meshgen as mg
+
+    def say(*a):
+        print(*a, file=sys.stderr, flush=True)
+    torch.cuda.init()
+    coords, cells = mg.kuhn_cube(args.ncell)
+    n = coords.shape[0]
+    mesh = nosh.Mesh(coords, cells)
+    mvp = nosh.VectorFieldExplicitValues(mesh, mg.magnetic_potential(coords), 0.0)
+    model = nosh.Nls(mesh, mvp, nosh.ScalarFieldConstant(mesh, -1.0), 1.0,
+                     nosh.ScalarFieldConstant(mesh, 1.0))
+    say(f"cont workload: mesh {args.ncell}^3, n={n}")
+    res = continuation_phase(args, model, n, say)
+    print(json.dumps(res))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -230,6 +348,9 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         batched_workload(args)
+        return
+    if args.workload == "cont":
+        cont_workload(args)
         return
     import torch
     import nosh_amd as nosh

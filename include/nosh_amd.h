@@ -275,16 +275,20 @@ typedef struct {
 int nosh_batch_create(int ngroups, const nosh_nls_t* models, const int* counts, const double* mu,
                       const double* g, nosh_batch_t* out);
 /* block_threads: threads per block (multiple of 64 and of the lane count);
+ * fold: row tiles (block_threads / lanes rows each) per launched block, 1..64
+ * (larger folds give fewer per-block partial sums to reduce);
  * mode 0: each MINRES iteration is a captured hipGraph replayed graph_iters
- * at a time, mode 1: one persistent cooperative kernel per MINRES solve;
+ * at a time, mode 1: one persistent cooperative kernel per MINRES solve
+ * (only for grids of at most 64 launched blocks; larger ones get
+ * NOSH_ERR_UNSUPPORTED and the batch stays in mode 0);
  * poll: iterations between host checks of the done flags (mode 0). */
-int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int mode, int graph_iters,
-                          int poll);
-/* Times every (block_threads, mode) candidate on one Newton step from psi and
- * keeps the fastest; with use_cache the choice is read from / appended to
- * $NOSH_AUTOTUNE_CACHE (default ~/.cache/nosh_amd_autotune.tsv). */
+int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int fold, int mode,
+                          int graph_iters, int poll);
+/* Times every (block_threads, fold, mode) candidate on one Newton step from
+ * psi and keeps the fastest; with use_cache the choice is read from /
+ * appended to $NOSH_AUTOTUNE_CACHE (default ~/.cache/nosh_amd_autotune.tsv). */
 int nosh_batch_autotune(nosh_batch_t batch, const double* psi, const nosh_newton_options* opts,
-                        int use_cache, int* best_threads, int* best_mode);
+                        int use_cache, int* best_threads, int* best_fold, int* best_mode);
 /* psi: all instances concatenated (each 2 n_k values, reference order);
  * overwritten by the solutions. stats: one record per instance (may be NULL). */
 int nosh_batch_solve(nosh_batch_t batch, double* psi, const nosh_newton_options* opts,
@@ -401,6 +405,38 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int nparams,
                                     const nosh_surrogate_options* sopts,
                                     nosh_continuation_step* record, int* steps_done,
                                     nosh_surrogate_stats* stats);
+
+/* ---- Batched continuation: many branches stepped in mu together ----
+ * Replaces, for a set of independent (mesh, g, mu_0) branches, the LOCA
+ * natural-parameter stepper the reference is driven by
+ * (executables/nosh-cont/nosh-cont.cpp:226-290,344; step control
+ * examples/conf.xml:68-74). Instance i of `batch` runs mu from its mu_0 to
+ * mu_0 + span in `steps` equal steps; every step solves all branches with
+ * the lockstep batched Newton (FP32 inner MINRES, FP64 outer) from a secant
+ * predictor or, once trained, from the MLP surrogate guess (sopts != NULL),
+ * with per-branch fallback to the secant guess and then to psi_k. psi
+ * (host, the instances' reference-order vectors concatenated) holds the
+ * start states and receives the last accepted states. record has
+ * steps + 1 entries (entry 0: the solve at mu_0). Statuses as for
+ * nosh_continuation_run. */
+typedef struct {
+  double param;         /* mu - mu_0 after the step */
+  int newton_iters;     /* lockstep Newton steps (all attempts) */
+  int minres_iters;     /* lockstep MINRES iterations (all attempts) */
+  int converged;        /* branches converged */
+  int seeded;           /* branches accepted from the surrogate guess */
+  int fallbacks;        /* branches re-solved after the surrogate guess */
+  double max_norm_f;    /* largest final ||F|| over the branches */
+  double seconds;
+  double pred_error;    /* mean over branches of ||guess - psi|| (the guess used first) */
+  double sec_error;     /* mean over branches of ||secant guess - psi|| */
+} nosh_batch_continuation_step;
+
+int nosh_batch_continuation_run(nosh_batch_t batch, double* psi, double span, int steps,
+                                const nosh_newton_options* opts,
+                                const nosh_surrogate_options* sopts,
+                                nosh_batch_continuation_step* record, int* steps_done,
+                                nosh_surrogate_stats* stats);
 
 #ifdef __cplusplus
 }

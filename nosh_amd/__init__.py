@@ -655,9 +655,9 @@ def _load_batch():
     vp, i32 = C.c_void_p, C.c_int
     sig = {
         "nosh_batch_create": (i32, [i32, vp, vp, vp, vp, C.POINTER(vp)]),
-        "nosh_batch_set_tuning": (i32, [vp, i32, i32, i32, i32]),
+        "nosh_batch_set_tuning": (i32, [vp, i32, i32, i32, i32, i32]),
         "nosh_batch_autotune": (i32, [vp, vp, C.POINTER(NewtonOptions), i32, C.POINTER(i32),
-                                      C.POINTER(i32)]),
+                                      C.POINTER(i32), C.POINTER(i32)]),
         "nosh_batch_solve": (i32, [vp, vp, C.POINTER(NewtonOptions),
                                    C.POINTER(BatchInstanceStats)]),
         "nosh_batch_solve_device": (i32, [vp, vp, C.POINTER(NewtonOptions),
@@ -708,16 +708,18 @@ class Batch(_Handle):
                 off += s
         return out
 
-    def set_tuning(self, block_threads=256, mode=0, graph_iters=4, poll=16):
-        _check(_lib.nosh_batch_set_tuning(self._h, block_threads, mode, graph_iters, poll))
+    def set_tuning(self, block_threads=256, mode=0, graph_iters=4, poll=16, fold=1):
+        _check(_lib.nosh_batch_set_tuning(self._h, block_threads, fold, mode, graph_iters,
+                                          poll))
 
     def autotune(self, psi, options: Optional[NewtonOptions] = None, use_cache=True, **kw):
+        """Returns (block_threads, mode, fold) of the fastest candidate."""
         o = options if options is not None else default_newton_options(**kw)
         x = _vec(psi, self.total_length)
-        t, m = C.c_int(), C.c_int()
+        t, f, m = C.c_int(), C.c_int(), C.c_int()
         _check(_lib.nosh_batch_autotune(self._h, _ptr(x), C.byref(o), 1 if use_cache else 0,
-                                        C.byref(t), C.byref(m)))
-        return t.value, m.value
+                                        C.byref(t), C.byref(f), C.byref(m)))
+        return t.value, m.value, f.value
 
     def solve(self, psi0, options: Optional[NewtonOptions] = None, **kw):
         o = options if options is not None else default_newton_options(**kw)
@@ -734,6 +736,50 @@ class Batch(_Handle):
                                             C.c_void_p(stream)))
         return [{k: getattr(st[i], k) for k, _ in BatchInstanceStats._fields_}
                 for i in range(self.ninst)]
+
+    def continuation(self, psi0, span: float, steps: int, surrogate: Optional[dict] = None,
+                     options: Optional[NewtonOptions] = None, **kw):
+        """Steps every instance (a branch) from its mu_0 to mu_0 + span in
+        ``steps`` equal steps, all branches solved together. With
+        ``surrogate`` (SurrogateOptions fields; ``{}`` for the defaults) the
+        MLP seeds Newton once trained. Returns (psi, step records, surrogate
+        stats or None)."""
+        lib = _load_mlp()
+        if not getattr(lib, "_nosh_bcont", False):
+            _load_surrogate(lib)
+            lib.nosh_batch_continuation_run.restype = C.c_int
+            lib.nosh_batch_continuation_run.argtypes = [
+                C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.POINTER(NewtonOptions),
+                C.POINTER(SurrogateOptions), C.POINTER(BatchContinuationStep), C.POINTER(C.c_int),
+                C.POINTER(SurrogateStats)]
+            lib._nosh_bcont = True
+        o = options if options is not None else default_newton_options(**kw)
+        psi = np.array(_vec(psi0, self.total_length), copy=True)
+        so = None
+        if surrogate is not None:
+            so = SurrogateOptions()
+            _check(lib.nosh_surrogate_default_options(C.byref(so)))
+            for k, v in surrogate.items():
+                setattr(so, k, v)
+        rec = (BatchContinuationStep * (steps + 1))()
+        done = C.c_int()
+        st = SurrogateStats()
+        rc = lib.nosh_batch_continuation_run(self._h, _ptr(psi), span, steps, C.byref(o),
+                                             C.byref(so) if so is not None else None, rec,
+                                             C.byref(done), C.byref(st))
+        if rc not in (_OK, _NOTCONV):
+            _check(rc)
+        out = [{k: getattr(rec[i], k) for k, _ in BatchContinuationStep._fields_}
+               for i in range(done.value)]
+        stats = {k: getattr(st, k) for k, _ in SurrogateStats._fields_} if so is not None else None
+        return psi, out, stats
+
+
+class BatchContinuationStep(C.Structure):
+    _fields_ = [("param", C.c_double), ("newton_iters", C.c_int), ("minres_iters", C.c_int),
+                ("converged", C.c_int), ("seeded", C.c_int), ("fallbacks", C.c_int),
+                ("max_norm_f", C.c_double), ("seconds", C.c_double), ("pred_error", C.c_double),
+                ("sec_error", C.c_double)]
 
 
 def _load_mlp():
@@ -817,12 +863,7 @@ class SurrogateStats(C.Structure):
                 ("steps_seeded", C.c_int), ("fallbacks", C.c_int)]
 
 
-def continuation_surrogate(model: Nls, psi0, params: Dict[str, float], parameter: str = "mu",
-                           start: float = 0.0, stop: float = 1.0, steps: int = 10,
-                           newton: Optional[NewtonOptions] = None, surrogate: Optional[dict] = None,
-                           **newton_kw):
-    """Surrogate-seeded continuation; returns (psi, step records, surrogate stats)."""
-    lib = _load_mlp()
+def _load_surrogate(lib):
     if not getattr(lib, "_nosh_sur", False):
         lib.nosh_surrogate_default_options.restype = C.c_int
         lib.nosh_surrogate_default_options.argtypes = [C.POINTER(SurrogateOptions)]
@@ -832,6 +873,15 @@ def continuation_surrogate(model: Nls, psi0, params: Dict[str, float], parameter
             C.POINTER(ContinuationOptions), C.POINTER(SurrogateOptions),
             C.POINTER(ContinuationStep), C.POINTER(C.c_int), C.POINTER(SurrogateStats)]
         lib._nosh_sur = True
+    return lib
+
+
+def continuation_surrogate(model: Nls, psi0, params: Dict[str, float], parameter: str = "mu",
+                           start: float = 0.0, stop: float = 1.0, steps: int = 10,
+                           newton: Optional[NewtonOptions] = None, surrogate: Optional[dict] = None,
+                           **newton_kw):
+    """Surrogate-seeded continuation; returns (psi, step records, surrogate stats)."""
+    lib = _load_surrogate(_load_mlp())
     o = ContinuationOptions()
     _check(lib.nosh_continuation_default_options(C.byref(o)))
     pname = parameter.encode()

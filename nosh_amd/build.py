@@ -17,7 +17,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnosh_amd.so")
 
 SOURCES = ["mesh_cache.hip", "operators.hip", "solver.hip", "batched.hip", "capi.cpp",
-           "continuation.cpp", "batch_capi.cpp", "mlp.hip", "mlp_capi.cpp"]
+           "continuation.cpp", "batch_capi.cpp", "batch_continuation.cpp", "mlp.hip",
+           "mlp_capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

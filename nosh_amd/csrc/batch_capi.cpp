@@ -17,18 +17,9 @@
 #include <string>
 #include <vector>
 
-#include "batched.hpp"
-#include "handles.hpp"
+#include "batch_handle.hpp"
 
 using namespace nosh;
-
-struct nosh_batch_s {
-  Batch b;
-  std::vector<nosh_nls_s*> models;
-  std::vector<int> counts;
-  int64_t total_len = 0;  // sum over instances of 2 n
-  DBuf<double> stage;
-};
 
 namespace {
 
@@ -82,7 +73,23 @@ void alloc_group(GroupHost& G) {
   d.W2 = G.W2.p;
 }
 
-// Instance vectors (reference order, concatenated) <-> interleaved state.
+// Deterministic relative perturbation of every lane (at most 1e-3), so that
+// the autotuner's timed Newton step has MINRES work even from an exact
+// solution.
+__global__ void k_perturb(int64_t m, double2* __restrict__ psi) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double f = 1.0 + 1e-3 * (double((i * 7919) % 2001) / 1000.0 - 1.0);
+  double2 p = psi[i];
+  p.x *= f;
+  p.y = p.y * f + 1e-4 * (f - 1.0);
+  psi[i] = p;
+}
+
+}  // namespace
+
+namespace nosh {
+
 void scatter_all(nosh_batch_s* h, const double* src_dev, hipStream_t s) {
   int64_t off = 0;
   for (auto& G : h->b.ghost) {
@@ -105,8 +112,12 @@ void gather_all(nosh_batch_s* h, double* dst_dev, hipStream_t s) {
   }
 }
 
-void newton_batch(nosh_batch_s* h, const nosh_newton_options& o,
-                  nosh_batch_instance_stats* st, hipStream_t s) {
+void batch_residuals(nosh_batch_s* h, std::vector<double>& norms, hipStream_t s) {
+  h->b.compute_norms(s, norms);
+}
+
+int newton_batch(nosh_batch_s* h, const nosh_newton_options& o, nosh_batch_instance_stats* st,
+                 hipStream_t s, const int* skip, const int* max_iters) {
   Batch& B = h->b;
   const int ni = B.args.ninst;
   B.args.iters = o.cheb_iters;
@@ -116,7 +127,10 @@ void newton_batch(nosh_batch_s* h, const nosh_newton_options& o,
   B.drop_graph();
   std::vector<double> nf, nf_prev(ni, -1.0), eta_prev(ni, o.forcing_max), rt(ni);
   std::vector<int> done(ni, 0), newton(ni, 0), mres(ni, 0);
+  if (skip)
+    for (int i = 0; i < ni; ++i) done[i] = skip[i] != 0;
   std::vector<double> final_nf(ni, 0.0);
+  int lockstep = 0;
   for (int it = 0;; ++it) {
     B.compute_norms(s, nf);
     int nactive = 0;
@@ -126,7 +140,8 @@ void newton_batch(nosh_batch_s* h, const nosh_newton_options& o,
         continue;
       }
       final_nf[i] = nf[i];
-      if (!(nf[i] > o.tol) || !std::isfinite(nf[i]) || it >= o.max_iters) {
+      if (!(nf[i] > o.tol) || !std::isfinite(nf[i]) ||
+          it >= (max_iters ? max_iters[i] : o.max_iters)) {
         done[i] = 1;
         B.active_h[i] = 0;
         continue;
@@ -155,18 +170,24 @@ void newton_batch(nosh_batch_s* h, const nosh_newton_options& o,
     NOSH_HIP(hipMemcpyAsync(B.active.p, B.active_h.data(), sizeof(int) * ni,
                             hipMemcpyHostToDevice, s));
     NOSH_HIP(hipMemcpyAsync(B.rtol.p, rt.data(), sizeof(double) * ni, hipMemcpyHostToDevice, s));
-    B.minres(s, o.minres_max_iters);
+    lockstep += B.minres(s, o.minres_max_iters);
     for (int i = 0; i < ni; ++i)
       if (B.active_h[i]) mres[i] += B.Sh[i].it;
   }
   if (st)
     for (int i = 0; i < ni; ++i) {
+      if (skip && skip[i]) continue;
       st[i].newton_iters = newton[i];
       st[i].minres_iters = mres[i];
       st[i].final_norm_f = final_nf[i];
       st[i].converged = (final_nf[i] <= o.tol) ? 1 : 0;
     }
+  return lockstep;
 }
+
+}  // namespace nosh
+
+namespace {
 
 std::string tune_key(nosh_batch_s* h, const nosh_newton_options& o) {
   hipDeviceProp_t prop{};
@@ -174,7 +195,7 @@ std::string tune_key(nosh_batch_s* h, const nosh_newton_options& o) {
   (void)hipGetDevice(&dev);
   (void)hipGetDeviceProperties(&prop, dev);
   std::ostringstream k;
-  k << prop.gcnArchName << "|cheb" << o.cheb_iters;
+  k << "v2|" << prop.gcnArchName << "|cheb" << o.cheb_iters;
   for (auto& G : h->b.ghost) k << "|" << G.mesh->n << "x" << G.dev.bp;
   return k.str();
 }
@@ -185,22 +206,26 @@ std::string tune_path() {
   return std::string(home ? home : "/tmp") + "/.cache/nosh_amd_autotune.tsv";
 }
 
-bool tune_lookup(const std::string& key, int* threads, int* mode) {
+bool tune_lookup(const std::string& key, int* threads, int* fold, int* mode) {
   std::ifstream f(tune_path());
-  std::string k;
-  int t, m;
-  while (f >> k >> t >> m)
-    if (k == key) {
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream ls(line);
+    std::string k;
+    int t, d, m;
+    if (ls >> k >> t >> d >> m && k == key) {
       *threads = t;
+      *fold = d;
       *mode = m;
       return true;
     }
+  }
   return false;
 }
 
-void tune_store(const std::string& key, int threads, int mode) {
+void tune_store(const std::string& key, int threads, int fold, int mode) {
   std::ofstream f(tune_path(), std::ios::app);
-  if (f) f << key << " " << threads << " " << mode << "\n";
+  if (f) f << key << " " << threads << " " << fold << " " << mode << "\n";
 }
 
 }  // namespace
@@ -268,6 +293,8 @@ int nosh_batch_create(int ngroups, const nosh_nls_t* models, const int* counts, 
     B.g.alloc(inst);
     B.mu.upload(mu, inst, s);
     B.g.upload(g, inst, s);
+    h->mu_h.assign(mu, mu + inst);
+    h->g_h.assign(g, g + inst);
     B.bound.alloc(inst);
     B.rtol.alloc(inst);
     B.norms.alloc(inst);
@@ -281,14 +308,14 @@ int nosh_batch_create(int ngroups, const nosh_nls_t* models, const int* counts, 
     B.args.bound = B.bound.p;
     B.args.rtol = B.rtol.p;
     B.args.active = B.active.p;
-    B.relayout(256);
+    B.relayout(256, 1);
     NOSH_HIP(hipStreamSynchronize(s));
     *out = h.release();
   });
 }
 
-int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int mode, int graph_iters,
-                          int poll) {
+int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int fold, int mode,
+                          int graph_iters, int poll) {
   return bguard([&] {
     NOSH_CHECK(batch, kInvalid, "null batch");
     int maxbp = 1;
@@ -296,14 +323,17 @@ int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int mode, int g
     NOSH_CHECK(block_threads >= 64 && block_threads <= 1024 && block_threads % 64 == 0 &&
                    block_threads % maxbp == 0,
                kInvalid, "block_threads must be a multiple of 64 and of the lane count, <= 1024");
+    NOSH_CHECK(fold >= 1 && fold <= 64, kInvalid, "fold must be in 1..64");
     NOSH_CHECK(mode == 0 || mode == 1, kInvalid, "mode must be 0 (graph) or 1 (persistent)");
     NOSH_CHECK(graph_iters >= 1 && poll >= graph_iters, kInvalid, "bad graph_iters / poll");
     batch->b.graph_iters = graph_iters;
     batch->b.poll = poll;
-    if (block_threads != batch->b.block_threads) batch->b.relayout(block_threads);
+    if (block_threads != batch->b.block_threads || fold != batch->b.fold)
+      batch->b.relayout(block_threads, fold);
     batch->b.drop_graph();
     batch->b.mode = 0;
-    NOSH_CHECK(mode == 0 || batch->b.args.total_blocks <= nosh::kPersistentMaxBlocks,
+    NOSH_CHECK(mode == 0 ||
+                   batch->b.args.total_blocks / batch->b.fold <= nosh::kPersistentMaxBlocks,
                kUnsupported,
                "persistent mode needs a grid of at most 64 workgroups at this block size; "
                "use mode 0 (hipGraph)");
@@ -343,7 +373,7 @@ int nosh_batch_solve(nosh_batch_t batch, double* psi, const nosh_newton_options*
 }
 
 int nosh_batch_autotune(nosh_batch_t batch, const double* psi, const nosh_newton_options* opts,
-                        int use_cache, int* best_threads, int* best_mode) {
+                        int use_cache, int* best_threads, int* best_fold, int* best_mode) {
   return bguard([&] {
     NOSH_CHECK(batch && psi, kInvalid, "null argument");
     nosh_newton_options o;
@@ -352,12 +382,12 @@ int nosh_batch_autotune(nosh_batch_t batch, const double* psi, const nosh_newton
     else
       nosh_newton_default_options(&o);
     const std::string key = tune_key(batch, o);
-    int bt = 256, bm = 0;
-    if (use_cache && tune_lookup(key, &bt, &bm)) {
-      nosh_batch_set_tuning(batch, bt, bm, batch->b.graph_iters, batch->b.poll);
+    int bt = 256, bf = 1, bm = 0;
+    if (use_cache && tune_lookup(key, &bt, &bf, &bm)) {
+      nosh_batch_set_tuning(batch, bt, bf, bm, batch->b.graph_iters, batch->b.poll);
     } else {
       // Time one Newton step (F, rebuild, MINRES capped at 40 iterations)
-      // for every candidate layout and iteration mode.
+      // for every candidate block size, fold and iteration mode.
       int maxbp = 1;
       for (auto& G : batch->b.ghost) maxbp = std::max(maxbp, G.dev.bp);
       hipStream_t s = default_stream();
@@ -370,35 +400,43 @@ int nosh_batch_autotune(nosh_batch_t batch, const double* psi, const nosh_newton
       double best = 1e30;
       for (int threads : {128, 256, 512, 1024}) {
         if (threads % maxbp || threads < maxbp) continue;
-        for (int mode : {0, 1}) {
-          if (nosh_batch_set_tuning(batch, threads, mode, batch->b.graph_iters,
-                                    batch->b.poll) != NOSH_OK)
-            continue;
-          double tmin = 1e30;
-          for (int rep = 0; rep < 2; ++rep) {
-            batch->stage.upload(psi, batch->total_len, s);
-            NOSH_HIP(hipStreamSynchronize(s));
-            const auto t0 = std::chrono::steady_clock::now();
-            scatter_all(batch, batch->stage.p, s);
-            newton_batch(batch, t, nullptr, s);
-            NOSH_HIP(hipStreamSynchronize(s));
-            tmin = std::min(tmin, std::chrono::duration<double>(
-                                      std::chrono::steady_clock::now() - t0).count());
-          }
-          if (std::getenv("NOSH_AUTOTUNE_VERBOSE"))
-            std::fprintf(stderr, "autotune: block_threads %d mode %d: %.4f s\n", threads, mode,
-                         tmin);
-          if (tmin < best) {
-            best = tmin;
-            bt = threads;
-            bm = mode;
+        for (int fold : {1, 4, 16}) {
+          for (int mode : {0, 1}) {
+            if (nosh_batch_set_tuning(batch, threads, fold, mode, batch->b.graph_iters,
+                                      batch->b.poll) != NOSH_OK)
+              continue;
+            double tmin = 1e30;
+            for (int rep = 0; rep < 2; ++rep) {
+              batch->stage.upload(psi, batch->total_len, s);
+              NOSH_HIP(hipStreamSynchronize(s));
+              const auto t0 = std::chrono::steady_clock::now();
+              scatter_all(batch, batch->stage.p, s);
+              for (auto& G : batch->b.ghost) {
+                const int64_t m = G.mesh->n * G.dev.bp;
+                k_perturb<<<grid_for(m, 256), 256, 0, s>>>(m, G.psi.p);
+              }
+              newton_batch(batch, t, nullptr, s);
+              NOSH_HIP(hipStreamSynchronize(s));
+              tmin = std::min(tmin, std::chrono::duration<double>(
+                                        std::chrono::steady_clock::now() - t0).count());
+            }
+            if (std::getenv("NOSH_AUTOTUNE_VERBOSE"))
+              std::fprintf(stderr, "autotune: block_threads %d fold %d mode %d: %.4f s\n",
+                           threads, fold, mode, tmin);
+            if (tmin < best) {
+              best = tmin;
+              bt = threads;
+              bf = fold;
+              bm = mode;
+            }
           }
         }
       }
-      nosh_batch_set_tuning(batch, bt, bm, batch->b.graph_iters, batch->b.poll);
-      if (use_cache) tune_store(key, bt, bm);
+      nosh_batch_set_tuning(batch, bt, bf, bm, batch->b.graph_iters, batch->b.poll);
+      if (use_cache) tune_store(key, bt, bf, bm);
     }
     if (best_threads) *best_threads = bt;
+    if (best_fold) *best_fold = bf;
     if (best_mode) *best_mode = bm;
   });
 }

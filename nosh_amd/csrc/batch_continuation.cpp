@@ -1,0 +1,520 @@
+// batch_continuation.cpp -- natural-parameter continuation in mu of a batch
+// of branches, solved together.
+//
+// Every instance of a batch is a branch (mesh, g, mu_0). One step moves all
+// branches by the same increment h in mu and solves them with the lockstep
+// batched Newton (FP32 inner MINRES, FP64 outer). Predictor: the secant
+// through the last two accepted solutions,
+//   psi_sec = psi_k + (h / h_prev) (psi_k - psi_{k-1})   (psi_k on the first step).
+// With a surrogate, after `train_after` steps a per-vertex MLP (BF16 MFMA,
+// trained on the run's own solutions) seeds Newton with
+//   psi_guess = psi_sec + h^2 sigma Y(x, psi_k, h s, mu, g),   h s = psi_sec - psi_k,
+// sigma the RMS of the training targets (psi_{k+1} - psi_sec) / h^2. A branch
+// whose Newton from the guess does not converge within the budget, or whose
+// Gibbs energy leaves its linear trend, is solved again from psi_sec, then,
+// if that fails too, from psi_k with the full iteration limit.
+//
+// This is the caller the reference delegates to LOCA (natural stepping,
+// executables/nosh-cont/nosh-cont.cpp:226-290,344 with examples/conf.xml:35-190);
+// the Gibbs energy per branch is the observer's
+// (src/observer.cpp:134-159): -sum cv |psi|^4 / sum cv.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+#include "batch_handle.hpp"
+#include "kernels.hpp"
+#include "mlp.hpp"
+
+using namespace nosh;
+
+namespace {
+
+template <class F>
+int bcguard(F&& f) {
+  try {
+    f();
+    set_last_error("");
+    return NOSH_OK;
+  } catch (const nosh::Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return NOSH_ERR_INVALID;
+  }
+}
+
+constexpr int kRed = 256;       // threads of the per-lane reduction blocks
+constexpr int kRedBlocks = 512; // blocks of the per-lane reduction (grid-stride)
+
+// out = cur + r (cur - prev) over a group's interleaved [n][bp] state.
+__global__ void k_bc_secant(int64_t m, double r, const double2* __restrict__ cur,
+                            const double2* __restrict__ prev, double2* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double2 c = cur[i], p = prev[i];
+  out[i] = make_double2(c.x + r * (c.x - p.x), c.y + r * (c.y - p.y));
+}
+
+// Feature rows (row, b) -> X[(row * B + b) * nin ...]:
+// coordinates * cs (dim), Re/Im psi_k, Re/Im (psi_sec - psi_k), mu_b, g_b.
+__global__ void k_bc_features(int64_t n, int bp, int B, int dim, const int32_t* __restrict__ perm,
+                              const double* __restrict__ coords, double cs,
+                              const double2* __restrict__ cur, const double2* __restrict__ sec,
+                              const double* __restrict__ mu, const double* __restrict__ g,
+                              int inst0, float* __restrict__ X) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n * B) return;
+  const int64_t row = i / B;
+  const int b = int(i - row * B);
+  const int nin = dim + 6;
+  float* x = X + i * nin;
+  const int64_t v = perm[row];
+  for (int d = 0; d < dim; ++d) x[d] = float(coords[3 * v + d] * cs);
+  const double2 c = cur[row * bp + b], s = sec[row * bp + b];
+  x[dim] = float(c.x);
+  x[dim + 1] = float(c.y);
+  x[dim + 2] = float(s.x - c.x);
+  x[dim + 3] = float(s.y - c.y);
+  x[dim + 4] = float(mu[inst0 + b]);
+  x[dim + 5] = float(g[inst0 + b]);
+}
+
+// Training target (psi_new - psi_sec) / h^2 per (row, b).
+__global__ void k_bc_target(int64_t n, int bp, int B, const double2* __restrict__ pnew,
+                            const double2* __restrict__ sec, double ih2, float* __restrict__ T) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n * B) return;
+  const int64_t row = i / B;
+  const int b = int(i - row * B);
+  const double2 a = pnew[row * bp + b], s = sec[row * bp + b];
+  T[2 * i] = float((a.x - s.x) * ih2);
+  T[2 * i + 1] = float((a.y - s.y) * ih2);
+}
+
+// out = sec + scale Y for real lanes; padding lanes copy sec.
+__global__ void k_bc_apply(int64_t n, int bp, int B, const double2* __restrict__ sec,
+                           const float* __restrict__ Y, double scale, double2* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n * bp) return;
+  const int64_t row = i / bp;
+  const int b = int(i - row * bp);
+  double2 s = sec[i];
+  if (b < B) {
+    const int64_t k = row * B + b;
+    s.x += scale * double(Y[2 * k]);
+    s.y += scale * double(Y[2 * k + 1]);
+  }
+  out[i] = s;
+}
+
+// dst[row][b] = src[row][b] for the lanes with take[b] != 0.
+__global__ void k_bc_take(int64_t n, int bp, const int* __restrict__ take,
+                          const double2* __restrict__ src, double2* __restrict__ dst) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n * bp) return;
+  if (take[i & (bp - 1)]) dst[i] = src[i];
+}
+
+// Per-lane sums over rows, deterministic (fixed grid, fixed order):
+// mode 0: cv |a|^4, mode 1: |a - b|^2, mode 2: cv. partials[lane][block].
+__global__ __launch_bounds__(kRed) void k_bc_lane_partials(int64_t n, int bp, int mode,
+                                                           const double* __restrict__ cv,
+                                                           const double2* __restrict__ a,
+                                                           const double2* __restrict__ b,
+                                                           double* __restrict__ partials) {
+  __shared__ double lds[kRed];
+  const int lane = threadIdx.x & (bp - 1);
+  const int rows_per_block = kRed / bp;
+  double acc = 0.0;
+  for (int64_t row = int64_t(blockIdx.x) * rows_per_block + threadIdx.x / bp; row < n;
+       row += int64_t(gridDim.x) * rows_per_block) {
+    const int64_t i = row * bp + lane;
+    if (mode == 0) {
+      const double2 x = a[i];
+      const double r2 = x.x * x.x + x.y * x.y;
+      acc += cv[row] * r2 * r2;
+    } else if (mode == 1) {
+      const double2 x = a[i], y = b[i];
+      const double dx = x.x - y.x, dy = x.y - y.y;
+      acc += dx * dx + dy * dy;
+    } else {
+      acc += cv[row];
+    }
+  }
+  lds[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < bp) {
+    double s = 0.0;
+    for (int r = 0; r < rows_per_block; ++r) s += lds[r * bp + threadIdx.x];
+    partials[int64_t(threadIdx.x) * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// One block per lane: out[lane] = sum of its nb partials (fixed order).
+__global__ void k_bc_lane_final(const double* __restrict__ partials, int nb,
+                                double* __restrict__ out) {
+  double v[1];
+  finalize_sum<1>(v, partials + int64_t(blockIdx.x) * nb, nb);
+  if (threadIdx.x == 0) out[blockIdx.x] = v[0];
+}
+
+struct GroupCont {
+  DBuf<double2> cur, prev, sec, guess;
+  DBuf<int> take;
+  std::vector<int> take_h;
+  double cv_sum = 0.0;
+  int64_t xoff = 0;  // first feature row of the group
+};
+
+}  // namespace
+
+extern "C" {
+
+int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int steps,
+                                const nosh_newton_options* opts,
+                                const nosh_surrogate_options* so,
+                                nosh_batch_continuation_step* record, int* steps_done,
+                                nosh_surrogate_stats* stats) {
+  int done = 0;
+  nosh_surrogate_stats sst{};
+  int rc = bcguard([&] {
+    NOSH_CHECK(bt && psi && record && steps >= 1, kInvalid, "invalid batch continuation call");
+    NOSH_CHECK(!so || (so->train_after >= 1 && so->batch >= 64 && so->newton_budget >= 1),
+               kInvalid, "bad surrogate options");
+    nosh_newton_options o;
+    if (opts)
+      o = *opts;
+    else
+      nosh_newton_default_options(&o);
+    Batch& B = bt->b;
+    const int ni = B.args.ninst;
+    hipStream_t s = default_stream();
+    const size_t ng = B.ghost.size();
+    std::vector<GroupCont> gc(ng);
+    DBuf<double> part(size_t(64) * kRedBlocks), lane_out(64);
+    std::vector<double> lane_h(64);
+    int64_t xrows = 0;
+    int dim = 2;
+    for (size_t k = 0; k < ng; ++k) {
+      GroupHost& G = B.ghost[k];
+      const int64_t m = G.mesh->n * G.dev.bp;
+      gc[k].cur.alloc(m);
+      gc[k].prev.alloc(m);
+      gc[k].sec.alloc(m);
+      gc[k].guess.alloc(m);
+      gc[k].take.alloc(G.dev.bp);
+      gc[k].xoff = xrows;
+      xrows += G.mesh->n * G.B;
+      if (G.mesh->nv == 4) dim = 3;
+    }
+    // Per-lane reductions over a group: lane_h[b] for b < bp.
+    auto lane_sums = [&](size_t k, int mode, const double2* a, const double2* b) {
+      GroupHost& G = B.ghost[k];
+      const int bp = G.dev.bp;
+      k_bc_lane_partials<<<kRedBlocks, kRed, 0, s>>>(G.mesh->n, bp, mode, G.mesh->cv_int.p, a, b,
+                                                     part.p);
+      k_bc_lane_final<<<bp, kFinalizeThreads, 0, s>>>(part.p, kRedBlocks, lane_out.p);
+      NOSH_HIP(hipMemcpyAsync(lane_h.data(), lane_out.p, sizeof(double) * bp,
+                              hipMemcpyDeviceToHost, s));
+      NOSH_HIP(hipStreamSynchronize(s));
+    };
+    for (size_t k = 0; k < ng; ++k) {
+      lane_sums(k, 2, nullptr, nullptr);
+      gc[k].cv_sum = lane_h[0];
+    }
+    // Gibbs energy of every instance at the current state.
+    auto energies = [&](std::vector<double>& e) {
+      e.assign(ni, 0.0);
+      for (size_t k = 0; k < ng; ++k) {
+        GroupHost& G = B.ghost[k];
+        lane_sums(k, 0, G.psi.p, nullptr);
+        for (int b = 0; b < G.B; ++b) e[G.dev.inst0 + b] = -lane_h[b] / gc[k].cv_sum;
+      }
+    };
+    auto copy_state = [&](DBuf<double2> GroupCont::*dst, bool from_psi) {
+      for (size_t k = 0; k < ng; ++k) {
+        GroupHost& G = B.ghost[k];
+        const size_t bytes = sizeof(double2) * G.mesh->n * G.dev.bp;
+        if (from_psi)
+          NOSH_HIP(hipMemcpyAsync((gc[k].*dst).p, G.psi.p, bytes, hipMemcpyDeviceToDevice, s));
+        else
+          NOSH_HIP(hipMemcpyAsync(G.psi.p, (gc[k].*dst).p, bytes, hipMemcpyDeviceToDevice, s));
+      }
+    };
+    // Copies src into psi for the instances with take[i] != 0.
+    auto take_into_psi = [&](DBuf<double2> GroupCont::*src, const std::vector<int>& take) {
+      for (size_t k = 0; k < ng; ++k) {
+        GroupHost& G = B.ghost[k];
+        std::vector<int>& t = gc[k].take_h;
+        t.assign(G.dev.bp, 0);
+        for (int b = 0; b < G.B; ++b) t[b] = take[G.dev.inst0 + b];
+        gc[k].take.upload(t.data(), t.size(), s);
+        const int64_t m = G.mesh->n * G.dev.bp;
+        k_bc_take<<<grid_for(m, 256), 256, 0, s>>>(G.mesh->n, G.dev.bp, gc[k].take.p,
+                                                   (gc[k].*src).p, G.psi.p);
+      }
+    };
+    std::vector<double> mu(bt->mu_h), e_now, e1(ni), e0(ni);
+    auto set_mu = [&]() { B.mu.upload(mu.data(), ni, s); };
+
+    // Surrogate state.
+    const int nin = dim + 6;
+    std::unique_ptr<Mlp> net;
+    DBuf<float> X, Y, Xs, Ts, xb, tb;
+    DBuf<int64_t> idx;
+    std::vector<int64_t> idx_h;
+    int P = 0, pairs = 0;
+    bool trained = false;
+    double sigma = 1.0, cs = 1.0;
+    std::mt19937_64 rng(so ? so->seed : 0);
+    if (so) {
+      P = so->train_after;
+      net = std::make_unique<Mlp>();
+      net->init(nin, so->width, so->depth, 2, so->seed);
+      X.alloc(xrows * nin);
+      Y.alloc(xrows * 2);
+      Xs.alloc(int64_t(P) * xrows * nin);
+      Ts.alloc(int64_t(P) * xrows * 2);
+      xb.alloc(int64_t(so->batch) * nin);
+      tb.alloc(int64_t(so->batch) * 2);
+      idx.alloc(so->batch);
+      idx_h.resize(so->batch);
+      double cmax = 1e-300;
+      for (size_t k = 0; k < ng; ++k) {
+        MeshCache& m = *B.ghost[k].mesh;
+        std::vector<double> ch(3 * m.n);
+        m.coords.download(ch.data(), ch.size(), s);
+        NOSH_HIP(hipStreamSynchronize(s));
+        for (double v : ch) cmax = std::max(cmax, std::fabs(v));
+      }
+      cs = 1.0 / cmax;
+    }
+    constexpr int64_t kChunk = 1 << 19;  // MLP rows per inference call
+    auto predict = [&](double scale) {
+      for (size_t k = 0; k < ng; ++k) {
+        GroupHost& G = B.ghost[k];
+        k_bc_features<<<grid_for(G.mesh->n * G.B, 256), 256, 0, s>>>(
+            G.mesh->n, G.dev.bp, G.B, dim, G.mesh->perm.p, G.mesh->coords.p, cs, gc[k].cur.p,
+            gc[k].sec.p, B.mu.p, B.g.p, G.dev.inst0, X.p + gc[k].xoff * nin);
+      }
+      for (int64_t r0 = 0; r0 < xrows; r0 += kChunk)
+        net->forward(X.p + r0 * nin, std::min(kChunk, xrows - r0), Y.p + r0 * 2, s);
+      for (size_t k = 0; k < ng; ++k) {
+        GroupHost& G = B.ghost[k];
+        k_bc_apply<<<grid_for(G.mesh->n * G.dev.bp, 256), 256, 0, s>>>(
+            G.mesh->n, G.dev.bp, G.B, gc[k].sec.p, Y.p + gc[k].xoff * 2, scale, gc[k].guess.p);
+      }
+    };
+    auto train = [&]() {
+      std::vector<float> th(size_t(P) * xrows * 2);
+      Ts.download(th.data(), th.size(), s);
+      NOSH_HIP(hipStreamSynchronize(s));
+      double ss = 0.0;
+      for (float v : th) ss += double(v) * v;
+      sigma = std::sqrt(ss / double(th.size()));
+      if (!(sigma > 0.0)) sigma = 1.0;
+      std::uniform_int_distribution<int64_t> pick(0, int64_t(P) * xrows - 1);
+      double lval = 0.0;
+      for (int it = 0; it < so->train_steps; ++it) {
+        for (auto& v : idx_h) v = pick(rng);
+        idx.upload(idx_h.data(), idx_h.size(), s);
+        launch_gather_rows(Xs.p, idx.p, so->batch, nin, 1.0f, xb.p, s);
+        launch_gather_rows(Ts.p, idx.p, so->batch, 2, float(1.0 / sigma), tb.p, s);
+        lval = net->gradients(xb.p, tb.p, so->batch, s);
+        net->adam(so->lr, s);
+      }
+      NOSH_HIP(hipStreamSynchronize(s));
+      return lval;
+    };
+
+    // Step 0: solve every branch at its mu_0.
+    if (bt->stage.n != size_t(bt->total_len)) bt->stage.alloc(bt->total_len);
+    bt->stage.upload(psi, bt->total_len, s);
+    scatter_all(bt, bt->stage.p, s);
+    std::vector<nosh_batch_instance_stats> st(ni);
+    {
+      const auto t0 = std::chrono::steady_clock::now();
+      set_mu();
+      nosh_batch_continuation_step& r = record[0];
+      r = nosh_batch_continuation_step{};
+      r.minres_iters = newton_batch(bt, o, st.data(), s);
+      for (int i = 0; i < ni; ++i) {
+        r.newton_iters = std::max(r.newton_iters, st[i].newton_iters);
+        r.converged += st[i].converged;
+        r.max_norm_f = std::max(r.max_norm_f, st[i].final_norm_f);
+      }
+      r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      NOSH_CHECK(r.converged == ni, kNotConverged, "batch continuation: initial solve failed");
+      energies(e1);
+      e0 = e1;
+      copy_state(&GroupCont::cur, true);
+      done = 1;
+    }
+    const double h = span / steps;
+    double hprev = 0.0;
+    const bool progress = std::getenv("NOSH_PROGRESS") != nullptr;
+    std::vector<int> ok(ni), skip(ni), use_sur(ni), maxit(ni);
+    std::vector<double> nf_sur, nf_sec;
+    for (int k = 1; k <= steps; ++k) {
+      const auto t0 = std::chrono::steady_clock::now();
+      nosh_batch_continuation_step& r = record[k];
+      r = nosh_batch_continuation_step{};
+      r.param = k * h;
+      for (int i = 0; i < ni; ++i) mu[i] = bt->mu_h[i] + k * h;
+      set_mu();
+      // Secant guess (psi_k on the first step).
+      const double ratio = hprev > 0.0 ? h / hprev : 0.0;
+      for (size_t q = 0; q < ng; ++q) {
+        GroupHost& G = B.ghost[q];
+        const int64_t m = G.mesh->n * G.dev.bp;
+        k_bc_secant<<<grid_for(m, 256), 256, 0, s>>>(m, ratio, gc[q].cur.p, gc[q].prev.p,
+                                                     gc[q].sec.p);
+      }
+      const bool seeded = so && trained && hprev > 0.0;
+      auto run = [&](const nosh_newton_options& oo, const int* sk) {
+        const int it = newton_batch(bt, oo, st.data(), s, sk);
+        int mx = 0;
+        for (int i = 0; i < ni; ++i)
+          if (!sk || !sk[i]) mx = std::max(mx, st[i].newton_iters);
+        r.newton_iters += mx;
+        r.minres_iters += it;
+      };
+      std::fill(ok.begin(), ok.end(), 0);
+      if (seeded) {
+        // Each branch starts from the guess with the smaller ||F||: the
+        // surrogate's (Newton budget so->newton_budget) or the secant's (the
+        // full iteration limit).
+        predict(h * h * sigma);
+        copy_state(&GroupCont::guess, false);
+        batch_residuals(bt, nf_sur, s);
+        copy_state(&GroupCont::sec, false);
+        batch_residuals(bt, nf_sec, s);
+        int nsur = 0;
+        for (int i = 0; i < ni; ++i) {
+          use_sur[i] = nf_sur[i] < nf_sec[i];
+          maxit[i] = use_sur[i] ? so->newton_budget : o.max_iters;
+          nsur += use_sur[i];
+        }
+        if (nsur > 0) {
+          take_into_psi(&GroupCont::guess, use_sur);
+          const int it = newton_batch(bt, o, st.data(), s, nullptr, maxit.data());
+          int mx = 0;
+          for (int i = 0; i < ni; ++i) mx = std::max(mx, st[i].newton_iters);
+          r.newton_iters += mx;
+          r.minres_iters += it;
+          energies(e_now);
+          for (int i = 0; i < ni; ++i) {
+            if (!use_sur[i]) {
+              ok[i] = st[i].converged;
+              continue;
+            }
+            const double extrap = e1[i] + (e1[i] - e0[i]);
+            ok[i] = st[i].converged &&
+                    std::fabs(e_now[i] - extrap) <=
+                        std::max(so->energy_tol, 10.0 * std::fabs(e1[i] - e0[i]));
+            r.seeded += ok[i];
+            r.fallbacks += !ok[i];
+          }
+          sst.steps_seeded += r.seeded;
+          sst.fallbacks += r.fallbacks;
+        }
+      }
+      // Secant guess for every branch not yet accepted, then psi_k.
+      for (int attempt = 0; attempt < 2; ++attempt) {
+        int todo = 0;
+        for (int i = 0; i < ni; ++i) {
+          skip[i] = ok[i];
+          todo += !ok[i];
+        }
+        if (todo == 0) break;
+        std::vector<int> take(ni);
+        for (int i = 0; i < ni; ++i) take[i] = !ok[i];
+        take_into_psi(attempt == 0 ? &GroupCont::sec : &GroupCont::cur, take);
+        run(o, skip.data());
+        for (int i = 0; i < ni; ++i)
+          if (!ok[i]) ok[i] = st[i].converged;
+      }
+      for (int i = 0; i < ni; ++i) {
+        r.converged += ok[i];
+        r.max_norm_f = std::max(r.max_norm_f, st[i].final_norm_f);
+      }
+      if (r.converged < ni) {
+        copy_state(&GroupCont::cur, false);
+        r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        throw nosh::Error(kNotConverged, "batch continuation: a branch failed to converge");
+      }
+      // Predictor errors (mean over branches) for the record.
+      double pe = 0.0, se = 0.0;
+      for (size_t q = 0; q < ng; ++q) {
+        GroupHost& G = B.ghost[q];
+        lane_sums(q, 1, G.psi.p, gc[q].sec.p);
+        for (int b = 0; b < G.B; ++b) se += std::sqrt(lane_h[b]);
+        if (seeded) {
+          lane_sums(q, 1, G.psi.p, gc[q].guess.p);
+          for (int b = 0; b < G.B; ++b) pe += std::sqrt(lane_h[b]);
+        }
+      }
+      r.sec_error = se / ni;
+      r.pred_error = seeded ? pe / ni : r.sec_error;
+      // Training pairs from whole secant steps.
+      if (so && !trained && hprev > 0.0 && pairs < P) {
+        for (size_t q = 0; q < ng; ++q) {
+          GroupHost& G = B.ghost[q];
+          const int64_t rows = G.mesh->n * G.B;
+          k_bc_features<<<grid_for(rows, 256), 256, 0, s>>>(
+              G.mesh->n, G.dev.bp, G.B, dim, G.mesh->perm.p, G.mesh->coords.p, cs, gc[q].cur.p,
+              gc[q].sec.p, B.mu.p, B.g.p, G.dev.inst0,
+              Xs.p + (int64_t(pairs) * xrows + gc[q].xoff) * nin);
+          k_bc_target<<<grid_for(rows, 256), 256, 0, s>>>(
+              G.mesh->n, G.dev.bp, G.B, G.psi.p, gc[q].sec.p, 1.0 / (h * h),
+              Ts.p + (int64_t(pairs) * xrows + gc[q].xoff) * 2);
+        }
+        ++pairs;
+        if (pairs == P) {
+          const auto tt = std::chrono::steady_clock::now();
+          sst.final_loss = train();
+          sst.train_seconds =
+              std::chrono::duration<double>(std::chrono::steady_clock::now() - tt).count();
+          sst.trained = 1;
+          trained = true;
+        }
+      }
+      energies(e_now);
+      e0 = e1;
+      e1 = e_now;
+      for (size_t q = 0; q < ng; ++q) std::swap(gc[q].prev, gc[q].cur);
+      copy_state(&GroupCont::cur, true);
+      hprev = h;
+      r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      done = k + 1;
+      if (progress)
+        std::fprintf(stderr,
+                     "[batch continuation] step %d/%d dmu %.4f newton %d minres %d seeded %d "
+                     "%.3fs\n",
+                     k, steps, r.param, r.newton_iters, r.minres_iters, r.seeded, r.seconds);
+    }
+  });
+  if (bt && psi && done > 0) {
+    hipStream_t s = default_stream();
+    gather_all(bt, bt->stage.p, s);
+    bt->stage.download(psi, bt->total_len, s);
+    (void)hipStreamSynchronize(s);
+    // The batch keeps its per-instance mu_0 for later solves.
+    bt->b.mu.upload(bt->mu_h.data(), bt->mu_h.size(), s);
+    (void)hipStreamSynchronize(s);
+  }
+  if (steps_done) *steps_done = done;
+  if (stats) *stats = sst;
+  return rc;
+}
+
+}  // extern "C"

@@ -67,18 +67,20 @@ __device__ __forceinline__ Lane lane_ctx(const BatchArgs& a, int64_t vb) {
   return L;
 }
 
-// Per-instance partial sums of one (virtual) block: rows of the block are
-// reduced in order for every lane b.
-__device__ __forceinline__ void lane_partials(const BatchArgs& a, const Lane& L, int64_t vb,
-                                              double v) {
+// Per-instance partial sums of one launched block, which covers the virtual
+// blocks v0 .. v0 + fold - 1 of one group; v is the thread's sum over them
+// (0 for padding lanes). Rows are reduced in order for every lane b, giving
+// partial (v0 - block0) / fold of the instance.
+__device__ __forceinline__ void lane_partials(const BatchArgs& a, int64_t v0, double v) {
   __shared__ double lds[1024];
-  lds[threadIdx.x] = L.valid ? v : 0.0;
+  const BGroup& G = a.groups[group_of(a, v0)];
+  lds[threadIdx.x] = v;
   __syncthreads();
-  const int bp = L.G->bp;
-  if (threadIdx.x < bp && threadIdx.x < L.G->B) {
+  const int bp = G.bp;
+  if (threadIdx.x < bp && threadIdx.x < G.B) {
     double s = 0.0;
     for (int r = 0; r < int(blockDim.x) / bp; ++r) s += lds[r * bp + threadIdx.x];
-    a.partials[int64_t(L.G->inst0 + threadIdx.x) * a.maxblocks + (vb - L.G->block0)] = s;
+    a.partials[int64_t(G.inst0 + threadIdx.x) * a.maxblocks + (v0 - G.block0) / a.fold] = s;
   }
   __syncthreads();
 }
@@ -98,11 +100,45 @@ __device__ __forceinline__ void sc(float x, float* s, float* c) { __sincosf(x, s
 __device__ __forceinline__ void sc(double x, double* s, double* c) { sincos(x, s, c); }
 
 // Off-diagonal product of one (row, instance) with values rebuilt in registers.
+// The slot count of a row is its slice's width, the same for the whole
+// wave; slots are taken 4 at a time so that their column, alpha and a~
+// loads, and then the x gathers, are all in flight together (4 measured
+// faster than 1 or 8 at 96^3 x 8 lanes).
+template <class T, int K>
+__device__ __forceinline__ void brow_slots(const BGroup& G, int32_t j, int b, T mu,
+                                           const T* __restrict__ al, const T* __restrict__ at,
+                                           const typename Vt<T>::v2* __restrict__ x, T& ar,
+                                           T& ai) {
+  int32_t c[K];
+  T av[K], tv[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    c[q] = G.scol[j + q * kSliceH];
+    av[q] = al[j + q * kSliceH];
+    tv[q] = at[j + q * kSliceH];
+  }
+  typename Vt<T>::v2 xv[K];
+  const int bp = G.bp;
+#pragma unroll
+  for (int q = 0; q < K; ++q) xv[q] = x[int64_t(c[q]) * bp + b];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    T sn, cs;
+    sc(mu * tv[q], &sn, &cs);
+    const T vr = -av[q] * cs, vi = av[q] * sn;
+    ar += vr * xv[q].x - vi * xv[q].y;
+    ai += vr * xv[q].y + vi * xv[q].x;
+  }
+}
+
+constexpr int kSlotBatch = 4;
+
 template <class T>
 __device__ __forceinline__ typename Vt<T>::v2 brow(const BGroup& G, int64_t row, int b, T mu,
                                                     const typename Vt<T>::v2* __restrict__ x) {
   const int32_t s = int32_t(row >> 6);
   const int32_t beg = G.slice_ptr[s], end = G.slice_ptr[s + 1];
+  const int32_t w = (end - beg) / kSliceH;
   const T* al;
   const T* at;
   if constexpr (std::is_same<T, float>::value) {
@@ -113,18 +149,11 @@ __device__ __forceinline__ typename Vt<T>::v2 brow(const BGroup& G, int64_t row,
     at = G.t64;
   }
   T ar = T(0), ai = T(0);
-  const int bp = G.bp;
-#pragma unroll 2
-  for (int32_t j = beg + int32_t(row & 63); j < end; j += kSliceH) {
-    const int32_t c = G.scol[j];
-    const T a = al[j];
-    T sn, cs;
-    sc(mu * at[j], &sn, &cs);
-    const T vr = -a * cs, vi = a * sn;
-    const typename Vt<T>::v2 xv = x[int64_t(c) * bp + b];
-    ar += vr * xv.x - vi * xv.y;
-    ai += vr * xv.y + vi * xv.x;
-  }
+  int32_t j = beg + int32_t(row & 63);
+  int32_t k = 0;
+  for (; k + kSlotBatch <= w; k += kSlotBatch, j += kSlotBatch * kSliceH)
+    brow_slots<T, kSlotBatch>(G, j, b, mu, al, at, x, ar, ai);
+  for (; k < w; ++k, j += kSliceH) brow_slots<T, 1>(G, j, b, mu, al, at, x, ar, ai);
   typename Vt<T>::v2 r;
   r.x = ar;
   r.y = ai;
@@ -135,7 +164,7 @@ __device__ __forceinline__ typename Vt<T>::v2 brow(const BGroup& G, int64_t row,
 // Every phase is a device function over one virtual block, used both by the
 // per-phase kernels (graph mode) and by the persistent kernel.
 
-__device__ void ph_compute_f(const BatchArgs& a, int64_t vb) {
+__device__ double ph_compute_f(const BatchArgs& a, int64_t vb) {
   const Lane L = lane_ctx(a, vb);
   double s2 = 0.0;
   if (L.valid) {
@@ -148,7 +177,7 @@ __device__ void ph_compute_f(const BatchArgs& a, int64_t vb) {
     G.f[L.idx] = r;
     s2 = r.x * r.x + r.y * r.y;
   }
-  lane_partials(a, L, vb, s2);
+  return s2;
 }
 
 // J and P blocks (FP32, including diag K) and the per-instance Chebyshev bound.
@@ -191,6 +220,8 @@ __device__ void ph_rebuild(const BatchArgs& a, int64_t vb) {
     atomicMax(reinterpret_cast<unsigned long long*>(a.bound) + L.G->inst0 + threadIdx.x,
               static_cast<unsigned long long>(__double_as_longlong(m)));
   }
+  // rmax is rewritten by the next row tile of a folded block.
+  __syncthreads();
 }
 
 __device__ void ph_setup(const BatchArgs& a, int64_t vb) {
@@ -218,7 +249,7 @@ __device__ __forceinline__ float2* wbuf(const BGroup& G, int k) {
 __device__ __forceinline__ float2* zbuf(const BGroup& G, int k) { return (k & 1) ? G.Z2 : G.Y; }
 
 // Chebyshev step 0. mode 1: r = R0 (MINRES start); mode 2: r_new = u - (alfa/beta) r2.
-__device__ void ph_cheb_first(const BatchArgs& a, int64_t vb, int mode) {
+__device__ double ph_cheb_first(const BatchArgs& a, int64_t vb, int mode) {
   const Lane L = lane_ctx(a, vb);
   double s2 = 0.0;
   const MinresScalars* S = L.valid ? &a.S[L.inst] : nullptr;
@@ -252,10 +283,10 @@ __device__ void ph_cheb_first(const BatchArgs& a, int64_t vb, int mode) {
     zbuf(G, a.z0)[L.idx] = z;
     s2 = double(rv.x) * double(z.x) + double(rv.y) * double(z.y);
   }
-  if (a.iters <= 1) lane_partials(a, L, vb, s2);
+  return s2;
 }
 
-__device__ void ph_cheb_step(const BatchArgs& a, int64_t vb, int mode, int j) {
+__device__ double ph_cheb_step(const BatchArgs& a, int64_t vb, int mode, int j) {
   const Lane L = lane_ctx(a, vb);
   double s2 = 0.0;
   const bool last = (j == a.iters - 1);
@@ -284,7 +315,7 @@ __device__ void ph_cheb_step(const BatchArgs& a, int64_t vb, int mode, int j) {
     zout[L.idx] = z;
     s2 = double(rv.x) * double(z.x) + double(rv.y) * double(z.y);
   }
-  if (last) lane_partials(a, L, vb, s2);
+  return last ? s2 : 0.0;
 }
 
 __device__ __forceinline__ void w_update(const BGroup& G, const MinresScalars* S, int k,
@@ -302,7 +333,7 @@ __device__ __forceinline__ void w_update(const BGroup& G, const MinresScalars* S
 }
 
 // v = y / beta, u = J v - (beta/beta_old) r1, previous w/x update, <v,u> partials.
-__device__ void ph_minres_a(const BatchArgs& a, int64_t vb) {
+__device__ double ph_minres_a(const BatchArgs& a, int64_t vb) {
   const Lane L = lane_ctx(a, vb);
   double s = 0.0;
   const MinresScalars* S = L.valid ? &a.S[L.inst] : nullptr;
@@ -331,7 +362,7 @@ __device__ void ph_minres_a(const BatchArgs& a, int64_t vb) {
     G.u[L.idx] = uu;
     s = double(vn.x) * double(uu.x) + double(vn.y) * double(uu.y);
   }
-  lane_partials(a, L, vb, s);
+  return s;
 }
 
 __device__ void ph_flush(const BatchArgs& a, int64_t vb) {
@@ -354,7 +385,7 @@ __device__ void ph_finalize(const BatchArgs& a, int inst, int what, double* out)
   if ((what == 2 || what == 3) && S->done) return;
   int gi = 0;
   while (gi + 1 < a.ngroups && inst >= a.groups[gi + 1].inst0) ++gi;
-  const int nb = int(a.groups[gi].nblocks);
+  const int nb = int(a.groups[gi].nblocks / a.fold);
   double v[1];
   finalize_sum<1>(v, a.partials + int64_t(inst) * a.maxblocks, nb);
   if (threadIdx.x == 0) {
@@ -372,15 +403,55 @@ __device__ void ph_finalize(const BatchArgs& a, int inst, int what, double* out)
 }
 
 // ---------------------------------------------------------------- kernels
-__global__ void kb_compute_f(BatchArgs a) { ph_compute_f(a, blockIdx.x); }
-__global__ void kb_rebuild(BatchArgs a) { ph_rebuild(a, blockIdx.x); }
-__global__ void kb_setup(BatchArgs a) { ph_setup(a, blockIdx.x); }
-__global__ void kb_cheb_first(BatchArgs a, int mode) { ph_cheb_first(a, blockIdx.x, mode); }
-__global__ void kb_cheb_step(BatchArgs a, int mode, int j) {
-  ph_cheb_step(a, blockIdx.x, mode, j);
+// Launched block rb covers the virtual blocks rb*fold .. rb*fold + fold - 1,
+// all of one group (every group's virtual block count is a multiple of fold).
+__device__ __forceinline__ void blk_compute_f(const BatchArgs& a, int64_t rb) {
+  const int64_t v0 = rb * a.fold;
+  double s = 0.0;
+  for (int r = 0; r < a.fold; ++r) s += ph_compute_f(a, v0 + r);
+  lane_partials(a, v0, s);
 }
-__global__ void kb_minres_a(BatchArgs a) { ph_minres_a(a, blockIdx.x); }
-__global__ void kb_flush(BatchArgs a) { ph_flush(a, blockIdx.x); }
+__device__ __forceinline__ void blk_cheb_first(const BatchArgs& a, int64_t rb, int mode) {
+  const int64_t v0 = rb * a.fold;
+  double s = 0.0;
+  for (int r = 0; r < a.fold; ++r) s += ph_cheb_first(a, v0 + r, mode);
+  if (a.iters <= 1) lane_partials(a, v0, s);
+}
+__device__ __forceinline__ void blk_cheb_step(const BatchArgs& a, int64_t rb, int mode, int j) {
+  const int64_t v0 = rb * a.fold;
+  double s = 0.0;
+  for (int r = 0; r < a.fold; ++r) s += ph_cheb_step(a, v0 + r, mode, j);
+  if (j == a.iters - 1) lane_partials(a, v0, s);
+}
+__device__ __forceinline__ void blk_minres_a(const BatchArgs& a, int64_t rb) {
+  const int64_t v0 = rb * a.fold;
+  double s = 0.0;
+  for (int r = 0; r < a.fold; ++r) s += ph_minres_a(a, v0 + r);
+  lane_partials(a, v0, s);
+}
+
+// Launched blocks are remapped so that each XCD works on one contiguous
+// range of row tiles and its L2 holds the x entries those rows gather.
+__global__ void kb_compute_f(BatchArgs a) { blk_compute_f(a, xcd_block(true)); }
+__global__ void kb_rebuild(BatchArgs a) {
+  const int64_t rb = xcd_block(true);
+  for (int r = 0; r < a.fold; ++r) ph_rebuild(a, rb * a.fold + r);
+}
+__global__ void kb_setup(BatchArgs a) {
+  const int64_t rb = xcd_block(true);
+  for (int r = 0; r < a.fold; ++r) ph_setup(a, rb * a.fold + r);
+}
+__global__ void kb_cheb_first(BatchArgs a, int mode) {
+  blk_cheb_first(a, xcd_block(true), mode);
+}
+__global__ void kb_cheb_step(BatchArgs a, int mode, int j) {
+  blk_cheb_step(a, xcd_block(true), mode, j);
+}
+__global__ void kb_minres_a(BatchArgs a) { blk_minres_a(a, xcd_block(true)); }
+__global__ void kb_flush(BatchArgs a) {
+  const int64_t rb = xcd_block(true);
+  for (int r = 0; r < a.fold; ++r) ph_flush(a, rb * a.fold + r);
+}
 __global__ void kb_finalize(BatchArgs a, int what, double* out) {
   ph_finalize(a, blockIdx.x, what, out);
 }
@@ -405,17 +476,17 @@ __global__ void kb_reset(BatchArgs a, int maxit) {
 // same post-barrier state.
 __global__ void kb_minres_persistent(BatchArgs a, int maxit) {
   cg::grid_group grid = cg::this_grid();
-  const int64_t nvb = a.total_blocks;
+  const int64_t nrb = a.total_blocks / a.fold;
   for (int it = 0; it < maxit; ++it) {
-    for (int64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) ph_minres_a(a, vb);
+    for (int64_t rb = blockIdx.x; rb < nrb; rb += gridDim.x) blk_minres_a(a, rb);
     grid.sync();
     for (int inst = blockIdx.x; inst < a.ninst; inst += gridDim.x)
       ph_finalize(a, inst, 2, nullptr);
     grid.sync();
-    for (int64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) ph_cheb_first(a, vb, 2);
+    for (int64_t rb = blockIdx.x; rb < nrb; rb += gridDim.x) blk_cheb_first(a, rb, 2);
     grid.sync();
     for (int j = 1; j < a.iters; ++j) {
-      for (int64_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) ph_cheb_step(a, vb, 2, j);
+      for (int64_t rb = blockIdx.x; rb < nrb; rb += gridDim.x) blk_cheb_step(a, rb, 2, j);
       grid.sync();
     }
     for (int inst = blockIdx.x; inst < a.ninst; inst += gridDim.x)
@@ -431,7 +502,7 @@ __global__ void kb_minres_persistent(BatchArgs a, int maxit) {
 
 // ------------------------------------------------------------------ host
 void Batch::launch_phases_iteration(hipStream_t s) {
-  const unsigned nb = unsigned(args.total_blocks), bt = unsigned(block_threads);
+  const unsigned nb = unsigned(args.total_blocks / args.fold), bt = unsigned(block_threads);
   kb_minres_a<<<nb, bt, 0, s>>>(args);
   kb_finalize<<<args.ninst, 256, 0, s>>>(args, 2, nullptr);
   kb_cheb_first<<<nb, bt, 0, s>>>(args, 2);
@@ -462,13 +533,13 @@ int Batch::persistent_grid() {
   NOSH_HIP(hipGetDevice(&dev));
   NOSH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const int64_t cap = int64_t(std::max(per_cu, 1)) * cus;
-  return int(std::min<int64_t>(cap, args.total_blocks));
+  return int(std::min<int64_t>(cap, args.total_blocks / args.fold));
 }
 
 int Batch::minres(hipStream_t s, int maxit) {
-  NOSH_CHECK(mode == 0 || args.total_blocks <= kPersistentMaxBlocks, kUnsupported,
+  NOSH_CHECK(mode == 0 || args.total_blocks / args.fold <= kPersistentMaxBlocks, kUnsupported,
              "persistent grid too large");
-  const unsigned nb = unsigned(args.total_blocks), bt = unsigned(block_threads);
+  const unsigned nb = unsigned(args.total_blocks / args.fold), bt = unsigned(block_threads);
   kb_reset<<<grid_for(args.ninst, 128), 128, 0, s>>>(args, maxit);
   kb_rebuild<<<nb, bt, 0, s>>>(args);
   kb_setup<<<nb, bt, 0, s>>>(args);
@@ -507,7 +578,7 @@ int Batch::minres(hipStream_t s, int maxit) {
 }
 
 void Batch::compute_norms(hipStream_t s, std::vector<double>& nrm) {
-  const unsigned nb = unsigned(args.total_blocks), bt = unsigned(block_threads);
+  const unsigned nb = unsigned(args.total_blocks / args.fold), bt = unsigned(block_threads);
   kb_compute_f<<<nb, bt, 0, s>>>(args);
   kb_finalize<<<args.ninst, 256, 0, s>>>(args, 0, norms.p);
   nrm.resize(args.ninst);
@@ -517,14 +588,17 @@ void Batch::compute_norms(hipStream_t s, std::vector<double>& nrm) {
   for (double& v : nrm) v = std::sqrt(v);
 }
 
-void Batch::relayout(int threads) {
+void Batch::relayout(int threads, int fold_) {
   block_threads = threads;
+  fold = fold_;
+  args.fold = fold_;
   int64_t b0 = 0, maxb = 0;
   for (auto& gh : ghost) {
     BGroup& G = gh.dev;
     G.block0 = b0;
     const int rows = threads / G.bp;
     G.nblocks = (G.n + rows - 1) / rows;
+    G.nblocks = (G.nblocks + fold - 1) / fold * fold;
     b0 += G.nblocks;
     maxb = std::max(maxb, G.nblocks);
   }

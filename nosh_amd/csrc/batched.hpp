@@ -38,8 +38,9 @@ struct BatchArgs {
   const BGroup* groups;
   int ngroups;
   int ninst;
-  int64_t total_blocks;
+  int64_t total_blocks;  // virtual blocks (block_threads / bp rows each)
   int64_t maxblocks;
+  int fold;              // virtual blocks per launched block
   MinresScalars* S;     // per instance
   const double* mu;     // per instance
   const double* g;      // per instance
@@ -81,12 +82,13 @@ struct Batch {
   DBuf<int> active;
   std::vector<int> active_h;
   int block_threads = 256;
+  int fold = 1;         // virtual blocks per launched block (fewer reduction partials)
   int mode = 0;         // 0: hipGraph-captured iterations, 1: persistent cooperative kernel
   int graph_iters = 4;  // MINRES iterations per captured graph
   int poll = 16;        // iterations between host checks of the done flags
   hipGraphExec_t graph_exec = nullptr;
 
-  void relayout(int threads);
+  void relayout(int threads, int fold);
   void launch_phases_iteration(hipStream_t s);
   void build_graph(hipStream_t s);
   void drop_graph();

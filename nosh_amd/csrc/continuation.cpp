@@ -7,6 +7,8 @@
 // trivial predictor, and step halving when the corrector fails.
 
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -267,6 +269,10 @@ int nosh_continuation_run(nosh_nls_t nls, double* psi, int np, const char* const
       if (!ok) throw nosh::Error(kNotConverged, "continuation: corrector failed after halvings");
       p_cur = target;
       done = k + 1;
+      if (std::getenv("NOSH_PROGRESS"))
+        std::fprintf(stderr, "[continuation] step %d/%d %s %.4f newton %d minres %d %.3fs\n", k,
+                     opts->steps, param.c_str(), target, r.newton_iters, r.minres_iters,
+                     r.seconds);
     }
   });
   if (nls && psi && done > 0) {
@@ -333,14 +339,6 @@ __global__ void k_apply_guess(int64_t n, const double2* __restrict__ po,
   const double s = h * h * sigma;
   out[i] = make_double2(b.x + h * tt.x + s * double(Y[2 * i]),
                         b.y + h * tt.y + s * double(Y[2 * i + 1]));
-}
-
-__global__ void k_gather_rows(const float* __restrict__ src, const int64_t* __restrict__ idx,
-                              int64_t B, int w, float scale, float* __restrict__ dst) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= B * w) return;
-  const int64_t r = i / w;
-  dst[i] = src[idx[r] * w + (i - r * w)] * scale;
 }
 
 __global__ void k_diff_partials(int64_t m, const double* __restrict__ a,
@@ -517,28 +515,58 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int np, const c
           ++sst.fallbacks;
         }
       }
+      int halvings = 0;
       if (!ok) {
-        NOSH_HIP(hipMemcpyAsync(x, tanbuf.p, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice,
-                                s));
-        newton_at(target, opts->newton.max_iters, &st);
-        r.newton_iters += st.newton_iters;
-        r.minres_iters += st.minres_iters_total;
-        ok = st.converged != 0;
+        // Tangent predictor; if Newton fails from it, the step is split in
+        // 2, 4, ... tangent sub-steps (as the plain driver does).
+        int sub = 1;
+        for (;;) {
+          double p_base = p_cur;
+          bool sub_ok = true;
+          const double hsub = hs / sub;
+          for (int q = 0; q < sub && sub_ok; ++q) {
+            if (sub == 1) {
+              NOSH_HIP(hipMemcpyAsync(x, tanbuf.p, sizeof(double) * 2 * n,
+                                      hipMemcpyDeviceToDevice, s));
+            } else {
+              if (q == 0)
+                NOSH_HIP(hipMemcpyAsync(x, w.psi_prev.p, sizeof(double) * 2 * n,
+                                        hipMemcpyDeviceToDevice, s));
+              p[param] = p_base;
+              r.minres_iters += tangent(nls, w, param, x, p, opts->newton, s);
+              k_axpy2<<<grid_for(2 * n, 256), 256, 0, s>>>(2 * n, hsub, w.t.p, x);
+            }
+            newton_at(p_base + hsub, opts->newton.max_iters, &st);
+            r.newton_iters += st.newton_iters;
+            r.minres_iters += st.minres_iters_total;
+            sub_ok = st.converged != 0;
+            if (sub_ok) p_base += hsub;
+          }
+          if (sub_ok) {
+            ok = true;
+            break;
+          }
+          if (halvings >= opts->max_halvings) break;
+          ++halvings;
+          sub *= 2;
+        }
         r.predictor_used = 1;
       }
+      r.halvings = halvings;
       r.norm_f = st.final_norm_f;
       r.converged = ok ? 1 : 0;
       if (!ok) {
         NOSH_HIP(hipMemcpyAsync(x, w.psi_prev.p, sizeof(double) * 2 * n,
                                 hipMemcpyDeviceToDevice, s));
-        throw nosh::Error(kNotConverged, "surrogate continuation: corrector failed");
+        throw nosh::Error(kNotConverged, "surrogate continuation: corrector failed after halvings");
       }
       r.gibbs = gibbs_dev(nls, w, x, s);
       r.tan_error = diff_norm(2 * n, tanbuf.p, x, dpart, dscal, s);
       r.pred_error = r.predictor_used == 3 ? diff_norm(2 * n, guess.p, x, dpart, dscal, s)
                                            : r.tan_error;
-      // Training pairs from the run's own solutions.
-      if (!trained && pairs < P) {
+      // Training pairs from the run's own solutions (whole tangent steps
+      // only: sub-steps overwrite the tangent the features were built from).
+      if (!trained && pairs < P && halvings == 0) {
         NOSH_HIP(hipMemcpyAsync(Xs.p + int64_t(pairs) * n * nin, X.p, sizeof(float) * n * nin,
                                 hipMemcpyDeviceToDevice, s));
         k_target<<<gn, 256, 0, s>>>(n, reinterpret_cast<const double2*>(x),
@@ -560,10 +588,8 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int np, const c
           for (int it = 0; it < so->train_steps; ++it) {
             for (auto& v : idx_h) v = pick(rng);
             idx.upload(idx_h.data(), idx_h.size(), s);
-            k_gather_rows<<<grid_for(int64_t(so->batch) * nin, 256), 256, 0, s>>>(
-                Xs.p, idx.p, so->batch, nin, 1.0f, xb.p);
-            k_gather_rows<<<grid_for(int64_t(so->batch) * 2, 256), 256, 0, s>>>(
-                Ts.p, idx.p, so->batch, 2, float(1.0 / sigma), tb.p);
+            launch_gather_rows(Xs.p, idx.p, so->batch, nin, 1.0f, xb.p, s);
+            launch_gather_rows(Ts.p, idx.p, so->batch, 2, float(1.0 / sigma), tb.p, s);
             lval = net.gradients(xb.p, tb.p, so->batch, s);
             net.adam(so->lr, s);
           }
@@ -578,6 +604,11 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int np, const c
       r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       p_cur = target;
       done = k + 1;
+      if (std::getenv("NOSH_PROGRESS"))
+        std::fprintf(stderr,
+                     "[surrogate continuation] step %d/%d %s %.4f newton %d predictor %d %.3fs\n",
+                     k, opts->steps, param.c_str(), target, r.newton_iters, r.predictor_used,
+                     r.seconds);
     }
   });
   if (nls && psi && done > 0) {

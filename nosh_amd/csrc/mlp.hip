@@ -492,4 +492,19 @@ void Mlp::sync_bf16(hipStream_t s) {
   k_f32_to_bf16<<<g1(nweights), 256, 0, s>>>(master.p, nweights, wbf.p);
 }
 
+namespace {
+__global__ void k_gather_rows(const float* __restrict__ src, const int64_t* __restrict__ idx,
+                              int64_t B, int w, float scale, float* __restrict__ dst) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= B * w) return;
+  const int64_t r = i / w;
+  dst[i] = src[idx[r] * w + (i - r * w)] * scale;
+}
+}  // namespace
+
+void launch_gather_rows(const float* src, const int64_t* idx, int64_t B, int w, float scale,
+                        float* dst, hipStream_t s) {
+  k_gather_rows<<<grid_for(B * w, 256), 256, 0, s>>>(src, idx, B, w, scale, dst);
+}
+
 }  // namespace nosh

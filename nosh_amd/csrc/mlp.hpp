@@ -39,4 +39,8 @@ struct Mlp {
   void sync_bf16(hipStream_t s);
 };
 
+// dst[r][:] = scale * src[idx[r]][:] for r < B rows of width w (minibatch gather).
+void launch_gather_rows(const float* src, const int64_t* idx, int64_t B, int w, float scale,
+                        float* dst, hipStream_t s);
+
 }  // namespace nosh

@@ -1,0 +1,88 @@
+"""GPU: batched continuation (branches stepped in mu together), checked with
+the oracle and against the single-instance FP64 continuation."""
+import numpy as np
+import pytest
+
+from nosh_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+MU0 = np.array([0.0, 0.04, 0.08, 0.12])
+G = np.array([1.0, 0.8, 1.25, 1.0])
+KW = dict(tol=1e-10, max_iters=40, minres_rtol=1e-10, minres_max_iters=4000, forcing=1,
+          forcing_min=1e-6, forcing_max=0.1)
+
+
+def _setup(ncell=8):
+    import nosh_amd as nosh
+    c, t = mg.kuhn_cube(ncell)
+    mesh = nosh.Mesh(c, t)
+    mvp = nosh.VectorFieldExplicitValues(mesh, mg.magnetic_potential(c), 0.0)
+    model = nosh.Nls(mesh, mvp, nosh.ScalarFieldConstant(mesh, -1.0), 1.0,
+                     nosh.ScalarFieldConstant(mesh, 1.0))
+    n = c.shape[0]
+    psi0 = []
+    for g in G:
+        p = np.zeros(2 * n)
+        p[0::2] = 1.0 / np.sqrt(g)
+        psi0.append(p)
+    batch = nosh.Batch([(model, len(G))], MU0, G)
+    return c, t, model, batch, np.concatenate(psi0)
+
+
+def _oracle_check(c, t, mu, psis, tol=1e-9):
+    from oracle import oracle as O
+    om = O.Model(O.Mesh(c, t), mg.magnetic_potential(c))
+    for b, p in enumerate(psis):
+        om.set_params(G[b], mu[b])
+        assert np.linalg.norm(om.f(p)) <= tol, b
+
+
+def test_batch_continuation_secant_matches_single_instance():
+    """Four (mu_0, g) branches over span 0.2 in 10 steps: every branch ends on
+    a solution of F = 0 (oracle) and on the state the single-instance FP64
+    tangent continuation reaches (up to the global phase)."""
+    import nosh_amd as nosh
+    from conftest import phase_align
+    c, t, model, batch, psi0 = _setup()
+    span, steps = 0.2, 10
+    psi, rec, st = batch.continuation(psi0, span, steps, **KW)
+    print("newton", [r["newton_iters"] for r in rec], "minres", [r["minres_iters"] for r in rec])
+    assert st is None
+    assert len(rec) == steps + 1 and all(r["converged"] == len(G) for r in rec)
+    assert max(r["max_norm_f"] for r in rec) <= 1e-10
+    finals = batch.split(psi)
+    _oracle_check(c, t, MU0 + span, finals)
+    for b in range(len(G)):
+        p, r1 = nosh.continuation(model, np.asarray(psi0[b * len(finals[b]):(b + 1) * len(finals[b])]),
+                                  {"g": G[b]}, "mu", MU0[b], MU0[b] + span, steps, predictor=1,
+                                  tol=1e-10, max_iters=40)
+        assert all(r["converged"] for r in r1)
+        d = np.linalg.norm(phase_align(finals[b], p) - p) / np.linalg.norm(p)
+        print("branch", b, "rel diff vs single-instance", d)
+        assert d < 1e-8
+
+
+def test_batch_continuation_surrogate_seeds_and_gains():
+    """Same branches over span 0.24 in 24 steps with the surrogate trained on
+    the first 8 secant steps: it seeds the later steps, its guesses are
+    closer to the converged states than the secant's, and every branch ends
+    on a solution of F = 0 (oracle)."""
+    c, t, model, batch, psi0 = _setup()
+    span, steps, P = 0.24, 24, 8
+    psi, rec, st = batch.continuation(
+        psi0, span, steps,
+        surrogate=dict(train_after=P, train_steps=400, batch=4096, lr=2e-3, newton_budget=4),
+        **KW)
+    print("stats", st)
+    print("newton", [r["newton_iters"] for r in rec])
+    print("pred/sec", [(r["seeded"], round(r["pred_error"], 7), round(r["sec_error"], 7))
+                       for r in rec[P + 2:]])
+    assert len(rec) == steps + 1 and all(r["converged"] == len(G) for r in rec)
+    assert st["trained"] == 1
+    seeded = [r for r in rec if r["seeded"] > 0]
+    assert len(seeded) >= steps - P - 3
+    ratio = np.mean([r["pred_error"] / r["sec_error"] for r in seeded])
+    print("mean pred/secant error ratio", ratio)
+    assert ratio < 0.9
+    _oracle_check(c, t, MU0 + span, batch.split(psi))

@@ -51,8 +51,8 @@ def _check_with_oracle(meshes, counts, mu, g, sols, tol=1e-9):
             k += 1
 
 
-@pytest.mark.parametrize("threads", [128, 256])
-def test_batched_graph_and_persistent_agree(setup, threads):
+@pytest.mark.parametrize("threads,fold", [(128, 1), (256, 1), (256, 4)])
+def test_batched_graph_and_persistent_agree(setup, threads, fold):
     import nosh_amd as nosh
     meshes, counts, models, mu, g, psi0 = setup
     batch = nosh.Batch(list(zip(models, counts)), mu, g)
@@ -62,8 +62,8 @@ def test_batched_graph_and_persistent_agree(setup, threads):
                 cheb_ratio=100.0, forcing=0)
     out = {}
     for mode in (0, 1):
-        print(f"[batched threads={threads} mode={mode}]", flush=True)
-        batch.set_tuning(threads, mode, 4, 16)
+        print(f"[batched threads={threads} fold={fold} mode={mode}]", flush=True)
+        batch.set_tuning(threads, mode, 4, 16, fold=fold)
         psi, st = batch.solve(psi0, **opts)
         assert all(s["converged"] for s in st), st
         assert max(s["final_norm_f"] for s in st) <= 1e-10
@@ -78,11 +78,10 @@ def test_batched_autotune(setup, tmp_path, monkeypatch):
     monkeypatch.setenv("NOSH_AUTOTUNE_CACHE", str(tmp_path / "tune.tsv"))
     meshes, counts, models, mu, g, psi0 = setup
     batch = nosh.Batch(list(zip(models, counts)), mu, g)
-    t, m = batch.autotune(psi0, cheb_iters=4, cheb_ratio=100.0)
-    assert t in (128, 256, 512, 1024) and m in (0, 1)
-    assert (tmp_path / "tune.tsv").read_text().strip().endswith(f"{t} {m}")
-    t2, m2 = batch.autotune(psi0, cheb_iters=4, cheb_ratio=100.0)  # from the cache
-    assert (t2, m2) == (t, m)
+    t, m, f = batch.autotune(psi0, cheb_iters=4, cheb_ratio=100.0)
+    assert t in (128, 256, 512, 1024) and m in (0, 1) and f in (1, 4, 16)
+    assert (tmp_path / "tune.tsv").read_text().strip().endswith(f"{t} {f} {m}")
+    assert batch.autotune(psi0, cheb_iters=4, cheb_ratio=100.0) == (t, m, f)  # from the cache
     psi, st = batch.solve(psi0, tol=1e-10, max_iters=80, minres_rtol=1e-6, cheb_iters=4,
                           cheb_ratio=100.0, forcing=0)
     assert all(s["converged"] for s in st)
