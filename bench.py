@@ -1,18 +1,28 @@
-"""Benchmark: Newton-MINRES solves/sec at ||F||_2 <= 1e-10 on the 96^3 Kuhn cube.
+"""Benchmark: Newton-MINRES solves/sec at ||F||_2 <= 1e-10 on the 96^3 Kuhn
+cube, mixed-precision batched surrogate-seeded continuation against plain
+FP64 GPU Newton-MINRES; and the Jacobian SpMV's HBM GB/s.
 
-Workload (one "step" = one complete Newton-MINRES solve): cube [-5,5]^3,
-96^3 cells split into 6 Kuhn tetrahedra each (912,673 vertices, 5,308,416
-tets, 6,276,384 edges), A = mu*0.5*(-y, x, 0) with mu = 0.4, V = -1, g = 1,
-thickness 1, psi_0 seeded random (|psi| ~ U(0.5,1), arg ~ U(0, 2 pi), seed 0),
-MINRES preconditioned by keo_regularized (Chebyshev-Jacobi). The state is on
-the device before the timed region starts; each step restores psi_0 and
-solves.
+Mesh: cube [-5,5]^3, 96^3 cells split into 6 Kuhn tetrahedra each (912,673
+vertices, 5,308,416 tets, 6,276,384 edges); A = mu*0.5*(-y, x, 0); V = -1;
+thickness 1.
 
-Variants: "plain" = FP64 Newton-MINRES (MINRES rtol 1e-10); "mixed" = FP32
-inner MINRES (values, blocks and Krylov vectors in FP32, inner products and
-scalars in FP64) inside FP64 Newton (F, update) with an Eisenstat-Walker
-forcing term. ``value`` is the solves/sec of the fastest variant that reaches
-||F|| <= 1e-10; both are reported.
+Headline workload (``value``; one step = one continuation step of every
+branch): mu-continuation from mu = 0 to 1 in 50 equal steps of 8 branches
+(mu_0 = 0, g_b = 0.6 + 0.2 b, psi_0 = 1/sqrt(g_b), exact at mu = 0), each step
+solved to ||F|| <= 1e-10. Combined method: the branches batched in one
+instance-interleaved state (one launch per phase, hipGraph-captured MINRES
+iterations, autotuned block size), FP32 inner MINRES with FP64 Newton
+(Eisenstat-Walker forcing), secant predictor, and the BF16 MFMA MLP surrogate
+trained online on the run's own solutions seeding Newton where its guess has
+the smaller residual (tangent-free fallback to the secant). ``value`` = 8 x
+50 / wall time of the run (training and inference included). Plain
+baseline: the single-instance FP64 continuation (MINRES rtol 1e-10, tangent
+predictor, Chebyshev-Jacobi(6, 300), its fastest setting) of branch 0; the
+ratio is value / its solves/sec.
+
+Side by side (the 96^3 random-start solve): plain FP64 and FP32-inner
+Newton-MINRES from psi_0 seeded random (|psi| ~ U(0.5,1), arg ~ U(0, 2 pi),
+seed 0) at mu = 0.4, g = 1, with ||F|| and ||psi - psi_FP64||.
 
 Roofline: the FP64 Jacobian apply J(psi) phi, timed with HIP events on the
 stream it is launched on; algorithmic bytes per launch 40E + 68n (complex CSR
@@ -21,9 +31,9 @@ row pointer and 8 B diagonal per row, phi / y / psi 16 B and c*t 8 B per
 vertex) against 8 TB/s.
 
 CPU baseline: the oracle (a C++ restatement of the reference operators and
-solver, OpenMP) on a bounded sample of the same workload on this host: the
-cost of F, the Jacobian rebuild and MINRES iterations is measured and
-combined with the GPU solve's own Newton/MINRES iteration counts.
+solver, OpenMP) on a bounded sample on this host: the cost of F, the
+Jacobian rebuild and MINRES iterations is measured and combined with the
+plain continuation's own per-step Newton / MINRES counts.
 """
 from __future__ import annotations
 
@@ -39,16 +49,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "Newton-MINRES solves/sec at ||F||<=1e-10 vs plain FP64 GPU; SpMV HBM GB/s"
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=2)
-    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--steps", type=int, default=None,
+                   help="timed steps: continuation steps (default 50; every branch solved "
+                        "once per step), or batch solves for --workload batched (default 1)")
+    p.add_argument("--warmup", type=int, default=None,
+                   help="untimed steps before the timed run (default 2 continuation steps, "
+                        "1 batch solve)")
     p.add_argument("--ncell", type=int, default=96)
-    p.add_argument("--mu", type=float, default=0.4)
-    p.add_argument("--cheb", type=int, default=6)
+    p.add_argument("--mu", type=float, default=0.4, help="mu of the random-start solves")
+    p.add_argument("--solve-reps", type=int, default=2,
+                   help="random-start solves timed per variant")
+    p.add_argument("--cheb", type=int, default=6, help="Chebyshev degree, plain FP64 solver")
+    p.add_argument("--cont-cheb", type=int, default=4, help="Chebyshev degree, batched solver")
     p.add_argument("--ratio", type=float, default=300.0)
     p.add_argument("--minres-maxit", type=int, default=2000)
     p.add_argument("--spmv-reps", type=int, default=50)
@@ -59,26 +77,32 @@ def parse():
     p.add_argument("--verbose", action="store_true")
     p.add_argument("--cont-branches", type=int, default=8)
     p.add_argument("--cont-span", type=float, default=1.0)
-    p.add_argument("--cont-steps", type=int, default=50)
     p.add_argument("--cont-train-after", type=int, default=10)
+    p.add_argument("--cont-retrain", type=int, default=0,
+                   help="Adam steps per continuation step after the first training")
+    p.add_argument("--cont-batch", type=int, default=8192, help="surrogate minibatch rows")
     p.add_argument("--cont-plain-branches", type=int, default=1,
                    help="branches run with the plain FP64 continuation for the ratio")
-    p.add_argument("--no-cont", action="store_true")
     p.add_argument("--cont-tuning", default="",
                    help="block_threads,fold for the batched continuation (skips the autotuner)")
     p.add_argument("--no-surrogate", action="store_true")
-    p.add_argument("--workload", default="kuhn96", choices=["kuhn96", "batched", "cont"],
-                   help="kuhn96: the headline single-instance workload; batched: 128 uneven "
-                        "instances on jittered 16^3/24^3/32^3/48^3 cubes")
+    p.add_argument("--workload", default="cont96", choices=["cont96", "batched", "cont"],
+                   help="cont96: the headline (continuation, side-by-side solves, roofline, "
+                        "CPU baseline); cont: the continuation part only; batched: 128 uneven "
+                        "random-start instances on jittered 16^3/24^3/32^3/48^3 cubes")
     p.add_argument("--instances", type=int, default=128)
     p.add_argument("--plain-sample", type=int, default=8,
                    help="batched: instances solved one by one with plain FP64 for the ratio")
-    return p.parse_args()
+    a = p.parse_args()
+    if a.steps is None:
+        a.steps = 1 if a.workload == "batched" else 50
+    if a.warmup is None:
+        a.warmup = 1 if a.workload == "batched" else 2
+    return a
 
 
-def log(args, *a):
-    if args.verbose:
-        print(*a, file=sys.stderr, flush=True)
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
 def _profiled_traffic(ncell):
@@ -93,8 +117,9 @@ def _profiled_traffic(ncell):
     return None, None
 
 
-def cpu_baseline(coords, cells, A, mu, psi0, phi, gpu_stats, cheb, ratio):
-    """Time the oracle on this host; extrapolate one solve from the GPU counts."""
+def cpu_baseline(coords, cells, A, mu, psi0, phi, counts, cheb, ratio, what):
+    """Time the oracle on this host; extrapolate one solve from the GPU
+    solver's per-solve counts (Newton steps and MINRES iterations)."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
@@ -124,16 +149,361 @@ def cpu_baseline(coords, cells, A, mu, psi0, phi, gpu_stats, cheb, ratio):
     s = time.perf_counter()
     _, k, _ = om.minres(b, rtol=1e-30, maxit=k_sample, cheb_iters=cheb, ratio=ratio)
     t_it = (time.perf_counter() - s) / max(k, 1)
-    steps = gpu_stats["newton_iters"]
-    its = gpu_stats["minres_iters_total"]
+    steps = counts["newton_iters"]
+    its = counts["minres_iters"]
     t_solve = (steps + 1) * t_f + steps * t_rb + its * t_it
-    sample = (f"oracle on {threads} threads: F {t_f*1e3:.1f} ms, K.phi {t_k*1e3:.1f} ms, "
-              f"J.phi {t_j*1e3:.1f} ms, rebuild {t_rb*1e3:.1f} ms, MINRES iteration "
-              f"{t_it*1e3:.1f} ms (mean of {k} with Chebyshev {cheb}); one solve = "
-              f"{steps + 1} F + {steps} rebuilds + {its} MINRES iterations, the plain GPU "
-              f"solve's counts; setup {setup:.1f} s")
+    sample = (f"oracle on {threads} threads, 96^3 cube: F {t_f*1e3:.1f} ms, K.phi "
+              f"{t_k*1e3:.1f} ms, J.phi {t_j*1e3:.1f} ms, rebuild {t_rb*1e3:.1f} ms, MINRES "
+              f"iteration {t_it*1e3:.1f} ms (mean of {k} with Chebyshev {cheb}); one solve = "
+              f"{steps + 1:.2f} F + {steps:.2f} rebuilds + {its:.1f} MINRES iterations, the "
+              f"{what}'s per-solve counts; setup {setup:.1f} s")
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
             "sample": sample, "jphi_ms": t_j * 1e3, "seconds_per_solve": t_solve}
+
+
+def _phase_align(psi, ref):
+    z = psi[0::2] + 1j * psi[1::2]
+    zr = ref[0::2] + 1j * ref[1::2]
+    w = z * np.exp(-1j * np.angle(np.vdot(zr, z)))
+    out = np.empty_like(psi)
+    out[0::2], out[1::2] = w.real, w.imag
+    return out
+
+
+def build_model(ncell):
+    import nosh_amd as nosh
+    from nosh_amd import meshgen as mg
+    coords, cells = mg.kuhn_cube(ncell)
+    t0 = time.perf_counter()
+    mesh = nosh.Mesh(coords, cells)
+    t_mesh = time.perf_counter() - t0
+    A = mg.magnetic_potential(coords)
+    mvp = nosh.VectorFieldExplicitValues(mesh, A, 0.0)
+    model = nosh.Nls(mesh, mvp, nosh.ScalarFieldConstant(mesh, -1.0), 1.0,
+                     nosh.ScalarFieldConstant(mesh, 1.0))
+    return coords, cells, A, mesh, model, t_mesh
+
+
+def continuation_phase(args, model, n):
+    """mu-continuation of several (mu_0, g) branches on one mesh: the batched,
+    mixed-precision, surrogate-seeded driver (timed over args.steps steps
+    after args.warmup untimed ones) against the plain FP64 single-instance
+    tangent continuation of a sample of the branches."""
+    import nosh_amd as nosh
+    nb, span, steps = args.cont_branches, args.cont_span, args.steps
+    g = 0.6 + 0.2 * np.arange(nb)
+    mu0 = np.zeros(nb)
+    psi0 = np.zeros(2 * n * nb)
+    for b in range(nb):
+        psi0[2 * n * b:2 * n * (b + 1):2] = 1.0 / np.sqrt(g[b])
+    kw = dict(tol=1e-10, max_iters=40, minres_rtol=1e-10, minres_max_iters=4000,
+              cheb_iters=args.cont_cheb, cheb_ratio=args.ratio, forcing=1, forcing_min=1e-6,
+              forcing_max=0.1)
+    batch = nosh.Batch([(model, nb)], mu0, g)
+    t0 = time.perf_counter()
+    if args.cont_tuning:
+        bt, fold = (int(v) for v in args.cont_tuning.split(","))
+        batch.set_tuning(bt, 0, 4, 16, fold=fold)
+        tuned = (bt, 0, fold)
+    else:
+        tuned = batch.autotune(psi0, use_cache=False, **kw)
+    t_tune = time.perf_counter() - t0
+    say(f"continuation: autotune {t_tune:.2f}s -> {tuned}")
+    sur = None if args.no_surrogate else dict(
+        width=256, depth=4, train_after=args.cont_train_after, train_steps=300,
+        batch=args.cont_batch, lr=1e-3, newton_budget=6, retrain_steps=args.cont_retrain)
+    if args.warmup > 0:
+        t0 = time.perf_counter()
+        batch.continuation(psi0, span * args.warmup / steps, args.warmup, **kw)
+        say(f"continuation: warmup {args.warmup} steps {time.perf_counter() - t0:.2f}s")
+    t0 = time.perf_counter()
+    psi, rec, sst = batch.continuation(psi0, span, steps, surrogate=sur, **kw)
+    t_comb = time.perf_counter() - t0
+    conv = len(rec) == steps + 1 and all(r["converged"] == nb for r in rec)
+    comb_rate = nb * steps / t_comb
+    say(f"continuation: combined {t_comb:.2f}s for {nb}x{steps} solves -> {comb_rate:.3f}/s, "
+        f"converged {conv}, surrogate {sst}")
+    finals = batch.split(psi)
+    # Error of the mixed-precision states against FP64: FP64 Newton from each
+    # final state (same branch) and the distance to where it lands.
+    fp64_err = []
+    for b in sorted({0, nb - 1}):
+        p, s64 = nosh.newton_solve(model, finals[b], {"g": g[b], "mu": mu0[b] + span}, tol=1e-12,
+                                   max_iters=10, minres_rtol=1e-12,
+                                   minres_max_iters=args.minres_maxit, cheb_iters=args.cheb,
+                                   cheb_ratio=args.ratio)
+        fp64_err.append(float(np.linalg.norm(_phase_align(finals[b], p) - p) /
+                              np.linalg.norm(p)))
+    say(f"continuation: mixed vs FP64-polished final states, rel. err. max {max(fp64_err):.2e}")
+    t_plain, plain_steps, plain_rec = 0.0, 0, []
+    for b in range(min(args.cont_plain_branches, nb)):
+        t1 = time.perf_counter()
+        p, r1 = nosh.continuation(model, psi0[2 * n * b:2 * n * (b + 1)], {"g": g[b]}, "mu",
+                                  mu0[b], mu0[b] + span, steps, predictor=1, tol=1e-10,
+                                  max_iters=40, minres_rtol=1e-10,
+                                  minres_max_iters=args.minres_maxit, cheb_iters=args.cheb,
+                                  cheb_ratio=args.ratio)
+        dt = time.perf_counter() - t1
+        t_plain += dt
+        plain_steps += steps
+        ok = len(r1) == steps + 1 and all(r["converged"] for r in r1)
+        plain_rec.append({"branch": b, "seconds": round(dt, 3), "converged": ok,
+                          "newton_iters": sum(r["newton_iters"] for r in r1[1:]),
+                          "minres_iters": sum(r["minres_iters"] for r in r1[1:]),
+                          "halvings": sum(r["halvings"] for r in r1[1:]),
+                          "final_gibbs": r1[-1]["gibbs"]})
+        say(f"continuation: plain branch {b}: {dt:.2f}s, {plain_rec[-1]}")
+    plain_rate = plain_steps / t_plain if t_plain > 0 else None
+    seeded = [r for r in rec if r["seeded"]]
+    return {
+        "solves_per_s": comb_rate, "seconds": t_comb, "converged": conv,
+        "branches": nb, "steps": steps, "span": span, "g": [round(v, 3) for v in g],
+        "autotuned": {"block_threads": tuned[0], "fold": tuned[2],
+                      "mode": ["hipGraph", "persistent"][tuned[1]], "seconds": round(t_tune, 2)},
+        "lockstep_newton_iters": sum(r["newton_iters"] for r in rec),
+        "lockstep_minres_iters": sum(r["minres_iters"] for r in rec),
+        "max_norm_f": max(r["max_norm_f"] for r in rec),
+        "rel_err_vs_fp64_polished": {"branches": sorted({0, nb - 1}), "values": fp64_err},
+        "surrogate": sst,
+        "seeded_branch_steps": sum(r["seeded"] for r in rec),
+        "fallback_branch_steps": sum(r["fallbacks"] for r in rec),
+        "mean_pred_error_seeded_steps": (float(np.mean([r["pred_error"] for r in seeded]))
+                                         if seeded else None),
+        "mean_secant_error_seeded_steps": (float(np.mean([r["sec_error"] for r in seeded]))
+                                           if seeded else None),
+        "per_step_seconds": [round(r["seconds"], 4) for r in rec],
+        "per_step_newton": [r["newton_iters"] for r in rec],
+        "plain_fp64_solves_per_s": plain_rate, "plain_sample": plain_rec,
+        "speedup_vs_plain_fp64": comb_rate / plain_rate if plain_rate else None,
+    }
+
+
+def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev):
+    """FP64 J(psi) phi timed on the library's launch stream (warm and cold)."""
+    import torch
+    E, n = mesh.num_edges, mesh.num_vertices
+    J = model.create_W_op()
+    J.rebuild(params, psi0)
+    x_ref = torch.from_numpy(phi).to(dev)
+    x = torch.empty_like(x_ref)
+    y = torch.empty_like(x_ref)
+    mesh.to_internal(x_ref.data_ptr(), x.data_ptr(), stream.cuda_stream)
+    with torch.cuda.stream(stream):
+        for _ in range(5):
+            J.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(args.spmv_reps):
+            J.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+        ev1.record(stream)
+    ev1.synchronize()
+    t_j = ev0.elapsed_time(ev1) / 1e3 / args.spmv_reps
+    bytes_canon = 40 * E + 68 * n
+    ach = bytes_canon / t_j / 1e9
+    # Cold-cache variant: reading 1 GiB between launches replaces the 256 MB
+    # Infinity Cache and the L2s with clean lines, so every launch streams its
+    # operands from HBM (a write-based flush would leave dirty lines whose
+    # write-back the next launch would pay for).
+    flush = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+    cold = []
+    with torch.cuda.stream(stream):
+        for _ in range(10):
+            fsum = flush.sum()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            J.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
+            e1.record(stream)
+            cold.append((e0, e1))
+    torch.cuda.synchronize()
+    t_cold = float(np.median([a.elapsed_time(b) for a, b in cold])) / 1e3
+    del flush, fsum
+    layout = mesh.layout()
+    bytes_layout = 20 * layout["nnz_pad"] + 4 * (layout["nslices"] + 1) + 56 * n
+    # Copy-kernel reference of the same size class (stream bandwidth).
+    cbuf = torch.empty(bytes_canon // 8, dtype=torch.float64, device=dev)
+    cdst = torch.empty_like(cbuf)
+    with torch.cuda.stream(stream):
+        for _ in range(3):
+            cdst.copy_(cbuf)
+        ev0.record(stream)
+        for _ in range(20):
+            cdst.copy_(cbuf)
+        ev1.record(stream)
+    ev1.synchronize()
+    t_copy = ev0.elapsed_time(ev1) / 1e3 / 20
+    copy_gbs = 2 * cbuf.numel() * 8 / t_copy / 1e9
+    del cbuf, cdst
+    # FP32 J apply (inner solver's operator).
+    x32 = x.float()
+    y32 = torch.empty_like(x32)
+    with torch.cuda.stream(stream):
+        J.apply_internal_fp32(x32.data_ptr(), y32.data_ptr(), stream.cuda_stream)
+        ev0.record(stream)
+        for _ in range(args.spmv_reps):
+            J.apply_internal_fp32(x32.data_ptr(), y32.data_ptr(), stream.cuda_stream)
+        ev1.record(stream)
+    ev1.synchronize()
+    t_j32 = ev0.elapsed_time(ev1) / 1e3 / args.spmv_reps  # includes the block conversion
+    traffic, traffic_src = _profiled_traffic(args.ncell)
+    say(f"J.phi {t_j*1e6:.1f} us -> {ach:.0f} GB/s ({ach/HBM_PEAK_GBS:.1%}); copy "
+        f"{copy_gbs:.0f} GB/s; fp32 J.phi {t_j32*1e6:.1f} us")
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": traffic_src,
+            "kernel": "k_block_apply<double> (J.phi FP64)",
+            "us_per_launch": round(t_j * 1e6, 2), "bytes_per_launch": bytes_canon,
+            "cold_us_per_launch": round(t_cold * 1e6, 2),
+            "cold_achieved": round(bytes_canon / t_cold / 1e9, 1),
+            "cold_frac": round(bytes_canon / t_cold / 1e9 / HBM_PEAK_GBS, 4),
+            "copy_kernel_GBs": round(copy_gbs, 1),
+            "layout_bytes_per_launch": bytes_layout,
+            "layout_note": "sliced ELLPACK (64-row slices) of the off-diagonal entries: "
+                           "16 B value + 4 B column per padded slot, 4 B per slice, "
+                           "24 B 2x2 block + 16 B x + 16 B y per vertex",
+            "nnz_pad": layout["nnz_pad"], "jphi_fp32_us": round(t_j32 * 1e6, 2)}
+
+
+def surrogate_mfma(rows, stream, dev, nin=9, width=256, depth=4, reps=5):
+    """Fused MLP inference (the surrogate's per-step cost) on `rows` feature
+    rows: TFLOP/s of its dense layers against the bf16 MFMA peak."""
+    import torch
+    import nosh_amd as nosh
+    net = nosh.Mlp(nin, width, depth, 2, 1)
+    X = torch.randn(rows, nin, dtype=torch.float32, device=dev)
+    Y = torch.empty(rows, 2, dtype=torch.float32, device=dev)
+    with torch.cuda.stream(stream):
+        net.infer_device(X.data_ptr(), rows, Y.data_ptr(), stream.cuda_stream)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            net.infer_device(X.data_ptr(), rows, Y.data_ptr(), stream.cuda_stream)
+        e1.record(stream)
+    e1.synchronize()
+    t = e0.elapsed_time(e1) / 1e3 / reps
+    flop_mfma = 2.0 * rows * (depth - 1) * width * width
+    flop_all = 2.0 * rows * (nin * width + (depth - 1) * width * width + width * 2)
+    ach = flop_mfma / t / 1e12
+    say(f"surrogate inference: {rows} rows in {t*1e3:.2f} ms -> {ach:.1f} TFLOP/s on MFMA")
+    return {"bound": "mfma", "kernel": "k_mlp_infer<8> (fused BF16 MLP inference)",
+            "rows": rows, "ms_per_launch": round(t * 1e3, 3), "achieved": round(ach, 1),
+            "peak": 2500.0, "unit": "TFLOP/s", "frac": round(ach / 2500.0, 4),
+            "all_layers_TFLOPs": round(flop_all / t / 1e12, 1)}
+
+
+def single_solves(args, model, params, psi0, stream, dev):
+    """The random-start solve, plain FP64 and FP32-inner, side by side."""
+    import torch
+    import nosh_amd as nosh
+    psi_dev0 = torch.from_numpy(psi0).to(dev)
+    psi = torch.empty_like(psi_dev0)
+    results, sols = {}, {}
+    for var in [v for v in args.variants.split(",") if v]:
+        kw = dict(tol=1e-10, max_iters=40, minres_rtol=1e-10,
+                  minres_max_iters=args.minres_maxit, cheb_iters=args.cheb,
+                  cheb_ratio=args.ratio)
+        if var == "mixed":
+            kw.update(precision=1, forcing=1, forcing_min=1e-6, forcing_max=0.1)
+        opts = nosh.default_newton_options(**kw)
+        psi.copy_(psi_dev0)
+        torch.cuda.synchronize()
+        nosh.newton_solve_device(model, psi.data_ptr(), params, opts, stream.cuda_stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        all_stats = []
+        for _ in range(args.solve_reps):
+            with torch.cuda.stream(stream):
+                psi.copy_(psi_dev0)
+            all_stats.append(nosh.newton_solve_device(model, psi.data_ptr(), params, opts,
+                                                      stream.cuda_stream))
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        ok = all(s["converged"] and s["final_norm_f"] <= 1e-10 for s in all_stats)
+        sols[var] = psi.cpu().numpy()
+        results[var] = {"solves_per_s": args.solve_reps / dt,
+                        "s_per_solve": dt / args.solve_reps, "converged": ok,
+                        "final_norm_f": all_stats[-1]["final_norm_f"],
+                        "newton_iters": all_stats[-1]["newton_iters"],
+                        "minres_iters": all_stats[-1]["minres_iters_total"]}
+        say(var, results[var])
+    if "plain" in sols:
+        ref = sols["plain"]
+        for var, s in sols.items():
+            d = np.linalg.norm(_phase_align(s, ref) - ref)
+            results[var]["rel_diff_vs_fp64"] = float(d / max(np.linalg.norm(ref), 1e-300))
+    return results
+
+
+def headline(args):
+    import torch
+    from nosh_amd import meshgen as mg
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    coords, cells, A, mesh, model, t_mesh = build_model(args.ncell)
+    n, E = mesh.num_vertices, mesh.num_edges
+    say(f"mesh {args.ncell}^3: n={n} E={E} build {t_mesh:.2f}s")
+    params = {"g": 1.0, "mu": args.mu}
+    psi0 = mg.random_state(n, 0)
+    phi = mg.random_state(n, 1)
+    stream = torch.cuda.Stream(device=dev)
+    roofline = jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev)
+    if args.roofline_only:
+        print(json.dumps({"roofline": roofline}))
+        return
+    solves = single_solves(args, model, params, psi0, stream, dev)
+    mfma = surrogate_mfma(n * args.cont_branches, stream, dev)
+    cont = continuation_phase(args, model, n)
+    cpu = None
+    if not args.no_cpu and cont["plain_sample"]:
+        ps = cont["plain_sample"][0]
+        cpu = cpu_baseline(coords, cells, A, 0.5 * args.cont_span, psi0, phi,
+                           {"newton_iters": ps["newton_iters"] / args.steps,
+                            "minres_iters": ps["minres_iters"] / args.steps},
+                           args.cheb, args.ratio, "plain FP64 continuation")
+    value = cont["solves_per_s"] if cont["converged"] else 0.0
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(cont["seconds"] / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64/f32 (FP32 inner MINRES, FP64 Newton); bf16/f32 MFMA surrogate",
+        "data": "synthetic (structured Kuhn mesh; branch start states 1/sqrt(g); seeded "
+                "SplitMix64 psi_0 for the random-start solves)",
+        "config": {"workload": f"Kuhn-tet cube {args.ncell}^3: mu-continuation 0 -> "
+                               f"{args.cont_span} in {args.steps} steps of "
+                               f"{args.cont_branches} (mu_0 = 0, g) branches, batched",
+                   "n_vertices": n, "n_edges": E, "n_tets": int(cells.shape[0]),
+                   "V": -1.0, "branches_g": cont["g"],
+                   "combined": f"batched FP32-inner/FP64 Newton, secant + surrogate predictor, "
+                               f"Chebyshev-Jacobi({args.cont_cheb}, {args.ratio})",
+                   "plain": f"FP64 Newton-MINRES rtol 1e-10, tangent predictor, "
+                            f"Chebyshev-Jacobi({args.cheb}, {args.ratio})"},
+        "speedup_vs_plain_fp64": (round(cont["speedup_vs_plain_fp64"], 3)
+                                  if cont["speedup_vs_plain_fp64"] else None),
+        "continuation": cont,
+        "random_start_solves": solves,
+        "roofline": roofline,
+        "surrogate_mfma": mfma,
+        "mesh_build_s": round(t_mesh, 3),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def cont_workload(args):
+    import torch
+    torch.cuda.init()
+    coords, cells, A, mesh, model, t_mesh = build_model(args.ncell)
+    say(f"cont workload: mesh {args.ncell}^3, n={mesh.num_vertices}")
+    print(json.dumps(continuation_phase(args, model, mesh.num_vertices)))
 
 
 def batched_workload(args):
@@ -232,112 +602,6 @@ def batched_workload(args):
     print(json.dumps(out))
 
 
-def _phase_align(psi, ref):
-    z = psi[0::2] + 1j * psi[1::2]
-    zr = ref[0::2] + 1j * ref[1::2]
-    w = z * np.exp(-1j * np.angle(np.vdot(zr, z)))
-    out = np.empty_like(psi)
-    out[0::2], out[1::2] = w.real, w.imag
-    return out
-
-
-def continuation_phase(args, model, n, say):
-    """mu-continuation of several (mu_0, g) branches on one mesh: the
-    batched, mixed-precision, surrogate-seeded driver against the plain FP64
-    single-instance tangent continuation on a sample of the branches.
-    Branch b: g_b = 0.6 + 0.2 b, mu_0 = 0, psi_0 = 1/sqrt(g_b) (exact at
-    mu = 0); mu runs to mu_0 + span in equal steps."""
-    import nosh_amd as nosh
-    nb, span, steps = args.cont_branches, args.cont_span, args.cont_steps
-    g = 0.6 + 0.2 * np.arange(nb)
-    mu0 = np.zeros(nb)
-    psi0 = np.zeros(2 * n * nb)
-    for b in range(nb):
-        psi0[2 * n * b:2 * n * (b + 1):2] = 1.0 / np.sqrt(g[b])
-    kw = dict(tol=1e-10, max_iters=40, minres_rtol=1e-10, minres_max_iters=4000,
-              cheb_iters=args.cheb, cheb_ratio=args.ratio, forcing=1, forcing_min=1e-6,
-              forcing_max=0.1)
-    batch = nosh.Batch([(model, nb)], mu0, g)
-    t0 = time.perf_counter()
-    if args.cont_tuning:
-        bt, fold = (int(v) for v in args.cont_tuning.split(","))
-        batch.set_tuning(bt, 0, 4, 16, fold=fold)
-        tuned = (bt, 0, fold)
-    else:
-        tuned = batch.autotune(psi0, use_cache=False, **kw)
-    say(f"continuation: autotune {time.perf_counter() - t0:.2f}s -> {tuned}")
-    sur = None if args.no_surrogate else dict(
-        width=256, depth=4, train_after=args.cont_train_after, train_steps=300, batch=16384,
-        lr=1e-3, newton_budget=6)
-    t0 = time.perf_counter()
-    psi, rec, sst = batch.continuation(psi0, span, steps, surrogate=sur, **kw)
-    t_comb = time.perf_counter() - t0
-    conv = len(rec) == steps + 1 and all(r["converged"] == nb for r in rec)
-    comb_rate = nb * steps / t_comb
-    say(f"continuation: combined {t_comb:.2f}s for {nb}x{steps} solves -> {comb_rate:.3f}/s, "
-        f"converged {conv}, surrogate {sst}")
-    finals = batch.split(psi)
-    # Plain FP64 continuation of a sample of the branches, one at a time.
-    t_plain, plain_steps, diffs, plain_rec = 0.0, 0, [], []
-    for b in range(min(args.cont_plain_branches, nb)):
-        t1 = time.perf_counter()
-        p, r1 = nosh.continuation(model, psi0[2 * n * b:2 * n * (b + 1)], {"g": g[b]}, "mu",
-                                  mu0[b], mu0[b] + span, steps, predictor=1, tol=1e-10,
-                                  max_iters=40, minres_rtol=1e-10,
-                                  minres_max_iters=args.minres_maxit, cheb_iters=args.cheb,
-                                  cheb_ratio=args.ratio)
-        dt = time.perf_counter() - t1
-        t_plain += dt
-        plain_steps += steps
-        ok = len(r1) == steps + 1 and all(r["converged"] for r in r1)
-        d = float(np.linalg.norm(_phase_align(finals[b], p) - p) / np.linalg.norm(p))
-        diffs.append(d)
-        plain_rec.append({"branch": b, "seconds": round(dt, 3), "converged": ok,
-                          "newton_iters": sum(r["newton_iters"] for r in r1),
-                          "minres_iters": sum(r["minres_iters"] for r in r1),
-                          "halvings": sum(r["halvings"] for r in r1)})
-        say(f"continuation: plain branch {b}: {dt:.2f}s, {plain_rec[-1]}, rel diff {d:.2e}")
-    plain_rate = plain_steps / t_plain if t_plain > 0 else None
-    return {
-        "solves_per_s": comb_rate, "seconds": t_comb, "converged": conv,
-        "branches": nb, "steps": steps, "span": span, "g": [round(v, 3) for v in g],
-        "autotuned": {"block_threads": tuned[0], "fold": tuned[2],
-                      "mode": ["hipGraph", "persistent"][tuned[1]]},
-        "lockstep_newton_iters": sum(r["newton_iters"] for r in rec),
-        "lockstep_minres_iters": sum(r["minres_iters"] for r in rec),
-        "max_norm_f": max(r["max_norm_f"] for r in rec),
-        "seeded_branch_steps": sum(r["seeded"] for r in rec),
-        "fallback_branch_steps": sum(r["fallbacks"] for r in rec),
-        "surrogate": sst,
-        "mean_pred_error_seeded": (float(np.mean([r["pred_error"] for r in rec if r["seeded"]]))
-                                   if any(r["seeded"] for r in rec) else None),
-        "mean_secant_error_seeded": (float(np.mean([r["sec_error"] for r in rec if r["seeded"]]))
-                                     if any(r["seeded"] for r in rec) else None),
-        "per_step_seconds": [round(r["seconds"], 4) for r in rec],
-        "per_step_newton": [r["newton_iters"] for r in rec],
-        "plain_fp64_solves_per_s": plain_rate, "plain_sample": plain_rec,
-        "rel_diff_vs_plain_fp64": diffs,
-        "speedup_vs_plain_fp64": comb_rate / plain_rate if plain_rate else None,
-    }
-
-
-def cont_workload(args):
-    import torch
-    import nosh_amd as nosh
-    from nosh_amd import meshgen as mg
-
-    def say(*a):
-        print(*a, file=sys.stderr, flush=True)
-    torch.cuda.init()
-    coords, cells = mg.kuhn_cube(args.ncell)
-    n = coords.shape[0]
-    mesh = nosh.Mesh(coords, cells)
-    mvp = nosh.VectorFieldExplicitValues(mesh, mg.magnetic_potential(coords), 0.0)
-    model = nosh.Nls(mesh, mvp, nosh.ScalarFieldConstant(mesh, -1.0), 1.0,
-                     nosh.ScalarFieldConstant(mesh, 1.0))
-    say(f"cont workload: mesh {args.ncell}^3, n={n}")
-    res = continuation_phase(args, model, n, say)
-    print(json.dumps(res))
 
 
 def main():
@@ -348,205 +612,10 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         batched_workload(args)
-        return
-    if args.workload == "cont":
+    elif args.workload == "cont":
         cont_workload(args)
-        return
-    import torch
-    import nosh_amd as nosh
-    from nosh_amd import meshgen as mg
-
-    dev = torch.device("cuda:0")
-    torch.cuda.init()
-    nc = args.ncell
-    coords, cells = mg.kuhn_cube(nc)
-    n = coords.shape[0]
-    A = mg.magnetic_potential(coords)
-    mu, g = args.mu, 1.0
-    params = {"g": g, "mu": mu}
-    psi0 = mg.random_state(n, 0)
-    phi = mg.random_state(n, 1)
-
-    t0 = time.perf_counter()
-    mesh = nosh.Mesh(coords, cells)
-    t_mesh = time.perf_counter() - t0
-    E = mesh.num_edges
-    log(args, f"mesh n={n} E={E} build {t_mesh:.2f}s")
-    mvp = nosh.VectorFieldExplicitValues(mesh, A, mu)
-    V = nosh.ScalarFieldConstant(mesh, -1.0)
-    t = nosh.ScalarFieldConstant(mesh, 1.0)
-    model = nosh.Nls(mesh, mvp, V, g, t)
-
-    # ---- roofline: FP64 J(psi) phi on the library's launch stream.
-    J = model.create_W_op()
-    J.rebuild(params, psi0)
-    stream = torch.cuda.Stream(device=dev)
-    x_ref = torch.from_numpy(phi).to(dev)
-    x = torch.empty_like(x_ref)
-    y = torch.empty_like(x_ref)
-    mesh.to_internal(x_ref.data_ptr(), x.data_ptr(), stream.cuda_stream)
-    with torch.cuda.stream(stream):
-        for _ in range(5):
-            J.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
-        ev0.record(stream)
-        for _ in range(args.spmv_reps):
-            J.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
-        ev1.record(stream)
-    ev1.synchronize()
-    t_j = ev0.elapsed_time(ev1) / 1e3 / args.spmv_reps
-    bytes_canon = 40 * E + 68 * n
-    ach = bytes_canon / t_j / 1e9
-    # Cold-cache variant: reading 1 GiB between launches replaces the 256 MB
-    # Infinity Cache and the L2s with clean lines, so every launch streams its
-    # operands from HBM (a write-based flush would leave dirty lines whose
-    # write-back the next launch would pay for).
-    flush = torch.ones(1 << 27, dtype=torch.float64, device=dev)
-    cold = []
-    with torch.cuda.stream(stream):
-        for _ in range(10):
-            fsum = flush.sum()
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            J.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
-            e1.record(stream)
-            cold.append((e0, e1))
-    torch.cuda.synchronize()
-    t_cold = float(np.median([a.elapsed_time(b) for a, b in cold])) / 1e3
-    del flush, fsum
-    layout = mesh.layout()
-    bytes_layout = 20 * layout["nnz_pad"] + 4 * (layout["nslices"] + 1) + 56 * n
-    # Copy-kernel reference of the same size class (stream bandwidth).
-    cbuf = torch.empty(bytes_canon // 8, dtype=torch.float64, device=dev)
-    cdst = torch.empty_like(cbuf)
-    with torch.cuda.stream(stream):
-        for _ in range(3):
-            cdst.copy_(cbuf)
-        ev0.record(stream)
-        for _ in range(20):
-            cdst.copy_(cbuf)
-        ev1.record(stream)
-    ev1.synchronize()
-    t_copy = ev0.elapsed_time(ev1) / 1e3 / 20
-    copy_gbs = 2 * cbuf.numel() * 8 / t_copy / 1e9
-    del cbuf, cdst
-    # FP32 J apply (inner solver's operator).
-    x32 = x.float()
-    y32 = torch.empty_like(x32)
-    with torch.cuda.stream(stream):
-        J.apply_internal_fp32(x32.data_ptr(), y32.data_ptr(), stream.cuda_stream)
-        ev0.record(stream)
-        for _ in range(args.spmv_reps):
-            J.apply_internal_fp32(x32.data_ptr(), y32.data_ptr(), stream.cuda_stream)
-        ev1.record(stream)
-    ev1.synchronize()
-    t_j32 = ev0.elapsed_time(ev1) / 1e3 / args.spmv_reps  # includes the block conversion
-    traffic, traffic_src = _profiled_traffic(nc)
-    roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": traffic_src,
-                "kernel": "k_block_apply<double> (J.phi FP64)",
-                "us_per_launch": round(t_j * 1e6, 2), "bytes_per_launch": bytes_canon,
-                "cold_us_per_launch": round(t_cold * 1e6, 2),
-                "cold_achieved": round(bytes_canon / t_cold / 1e9, 1),
-                "cold_frac": round(bytes_canon / t_cold / 1e9 / HBM_PEAK_GBS, 4),
-                "copy_kernel_GBs": round(copy_gbs, 1),
-                "layout_bytes_per_launch": bytes_layout,
-                "layout_note": "sliced ELLPACK (64-row slices) of the off-diagonal entries: "
-                               "16 B value + 4 B column per padded slot, 4 B per slice, "
-                               "24 B 2x2 block + 16 B x + 16 B y per vertex",
-                "nnz_pad": layout["nnz_pad"]}
-    log(args, f"J.phi {t_j*1e6:.1f} us -> {ach:.0f} GB/s ({ach/HBM_PEAK_GBS:.1%}); "
-              f"copy {copy_gbs:.0f} GB/s; fp32 J.phi {t_j32*1e6:.1f} us")
-    if args.roofline_only:
-        print(json.dumps({"roofline": roofline, "jphi_fp32_us": t_j32 * 1e6}))
-        return
-
-    # ---- solves.
-    psi_dev0 = torch.from_numpy(psi0).to(dev)
-    psi = torch.empty_like(psi_dev0)
-    results = {}
-    sols = {}
-    for var in [v for v in args.variants.split(",") if v]:
-        kw = dict(tol=1e-10, max_iters=40, minres_rtol=1e-10,
-                  minres_max_iters=args.minres_maxit, cheb_iters=args.cheb,
-                  cheb_ratio=args.ratio)
-        if var == "mixed":
-            kw.update(precision=1, forcing=1, forcing_min=1e-6, forcing_max=0.1)
-        opts = nosh.default_newton_options(**kw)
-        stats = None
-        for w in range(args.warmup):
-            psi.copy_(psi_dev0)
-            torch.cuda.synchronize()
-            stats = nosh.newton_solve_device(model, psi.data_ptr(), params, opts,
-                                             stream.cuda_stream)
-            log(args, var, "warmup", stats["newton_iters"], stats["minres_iters_total"],
-                f"{stats['seconds']:.3f}s")
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        all_stats = []
-        for k in range(args.steps):
-            with torch.cuda.stream(stream):
-                psi.copy_(psi_dev0)
-            stats = nosh.newton_solve_device(model, psi.data_ptr(), params, opts,
-                                             stream.cuda_stream)
-            all_stats.append(stats)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        ok = all(s["converged"] and s["final_norm_f"] <= 1e-10 for s in all_stats)
-        sols[var] = psi.cpu().numpy()
-        results[var] = {"solves_per_s": args.steps / dt, "s_per_solve": dt / args.steps,
-                        "converged": ok, "final_norm_f": all_stats[-1]["final_norm_f"],
-                        "newton_iters": all_stats[-1]["newton_iters"],
-                        "minres_iters": all_stats[-1]["minres_iters_total"],
-                        "minres_history": all_stats[-1]["minres_history"]}
-        log(args, var, results[var])
-    if "plain" in sols:
-        ref = sols["plain"]
-        for var, s in sols.items():
-            d = np.linalg.norm(s - ref)
-            results[var]["abs_diff_vs_fp64"] = float(d)
-            results[var]["rel_diff_vs_fp64"] = float(d / max(np.linalg.norm(ref), 1e-300))
-            cvz = s[0::2] ** 2 + s[1::2] ** 2
-            results[var]["mean_abs_psi2"] = float(cvz.mean())
-
-    best = max((v for v in results if results[v]["converged"]),
-               key=lambda v: results[v]["solves_per_s"], default=None)
-    value = results[best]["solves_per_s"] if best else 0.0
-    cpu = None
-    if not args.no_cpu and "plain" in results:
-        cpu = cpu_baseline(coords, cells, A, mu, psi0, phi,
-                           {"newton_iters": results["plain"]["newton_iters"],
-                            "minres_iters_total": results["plain"]["minres_iters"]},
-                           args.cheb, args.ratio)
-    out = {
-        "metric": "Newton-MINRES solves/sec at ||F||<=1e-10 vs plain FP64 GPU; SpMV HBM GB/s",
-        "value": round(value, 4),
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 / value, 2) if value else None,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if best == "plain" else "f64/f32 (FP32 inner MINRES, FP64 Newton)",
-        "data": "synthetic (seeded SplitMix64 psi_0; structured Kuhn mesh)",
-        "config": {"workload": f"Kuhn-tet cube {nc}^3 Newton-MINRES from seeded random psi",
-                   "n_vertices": n, "n_edges": E, "n_tets": int(cells.shape[0]), "mu": mu,
-                   "g": g, "V": -1.0, "preconditioner": f"Chebyshev-Jacobi({args.cheb}, "
-                   f"ratio {args.ratio})", "best_variant": best},
-        "variants": results,
-        "speedup_vs_plain_fp64": (round(value / results["plain"]["solves_per_s"], 3)
-                                  if "plain" in results and value else None),
-        "roofline": roofline,
-        "jphi_fp32_us": round(t_j32 * 1e6, 2),
-        "mesh_build_s": round(t_mesh, 3),
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out))
+    else:
+        headline(args)
 
 
 if __name__ == "__main__":

@@ -312,6 +312,11 @@ int nosh_mlp_set_params(nosh_mlp_t mlp, const float* params);
 /* X: M x n_in, Y: M x n_out (host). */
 int nosh_mlp_forward(nosh_mlp_t mlp, const float* X, int64_t M, float* Y);
 int nosh_mlp_forward_device(nosh_mlp_t mlp, const float* X, int64_t M, float* Y, void* stream);
+/* Inference only: the same outputs as nosh_mlp_forward from one fused kernel
+ * per 128 rows (activations in LDS, hidden layers on MFMA) when width is 64,
+ * 128 or 256 and n_in <= 32; otherwise the layer-by-layer path. */
+int nosh_mlp_infer(nosh_mlp_t mlp, const float* X, int64_t M, float* Y);
+int nosh_mlp_infer_device(nosh_mlp_t mlp, const float* X, int64_t M, float* Y, void* stream);
 /* Mean-squared-error loss over the M x n_out outputs against T and its
  * gradient (nparams values, parameter layout). grad may be NULL. */
 int nosh_mlp_gradients(nosh_mlp_t mlp, const float* X, const float* T, int64_t M, float* grad,
@@ -388,6 +393,9 @@ typedef struct {
   int newton_budget;    /* 4 */
   double energy_tol;    /* 1e-8 */
   uint64_t seed;
+  int retrain_steps;    /* 0; > 0: after the first training, every accepted
+                           step replaces the oldest of the train_after stored
+                           pairs and runs this many more Adam steps */
 } nosh_surrogate_options;
 
 typedef struct {

@@ -794,6 +794,8 @@ def _load_mlp():
         "nosh_mlp_set_params": (i32, [vp, vp]),
         "nosh_mlp_forward": (i32, [vp, vp, i64, vp]),
         "nosh_mlp_forward_device": (i32, [vp, vp, i64, vp, vp]),
+        "nosh_mlp_infer": (i32, [vp, vp, i64, vp]),
+        "nosh_mlp_infer_device": (i32, [vp, vp, i64, vp, vp]),
         "nosh_mlp_gradients": (i32, [vp, vp, vp, i64, vp, C.POINTER(C.c_double)]),
         "nosh_mlp_train_step_device": (i32, [vp, vp, vp, i64, C.c_float,
                                              C.POINTER(C.c_double), vp]),
@@ -836,6 +838,17 @@ class Mlp(_Handle):
         _check(_lib.nosh_mlp_forward(self._h, _ptr(X), X.shape[0], _ptr(Y)))
         return Y
 
+    def infer(self, X) -> np.ndarray:
+        """Same outputs as forward() from the fused inference kernel."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.empty((X.shape[0], self.n_out), np.float32)
+        _check(_lib.nosh_mlp_infer(self._h, _ptr(X), X.shape[0], _ptr(Y)))
+        return Y
+
+    def infer_device(self, x_ptr: int, M: int, y_ptr: int, stream: int = 0):
+        _check(_lib.nosh_mlp_infer_device(self._h, C.c_void_p(x_ptr), M, C.c_void_p(y_ptr),
+                                          C.c_void_p(stream)))
+
     def gradients(self, X, T):
         X = np.ascontiguousarray(X, dtype=np.float32)
         T = np.ascontiguousarray(T, dtype=np.float32)
@@ -855,7 +868,8 @@ class Mlp(_Handle):
 class SurrogateOptions(C.Structure):
     _fields_ = [("width", C.c_int), ("depth", C.c_int), ("train_after", C.c_int),
                 ("train_steps", C.c_int), ("batch", C.c_int), ("lr", C.c_float),
-                ("newton_budget", C.c_int), ("energy_tol", C.c_double), ("seed", C.c_uint64)]
+                ("newton_budget", C.c_int), ("energy_tol", C.c_double), ("seed", C.c_uint64),
+                ("retrain_steps", C.c_int)]
 
 
 class SurrogateStats(C.Structure):

@@ -400,7 +400,7 @@ int nosh_batch_autotune(nosh_batch_t batch, const double* psi, const nosh_newton
       double best = 1e30;
       for (int threads : {128, 256, 512, 1024}) {
         if (threads % maxbp || threads < maxbp) continue;
-        for (int fold : {1, 4, 16}) {
+        for (int fold : {1, 2, 4, 8}) {
           for (int mode : {0, 1}) {
             if (nosh_batch_set_tuning(batch, threads, fold, mode, batch->b.graph_iters,
                                       batch->b.poll) != NOSH_OK)

@@ -9,10 +9,11 @@
 // With a surrogate, after `train_after` steps a per-vertex MLP (BF16 MFMA,
 // trained on the run's own solutions) seeds Newton with
 //   psi_guess = psi_sec + h^2 sigma Y(x, psi_k, h s, mu, g),   h s = psi_sec - psi_k,
-// sigma the RMS of the training targets (psi_{k+1} - psi_sec) / h^2. A branch
-// whose Newton from the guess does not converge within the budget, or whose
-// Gibbs energy leaves its linear trend, is solved again from psi_sec, then,
-// if that fails too, from psi_k with the full iteration limit.
+// sigma the RMS of the training targets (psi_{k+1} - psi_sec) / h^2; each
+// branch starts from whichever of the two guesses has the smaller ||F||. A
+// branch whose Newton from the surrogate guess does not converge within the
+// budget, or whose Gibbs energy leaves its linear trend, is solved again
+// from psi_sec, then, if that fails too (a branch transition), from psi_k.
 //
 // This is the caller the reference delegates to LOCA (natural stepping,
 // executables/nosh-cont/nosh-cont.cpp:226-290,344 with examples/conf.xml:35-190);
@@ -188,7 +189,8 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
   nosh_surrogate_stats sst{};
   int rc = bcguard([&] {
     NOSH_CHECK(bt && psi && record && steps >= 1, kInvalid, "invalid batch continuation call");
-    NOSH_CHECK(!so || (so->train_after >= 1 && so->batch >= 64 && so->newton_budget >= 1),
+    NOSH_CHECK(!so || (so->train_after >= 1 && so->batch >= 64 && so->newton_budget >= 1 &&
+                       so->retrain_steps >= 0),
                kInvalid, "bad surrogate options");
     nosh_newton_options o;
     if (opts)
@@ -298,7 +300,6 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
       }
       cs = 1.0 / cmax;
     }
-    constexpr int64_t kChunk = 1 << 19;  // MLP rows per inference call
     auto predict = [&](double scale) {
       for (size_t k = 0; k < ng; ++k) {
         GroupHost& G = B.ghost[k];
@@ -306,25 +307,28 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
             G.mesh->n, G.dev.bp, G.B, dim, G.mesh->perm.p, G.mesh->coords.p, cs, gc[k].cur.p,
             gc[k].sec.p, B.mu.p, B.g.p, G.dev.inst0, X.p + gc[k].xoff * nin);
       }
-      for (int64_t r0 = 0; r0 < xrows; r0 += kChunk)
-        net->forward(X.p + r0 * nin, std::min(kChunk, xrows - r0), Y.p + r0 * 2, s);
+      net->infer(X.p, xrows, Y.p, s);
       for (size_t k = 0; k < ng; ++k) {
         GroupHost& G = B.ghost[k];
         k_bc_apply<<<grid_for(G.mesh->n * G.dev.bp, 256), 256, 0, s>>>(
             G.mesh->n, G.dev.bp, G.B, gc[k].sec.p, Y.p + gc[k].xoff * 2, scale, gc[k].guess.p);
       }
     };
-    auto train = [&]() {
-      std::vector<float> th(size_t(P) * xrows * 2);
-      Ts.download(th.data(), th.size(), s);
-      NOSH_HIP(hipStreamSynchronize(s));
-      double ss = 0.0;
-      for (float v : th) ss += double(v) * v;
-      sigma = std::sqrt(ss / double(th.size()));
-      if (!(sigma > 0.0)) sigma = 1.0;
+    // Adam steps on random vertex minibatches of the stored pairs; the first
+    // call also fixes the output scale sigma (RMS of the targets).
+    auto train = [&](int nsteps) {
+      if (!trained) {
+        std::vector<float> th(size_t(P) * xrows * 2);
+        Ts.download(th.data(), th.size(), s);
+        NOSH_HIP(hipStreamSynchronize(s));
+        double ss = 0.0;
+        for (float v : th) ss += double(v) * v;
+        sigma = std::sqrt(ss / double(th.size()));
+        if (!(sigma > 0.0)) sigma = 1.0;
+      }
       std::uniform_int_distribution<int64_t> pick(0, int64_t(P) * xrows - 1);
       double lval = 0.0;
-      for (int it = 0; it < so->train_steps; ++it) {
+      for (int it = 0; it < nsteps; ++it) {
         for (auto& v : idx_h) v = pick(rng);
         idx.upload(idx_h.data(), idx_h.size(), s);
         launch_gather_rows(Xs.p, idx.p, so->batch, nin, 1.0f, xb.p, s);
@@ -362,7 +366,8 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
     const double h = span / steps;
     double hprev = 0.0;
     const bool progress = std::getenv("NOSH_PROGRESS") != nullptr;
-    std::vector<int> ok(ni), skip(ni), use_sur(ni), maxit(ni);
+    std::vector<int> ok(ni), skip(ni), use_sur(ni), maxit(ni), tried_sec(ni);
+    int skip_until = 0, losses = 0;
     std::vector<double> nf_sur, nf_sec;
     for (int k = 1; k <= steps; ++k) {
       const auto t0 = std::chrono::steady_clock::now();
@@ -379,16 +384,13 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
         k_bc_secant<<<grid_for(m, 256), 256, 0, s>>>(m, ratio, gc[q].cur.p, gc[q].prev.p,
                                                      gc[q].sec.p);
       }
-      const bool seeded = so && trained && hprev > 0.0;
-      auto run = [&](const nosh_newton_options& oo, const int* sk) {
-        const int it = newton_batch(bt, oo, st.data(), s, sk);
-        int mx = 0;
-        for (int i = 0; i < ni; ++i)
-          if (!sk || !sk[i]) mx = std::max(mx, st[i].newton_iters);
-        r.newton_iters += mx;
-        r.minres_iters += it;
-      };
+      // The surrogate is evaluated while it earns its cost: when it gives the
+      // smaller residual for fewer than a quarter of the branches, its next
+      // evaluations are skipped for 2, 4, ... (at most 16) steps; a win
+      // resets the back-off.
+      const bool seeded = so && trained && hprev > 0.0 && k >= skip_until;
       std::fill(ok.begin(), ok.end(), 0);
+      std::fill(tried_sec.begin(), tried_sec.end(), 0);
       if (seeded) {
         // Each branch starts from the guess with the smaller ||F||: the
         // surrogate's (Newton budget so->newton_budget) or the secant's (the
@@ -402,7 +404,14 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
         for (int i = 0; i < ni; ++i) {
           use_sur[i] = nf_sur[i] < nf_sec[i];
           maxit[i] = use_sur[i] ? so->newton_budget : o.max_iters;
+          tried_sec[i] = !use_sur[i];
           nsur += use_sur[i];
+        }
+        if (4 * nsur < ni) {
+          losses = std::min(losses + 1, 4);
+          skip_until = k + 1 + (1 << losses);
+        } else {
+          losses = 0;
         }
         if (nsur > 0) {
           take_into_psi(&GroupCont::guess, use_sur);
@@ -429,19 +438,27 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
         }
       }
       // Secant guess for every branch not yet accepted, then psi_k.
+      // Branches not yet accepted: Newton from the secant guess (unless a
+      // branch already started from it), then from psi_k.
       for (int attempt = 0; attempt < 2; ++attempt) {
         int todo = 0;
         for (int i = 0; i < ni; ++i) {
-          skip[i] = ok[i];
-          todo += !ok[i];
+          skip[i] = ok[i] || (attempt == 0 && tried_sec[i]);
+          todo += !skip[i];
+          maxit[i] = o.max_iters;
         }
-        if (todo == 0) break;
+        if (todo == 0) continue;
         std::vector<int> take(ni);
-        for (int i = 0; i < ni; ++i) take[i] = !ok[i];
+        for (int i = 0; i < ni; ++i) take[i] = !skip[i];
         take_into_psi(attempt == 0 ? &GroupCont::sec : &GroupCont::cur, take);
-        run(o, skip.data());
+        const int it = newton_batch(bt, o, st.data(), s, skip.data(), maxit.data());
+        int mx = 0;
         for (int i = 0; i < ni; ++i)
-          if (!ok[i]) ok[i] = st[i].converged;
+          if (!skip[i]) mx = std::max(mx, st[i].newton_iters);
+        r.newton_iters += mx;
+        r.minres_iters += it;
+        for (int i = 0; i < ni; ++i)
+          if (!skip[i]) ok[i] = st[i].converged;
       }
       for (int i = 0; i < ni; ++i) {
         r.converged += ok[i];
@@ -465,27 +482,36 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
       }
       r.sec_error = se / ni;
       r.pred_error = seeded ? pe / ni : r.sec_error;
-      // Training pairs from whole secant steps.
-      if (so && !trained && hprev > 0.0 && pairs < P) {
+      // Training pairs from whole secant steps, kept in a ring of P steps.
+      // Steps whose Newton needed more than the budget (branch transitions)
+      // are not stored: their targets dwarf the smooth ones.
+      if (so && hprev > 0.0 && (!trained || so->retrain_steps > 0) &&
+          r.newton_iters <= so->newton_budget) {
+        const int slot = pairs % P;
         for (size_t q = 0; q < ng; ++q) {
           GroupHost& G = B.ghost[q];
           const int64_t rows = G.mesh->n * G.B;
           k_bc_features<<<grid_for(rows, 256), 256, 0, s>>>(
               G.mesh->n, G.dev.bp, G.B, dim, G.mesh->perm.p, G.mesh->coords.p, cs, gc[q].cur.p,
               gc[q].sec.p, B.mu.p, B.g.p, G.dev.inst0,
-              Xs.p + (int64_t(pairs) * xrows + gc[q].xoff) * nin);
+              Xs.p + (int64_t(slot) * xrows + gc[q].xoff) * nin);
           k_bc_target<<<grid_for(rows, 256), 256, 0, s>>>(
               G.mesh->n, G.dev.bp, G.B, G.psi.p, gc[q].sec.p, 1.0 / (h * h),
-              Ts.p + (int64_t(pairs) * xrows + gc[q].xoff) * 2);
+              Ts.p + (int64_t(slot) * xrows + gc[q].xoff) * 2);
         }
         ++pairs;
-        if (pairs == P) {
+        if (!trained && pairs == P) {
           const auto tt = std::chrono::steady_clock::now();
-          sst.final_loss = train();
+          sst.final_loss = train(so->train_steps);
           sst.train_seconds =
               std::chrono::duration<double>(std::chrono::steady_clock::now() - tt).count();
           sst.trained = 1;
           trained = true;
+        } else if (trained) {
+          const auto tt = std::chrono::steady_clock::now();
+          sst.final_loss = train(so->retrain_steps);
+          sst.train_seconds +=
+              std::chrono::duration<double>(std::chrono::steady_clock::now() - tt).count();
         }
       }
       energies(e_now);

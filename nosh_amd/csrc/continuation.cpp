@@ -386,6 +386,7 @@ int nosh_surrogate_default_options(nosh_surrogate_options* o) {
     o->newton_budget = 4;
     o->energy_tol = 1e-8;
     o->seed = 3;
+    o->retrain_steps = 0;
   });
 }
 
@@ -398,8 +399,9 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int np, const c
   nosh_surrogate_stats sst{};
   int rc = cguard([&] {
     NOSH_CHECK(nls && psi && opts && so && record, kInvalid, "null argument");
-    NOSH_CHECK(opts->steps >= 1 && so->train_after >= 1 && so->batch >= 64, kInvalid,
-               "bad continuation / surrogate options");
+    NOSH_CHECK(opts->steps >= 1 && so->train_after >= 1 && so->batch >= 64 &&
+                   so->retrain_steps >= 0,
+               kInvalid, "bad continuation / surrogate options");
     const std::string param = opts->parameter ? opts->parameter : "mu";
     NOSH_CHECK(param == "mu" || param == "g", kInvalid, "continuation parameter must be mu or g");
     std::map<std::string, double> p;
@@ -473,6 +475,21 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int np, const c
     DBuf<int64_t> idx(so->batch);
     DBuf<float> xb(int64_t(so->batch) * nin), tb(int64_t(so->batch) * 2);
     std::vector<int64_t> idx_h(so->batch);
+    // Adam steps on random vertex minibatches of the stored pairs.
+    auto train_steps = [&](int nsteps) {
+      std::uniform_int_distribution<int64_t> pick(0, int64_t(P) * n - 1);
+      double lval = 0.0;
+      for (int it = 0; it < nsteps; ++it) {
+        for (auto& v : idx_h) v = pick(rng);
+        idx.upload(idx_h.data(), idx_h.size(), s);
+        launch_gather_rows(Xs.p, idx.p, so->batch, nin, 1.0f, xb.p, s);
+        launch_gather_rows(Ts.p, idx.p, so->batch, 2, float(1.0 / sigma), tb.p, s);
+        lval = net.gradients(xb.p, tb.p, so->batch, s);
+        net.adam(so->lr, s);
+      }
+      NOSH_HIP(hipStreamSynchronize(s));
+      return lval;
+    };
     for (int k = 1; k <= opts->steps; ++k) {
       const auto t0 = std::chrono::steady_clock::now();
       const double target = opts->start + k * h;
@@ -493,7 +510,7 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int np, const c
       bool ok = false;
       nosh_newton_stats st{};
       if (trained) {
-        net.forward(X.p, n, Y.p, s);
+        net.infer(X.p, n, Y.p, s);
         k_apply_guess<<<gn, 256, 0, s>>>(n, reinterpret_cast<const double2*>(w.psi_prev.p),
                                          reinterpret_cast<const double2*>(w.t.p), hs, Y.p,
                                          sigma, reinterpret_cast<double2*>(guess.p));
@@ -565,16 +582,18 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int np, const c
       r.pred_error = r.predictor_used == 3 ? diff_norm(2 * n, guess.p, x, dpart, dscal, s)
                                            : r.tan_error;
       // Training pairs from the run's own solutions (whole tangent steps
-      // only: sub-steps overwrite the tangent the features were built from).
-      if (!trained && pairs < P && halvings == 0) {
-        NOSH_HIP(hipMemcpyAsync(Xs.p + int64_t(pairs) * n * nin, X.p, sizeof(float) * n * nin,
+      // only: sub-steps overwrite the tangent the features were built from),
+      // kept in a ring of P steps.
+      if ((!trained || so->retrain_steps > 0) && halvings == 0) {
+        const int slot = pairs % P;
+        NOSH_HIP(hipMemcpyAsync(Xs.p + int64_t(slot) * n * nin, X.p, sizeof(float) * n * nin,
                                 hipMemcpyDeviceToDevice, s));
         k_target<<<gn, 256, 0, s>>>(n, reinterpret_cast<const double2*>(x),
                                     reinterpret_cast<const double2*>(w.psi_prev.p),
                                     reinterpret_cast<const double2*>(w.t.p), hs,
-                                    Ts.p + int64_t(pairs) * n * 2);
+                                    Ts.p + int64_t(slot) * n * 2);
         ++pairs;
-        if (pairs == P) {
+        if (!trained && pairs == P) {
           const auto tt = std::chrono::steady_clock::now();
           std::vector<float> th(int64_t(P) * n * 2);
           Ts.download(th.data(), th.size(), s);
@@ -583,21 +602,15 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int np, const c
           for (float v : th) ss += double(v) * v;
           sigma = std::sqrt(ss / double(th.size()));
           if (!(sigma > 0.0)) sigma = 1.0;
-          std::uniform_int_distribution<int64_t> pick(0, int64_t(P) * n - 1);
-          double lval = 0.0;
-          for (int it = 0; it < so->train_steps; ++it) {
-            for (auto& v : idx_h) v = pick(rng);
-            idx.upload(idx_h.data(), idx_h.size(), s);
-            launch_gather_rows(Xs.p, idx.p, so->batch, nin, 1.0f, xb.p, s);
-            launch_gather_rows(Ts.p, idx.p, so->batch, 2, float(1.0 / sigma), tb.p, s);
-            lval = net.gradients(xb.p, tb.p, so->batch, s);
-            net.adam(so->lr, s);
-          }
-          NOSH_HIP(hipStreamSynchronize(s));
+          sst.final_loss = train_steps(so->train_steps);
           trained = true;
           sst.trained = 1;
-          sst.final_loss = lval;
           sst.train_seconds =
+              std::chrono::duration<double>(std::chrono::steady_clock::now() - tt).count();
+        } else if (trained) {
+          const auto tt = std::chrono::steady_clock::now();
+          sst.final_loss = train_steps(so->retrain_steps);
+          sst.train_seconds +=
               std::chrono::duration<double>(std::chrono::steady_clock::now() - tt).count();
         }
       }

@@ -189,7 +189,7 @@ __global__ void k_layer_in(const float* __restrict__ X, int64_t M, int nin,
   const int n = int(i - m * N);
   float s = b[n];
   for (int c = 0; c < nin; ++c) s += X[m * nin + c] * float(W[n * nin + c]);
-  z[i] = s;
+  if (z) z[i] = s;
   h[i] = __bf16(gelu(s));
 }
 
@@ -288,6 +288,99 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ gr, floa
 __global__ void k_f32_to_bf16(const float* __restrict__ a, int64_t n, __bf16* __restrict__ b) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) b[i] = __bf16(a[i]);
+}
+
+// Fused inference: the whole network for 128 rows per block (4 waves x 32
+// rows), activations kept in LDS between layers, hidden width x width layers
+// on MFMA with the weights read from L2. Same arithmetic and summation order
+// as forward(): first layer FP32 over the inputs, hidden layers in 16-wide k
+// chunks in increasing order, last layer FP32 over k in order; BF16
+// activations and weights, FP32 accumulation.
+constexpr int kInferRows = 128;
+constexpr int kInferMaxIn = 32;
+constexpr int kInferMaxLayers = 18;
+
+struct InferArgs {
+  const float* X;
+  int64_t M;
+  int nin, width, depth, nout;
+  const __bf16* w;     // BF16 weights, layer l at woff[l], [out][in]
+  const float* bias;   // FP32 biases, layer l at boff[l]
+  int64_t woff[kInferMaxLayers];
+  int64_t boff[kInferMaxLayers];
+  float* Y;
+};
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
+  constexpr int W = NT * 32;
+  constexpr int LD = W + 8;  // padded row (bf16) against LDS bank conflicts
+  __shared__ __bf16 act[kInferRows][LD];
+  __shared__ float xs[kInferRows][kInferMaxIn];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t r0 = int64_t(blockIdx.x) * kInferRows;
+  const int rows = int(min<int64_t>(kInferRows, g.M - r0));
+  for (int i = tid; i < kInferRows * g.nin; i += 256) {
+    const int r = i / g.nin, c = i - r * g.nin;
+    xs[r][c] = r < rows ? g.X[(r0 + r) * g.nin + c] : 0.0f;
+  }
+  __syncthreads();
+  // Layer 0: n_in -> W on the vector units, one output column per thread.
+  for (int n = tid; n < W; n += 256) {
+    float wr[kInferMaxIn];
+    for (int c = 0; c < g.nin; ++c) wr[c] = float(g.w[g.woff[0] + int64_t(n) * g.nin + c]);
+    const float bn = g.bias[g.boff[0] + n];
+    for (int r = 0; r < kInferRows; ++r) {
+      float z = bn;
+      for (int c = 0; c < g.nin; ++c) z += xs[r][c] * wr[c];
+      act[r][n] = __bf16(gelu(z));
+    }
+  }
+  __syncthreads();
+  // Hidden layers: each wave owns rows 32 wave .. 32 wave + 31.
+  const int row = wave * 32 + (lane & 31);
+  const int kh = 8 * (lane >> 5);
+  for (int l = 1; l < g.depth; ++l) {
+    const __bf16* Wl = g.w + g.woff[l];
+    f32x16 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+#pragma unroll 2
+    for (int kc = 0; kc < W / 16; ++kc) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&act[row][kc * 16 + kh]);
+      bf16x8 b[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        b[t] = *reinterpret_cast<const bf16x8*>(Wl + int64_t(t * 32 + (lane & 31)) * W +
+                                                  kc * 16 + kh);
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[t], acc[t], 0, 0, 0);
+    }
+    const float* bl = g.bias + g.boff[l];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int n = t * 32 + (lane & 31);
+      const float bn = bl[n];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        act[wave * 32 + i][n] = __bf16(gelu(acc[t][r] + bn));
+      }
+    }
+  }
+  __syncthreads();
+  // Last layer: W -> n_out, FP32 dot products over k in order.
+  const int L = g.depth;
+  for (int i = tid; i < kInferRows * g.nout; i += 256) {
+    const int r = i / g.nout, o = i - r * g.nout;
+    if (r >= rows) continue;
+    const __bf16* wo = g.w + g.woff[L] + int64_t(o) * W;
+    float s = g.bias[g.boff[L] + o];
+    for (int k = 0; k < W; ++k) s += float(act[r][k]) * float(wo[k]);
+    g.Y[(r0 + r) * g.nout + o] = s;
+  }
 }
 
 unsigned g1(int64_t n, int b = 256) { return grid_for(n, b); }
@@ -395,18 +488,50 @@ void Mlp::gemm(const __bf16* A, int64_t sam, int64_t sak, const __bf16* B, int64
   k_gemm_bf16<<<grid, 256, 0, s>>>(g);
 }
 
-void Mlp::forward(const float* X, int64_t M, float* Y, hipStream_t s) {
+void Mlp::forward(const float* X, int64_t M, float* Y, hipStream_t s, bool keep_pre) {
   ensure(M);
   const int L = int(dims.size()) - 1;
   k_layer_in<<<g1(M * dims[1]), 256, 0, s>>>(X, M, nin, wbf.p + woff[0], bias(0), dims[1],
-                                             zbuf[0].p, hbuf[0].p);
+                                             keep_pre ? zbuf[0].p : nullptr, hbuf[0].p);
   for (int l = 1; l < L - 1; ++l) {
     // z = h W^T: A = h [M][K], B(k,n) = W[n][k].
     gemm(hbuf[l - 1].p, dims[l], 1, wbf.p + woff[l], 1, dims[l], M, dims[l + 1], dims[l], 0,
-         bias(l), nullptr, zbuf[l].p, hbuf[l].p, 1, s);
+         bias(l), nullptr, keep_pre ? zbuf[l].p : nullptr, hbuf[l].p, 1, s);
   }
   k_layer_out<<<g1(M), 256, 0, s>>>(hbuf[L - 2].p, M, dims[L - 1], wbf.p + woff[L - 1],
                                     bias(L - 1), nout, Y ? Y : y.p);
+}
+
+void Mlp::infer(const float* X, int64_t M, float* Y, hipStream_t s) {
+  const int nt = width / 32;
+  if (width % 32 != 0 || (nt != 2 && nt != 4 && nt != 8) || nin > kInferMaxIn ||
+      depth + 1 > kInferMaxLayers) {
+    constexpr int64_t kChunk = 1 << 19;
+    for (int64_t r0 = 0; r0 < M; r0 += kChunk)
+      forward(X + r0 * nin, std::min(kChunk, M - r0), Y + r0 * nout, s, false);
+    return;
+  }
+  InferArgs g{};
+  g.X = X;
+  g.M = M;
+  g.nin = nin;
+  g.width = width;
+  g.depth = depth;
+  g.nout = nout;
+  g.w = wbf.p;
+  g.bias = master.p + nweights;
+  for (int l = 0; l <= depth; ++l) {
+    g.woff[l] = woff[l];
+    g.boff[l] = boff[l];
+  }
+  g.Y = Y;
+  const unsigned nb = unsigned((M + kInferRows - 1) / kInferRows);
+  if (nt == 8)
+    k_mlp_infer<8><<<nb, 256, 0, s>>>(g);
+  else if (nt == 4)
+    k_mlp_infer<4><<<nb, 256, 0, s>>>(g);
+  else
+    k_mlp_infer<2><<<nb, 256, 0, s>>>(g);
 }
 
 double Mlp::gradients(const float* X, const float* T, int64_t M, hipStream_t s) {

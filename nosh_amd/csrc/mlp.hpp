@@ -31,7 +31,11 @@ struct Mlp {
   void gemm(const __bf16* A, int64_t sam, int64_t sak, const __bf16* B, int64_t sbk, int64_t sbn,
             int64_t M, int64_t N, int64_t K, int epi, const float* bias, const float* zin,
             float* outf, __bf16* outb, int splits, hipStream_t s);
-  void forward(const float* X, int64_t M, float* Y, hipStream_t s);
+  // keep_pre: store the hidden pre-activations (needed by gradients()).
+  void forward(const float* X, int64_t M, float* Y, hipStream_t s, bool keep_pre = true);
+  // Inference only (no stored activations): the fused one-kernel network for
+  // width 64, 128 or 256 and n_in <= 32, else forward() in row chunks.
+  void infer(const float* X, int64_t M, float* Y, hipStream_t s);
   // Fills grad (same layout as the parameters) for the mean squared error
   // of the outputs against T; returns the loss.
   double gradients(const float* X, const float* T, int64_t M, hipStream_t s);

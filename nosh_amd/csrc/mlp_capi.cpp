@@ -88,6 +88,27 @@ int nosh_mlp_forward(nosh_mlp_t mlp, const float* X, int64_t M, float* Y) {
   });
 }
 
+int nosh_mlp_infer(nosh_mlp_t mlp, const float* X, int64_t M, float* Y) {
+  return mguard([&] {
+    NOSH_CHECK(mlp && X && Y && M >= 1, kInvalid, "invalid argument");
+    Mlp& n = mlp->net;
+    hipStream_t s = default_stream();
+    if (mlp->x.n < size_t(M * n.nin)) mlp->x.alloc(M * n.nin);
+    if (mlp->y.n < size_t(M * n.nout)) mlp->y.alloc(M * n.nout);
+    mlp->x.upload(X, M * n.nin, s);
+    n.infer(mlp->x.p, M, mlp->y.p, s);
+    mlp->y.download(Y, M * n.nout, s);
+    NOSH_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int nosh_mlp_infer_device(nosh_mlp_t mlp, const float* X, int64_t M, float* Y, void* stream) {
+  return mguard([&] {
+    NOSH_CHECK(mlp && X && Y && M >= 1, kInvalid, "invalid argument");
+    mlp->net.infer(X, M, Y, stream ? static_cast<hipStream_t>(stream) : default_stream());
+  });
+}
+
 int nosh_mlp_gradients(nosh_mlp_t mlp, const float* X, const float* T, int64_t M, float* grad,
                        double* loss) {
   return mguard([&] {

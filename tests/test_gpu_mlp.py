@@ -70,3 +70,22 @@ def test_training_reduces_loss():
     losses = [net.train_step_device(Xd.data_ptr(), Td.data_ptr(), M, 1e-3) for _ in range(300)]
     print("loss", losses[0], "->", losses[-1])
     assert losses[-1] < 0.1 * losses[0]
+
+
+@pytest.mark.parametrize("width,M", [(256, 1000), (256, 70000), (128, 300), (64, 129)])
+def test_fused_inference_matches_forward(width, M):
+    """The one-kernel inference (activations in LDS, MFMA hidden layers) gives
+    forward()'s outputs: same roundings and summation order, so equal up to
+    FMA contraction differences; and it agrees with the bf16-emulating
+    oracle."""
+    net = _net(nin=9, width=width, seed=5)
+    rng = np.random.default_rng(7)
+    X = rng.normal(0, 1, (M, 9)).astype(np.float32)
+    y_f = net.forward(X)
+    y_i = net.infer(X)
+    d = np.abs(y_i - y_f).max() / max(np.abs(y_f).max(), 1e-30)
+    print("fused vs layered max rel diff", d)
+    assert d < 1e-5
+    if M <= 1000:
+        y_bf, _, _ = MO.forward(net.get_params(), X, net.dims, bf=True)
+        assert np.linalg.norm(y_i - y_bf) / np.linalg.norm(y_bf) < 5e-3
