@@ -68,6 +68,9 @@ def parse():
     p.add_argument("--cheb", type=int, default=6, help="Chebyshev degree, plain FP64 solver")
     p.add_argument("--cont-cheb", type=int, default=4, help="Chebyshev degree, batched solver")
     p.add_argument("--ratio", type=float, default=300.0)
+    p.add_argument("--cont-ratio", type=float, default=300.0,
+                   help="Chebyshev interval ratio, batched solver")
+    p.add_argument("--cont-forcing-max", type=float, default=0.1)
     p.add_argument("--minres-maxit", type=int, default=2000)
     p.add_argument("--spmv-reps", type=int, default=50)
     p.add_argument("--variants", default="plain,mixed")
@@ -197,8 +200,8 @@ def continuation_phase(args, model, n):
     for b in range(nb):
         psi0[2 * n * b:2 * n * (b + 1):2] = 1.0 / np.sqrt(g[b])
     kw = dict(tol=1e-10, max_iters=40, minres_rtol=1e-10, minres_max_iters=4000,
-              cheb_iters=args.cont_cheb, cheb_ratio=args.ratio, forcing=1, forcing_min=1e-6,
-              forcing_max=0.1)
+              cheb_iters=args.cont_cheb, cheb_ratio=args.cont_ratio, forcing=1,
+              forcing_min=1e-6, forcing_max=args.cont_forcing_max)
     batch = nosh.Batch([(model, nb)], mu0, g)
     t0 = time.perf_counter()
     if args.cont_tuning:
@@ -483,7 +486,8 @@ def headline(args):
                    "n_vertices": n, "n_edges": E, "n_tets": int(cells.shape[0]),
                    "V": -1.0, "branches_g": cont["g"],
                    "combined": f"batched FP32-inner/FP64 Newton, secant + surrogate predictor, "
-                               f"Chebyshev-Jacobi({args.cont_cheb}, {args.ratio})",
+                               f"Chebyshev-Jacobi({args.cont_cheb}, {args.cont_ratio}), "
+                               f"Eisenstat-Walker forcing <= {args.cont_forcing_max}",
                    "plain": f"FP64 Newton-MINRES rtol 1e-10, tangent predictor, "
                             f"Chebyshev-Jacobi({args.cheb}, {args.ratio})"},
         "speedup_vs_plain_fp64": (round(cont["speedup_vs_plain_fp64"], 3)

@@ -254,13 +254,12 @@ int nosh_batch_create(int ngroups, const nosh_nls_t* models, const int* counts, 
       // alpha_e with the model's thickness (keo::build_alpha_cache_).
       std::map<std::string, double> p0{{"mu", mu[inst]}, {"g", g[inst]}};
       md->keo->set_parameters(p0, s);
-      G.a32.alloc(m.nnz_pad);
-      G.t32.alloc(m.nnz_pad);
+      G.s32.alloc(m.nnz_pad);
       G.a64.alloc(m.nnz_pad);
       G.t64.alloc(m.nnz_pad);
       G.diag.alloc(m.n);
       G.rowabs.alloc(m.n);
-      launch_slot_fields(m, md->keo->alpha.p, md->mvp->vf.atilde.p, G.a32.p, G.t32.p, G.a64.p,
+      launch_slot_fields(m, md->keo->alpha.p, md->mvp->vf.atilde.p, G.s32.p, G.a64.p,
                          G.t64.p, G.diag.p, G.rowabs.p, s);
       G.ct.alloc(m.n);
       compute_ct(m, md->thickness, G.ct, p0, s);
@@ -274,8 +273,7 @@ int nosh_batch_create(int ngroups, const nosh_nls_t* models, const int* counts, 
       d.inst0 = inst;
       d.slice_ptr = m.slice_ptr.p;
       d.scol = m.scol.p;
-      d.a32 = G.a32.p;
-      d.t32 = G.t32.p;
+      d.s32 = G.s32.p;
       d.a64 = G.a64.p;
       d.t64 = G.t64.p;
       d.diag = G.diag.p;

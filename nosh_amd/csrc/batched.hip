@@ -96,27 +96,91 @@ struct Vt<double> {
   using v2 = double2;
 };
 
+// Streaming accesses of a row's own entries (blocks, residuals, Chebyshev
+// and MINRES work vectors) are non-temporal, so that they do not evict from
+// L2 the x entries the neighbouring rows gather: with 8 lanes per row the
+// streamed bytes per row are 8x those of a single instance, enough to push
+// the gather window out of a 4 MB L2.
+#ifndef NOSH_BATCH_NT
+#define NOSH_BATCH_NT 1
+#endif
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ldn(const float2* p) {
+#if NOSH_BATCH_NT
+  const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+  return make_float2(v.x, v.y);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ double2 ldn(const double2* p) {
+#if NOSH_BATCH_NT
+  const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+  return make_double2(v.x, v.y);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ float ldn(const float* p) {
+#if NOSH_BATCH_NT
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ void stn(float2* p, float2 v) {
+#if NOSH_BATCH_NT
+  f2v w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
+#else
+  *p = v;
+#endif
+}
+__device__ __forceinline__ void stn(double2* p, double2 v) {
+#if NOSH_BATCH_NT
+  d2v w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<d2v*>(p));
+#else
+  *p = v;
+#endif
+}
+
 __device__ __forceinline__ void sc(float x, float* s, float* c) { __sincosf(x, s, c); }
 __device__ __forceinline__ void sc(double x, double* s, double* c) { sincos(x, s, c); }
 
 // Off-diagonal product of one (row, instance) with values rebuilt in registers.
 // The slot count of a row is its slice's width, the same for the whole
-// wave; slots are taken 4 at a time so that their column, alpha and a~
-// loads, and then the x gathers, are all in flight together (4 measured
-// faster than 1 or 8 at 96^3 x 8 lanes).
+// wave; slots are taken 4 at a time so that their metadata loads, and then
+// the x gathers, are all in flight together (4 measured faster than 1 or 8
+// at 96^3 x 8 lanes). FP32: one 16-byte {column, alpha, a~} load per slot;
+// FP64: column, alpha and a~ from separate arrays.
+template <class T>
+__device__ __forceinline__ void slot_meta(const BGroup& G, int32_t j, int32_t& c, T& al, T& at) {
+  if constexpr (std::is_same<T, float>::value) {
+    const int4 m = G.s32[j];
+    c = m.x;
+    al = __int_as_float(m.y);
+    at = __int_as_float(m.z);
+  } else {
+    c = G.scol[j];
+    al = G.a64[j];
+    at = G.t64[j];
+  }
+}
+
 template <class T, int K>
 __device__ __forceinline__ void brow_slots(const BGroup& G, int32_t j, int b, T mu,
-                                           const T* __restrict__ al, const T* __restrict__ at,
                                            const typename Vt<T>::v2* __restrict__ x, T& ar,
                                            T& ai) {
   int32_t c[K];
   T av[K], tv[K];
 #pragma unroll
-  for (int q = 0; q < K; ++q) {
-    c[q] = G.scol[j + q * kSliceH];
-    av[q] = al[j + q * kSliceH];
-    tv[q] = at[j + q * kSliceH];
-  }
+  for (int q = 0; q < K; ++q) slot_meta<T>(G, j + q * kSliceH, c[q], av[q], tv[q]);
   typename Vt<T>::v2 xv[K];
   const int bp = G.bp;
 #pragma unroll
@@ -139,21 +203,12 @@ __device__ __forceinline__ typename Vt<T>::v2 brow(const BGroup& G, int64_t row,
   const int32_t s = int32_t(row >> 6);
   const int32_t beg = G.slice_ptr[s], end = G.slice_ptr[s + 1];
   const int32_t w = (end - beg) / kSliceH;
-  const T* al;
-  const T* at;
-  if constexpr (std::is_same<T, float>::value) {
-    al = G.a32;
-    at = G.t32;
-  } else {
-    al = G.a64;
-    at = G.t64;
-  }
   T ar = T(0), ai = T(0);
   int32_t j = beg + int32_t(row & 63);
   int32_t k = 0;
   for (; k + kSlotBatch <= w; k += kSlotBatch, j += kSlotBatch * kSliceH)
-    brow_slots<T, kSlotBatch>(G, j, b, mu, al, at, x, ar, ai);
-  for (; k < w; ++k, j += kSliceH) brow_slots<T, 1>(G, j, b, mu, al, at, x, ar, ai);
+    brow_slots<T, kSlotBatch>(G, j, b, mu, x, ar, ai);
+  for (; k < w; ++k, j += kSliceH) brow_slots<T, 1>(G, j, b, mu, x, ar, ai);
   typename Vt<T>::v2 r;
   r.x = ar;
   r.y = ai;
@@ -248,119 +303,180 @@ __device__ __forceinline__ float2* wbuf(const BGroup& G, int k) {
 }
 __device__ __forceinline__ float2* zbuf(const BGroup& G, int k) { return (k & 1) ? G.Z2 : G.Y; }
 
-// Chebyshev step 0. mode 1: r = R0 (MINRES start); mode 2: r_new = u - (alfa/beta) r2.
-__device__ double ph_cheb_first(const BatchArgs& a, int64_t vb, int mode) {
-  const Lane L = lane_ctx(a, vb);
-  double s2 = 0.0;
-  const MinresScalars* S = L.valid ? &a.S[L.inst] : nullptr;
-  if (L.valid && !S->done) {
-    const BGroup& G = *L.G;
-    const int it = mode == 2 ? S->it : 0;
-    float2* r = rbuf(G, it);
+// Per-thread context of a launched block. All fold row tiles of a block
+// belong to one group and a thread keeps its lane (instance) in all of them,
+// so the group record and the instance's MINRES scalars are read once per
+// thread, not once per tile.
+struct Ctx {
+  const BGroup* G;
+  int b, inst, rows_per_vb;
+  int64_t row0;  // the thread's row in the block's first tile
+  bool lane_ok;  // a real instance (b < B)
+};
+
+__device__ __forceinline__ Ctx make_ctx(const BatchArgs& a, int64_t v0) {
+  Ctx c;
+  c.G = &a.groups[group_of(a, v0)];
+  const int bp = c.G->bp;
+  c.b = threadIdx.x & (bp - 1);
+  c.inst = c.G->inst0 + c.b;
+  c.rows_per_vb = int(blockDim.x) / bp;
+  c.row0 = (v0 - c.G->block0) * c.rows_per_vb + threadIdx.x / bp;
+  c.lane_ok = c.b < c.G->B;
+  return c;
+}
+
+// Chebyshev step 0 over the block's rows. mode 1: r = R0 (MINRES start);
+// mode 2: r_new = u - (alfa/beta) r2. Returns the thread's <r, z> sum.
+__device__ double rows_cheb_first(const BatchArgs& a, const Ctx& c, int mode) {
+  double s = 0.0;
+  if (!c.lane_ok) return s;
+  const MinresScalars& S = a.S[c.inst];
+  const int done = S.done, it = mode == 2 ? S.it : 0;
+  const double alfa = S.alfa, beta = S.beta;
+  if (done) return s;
+  const BGroup& G = *c.G;
+  float2* r = rbuf(G, it);
+  const float2* r2 = rbuf(G, it + 1);
+  const float cab = mode == 2 ? float(alfa / beta) : 0.0f;
+  const double hi = hi_of(a, c.inst);
+  const float th = float(0.5 * (hi + hi / a.ratio));
+  const int64_t nb3 = G.n * G.bp;
+  float2* z0 = zbuf(G, a.z0);
+  for (int t = 0; t < a.fold; ++t) {
+    const int64_t row = c.row0 + int64_t(t) * c.rows_per_vb;
+    if (row >= G.n) break;
+    const int64_t idx = row * G.bp + c.b;
     float2 rv;
     if (mode == 2) {
-      const float2* r2 = rbuf(G, it + 1);
-      const float c = float(S->alfa / S->beta);
-      const float2 uu = G.u[L.idx], r2v = r2[L.idx];
-      rv = make_float2(uu.x - c * r2v.x, uu.y - c * r2v.y);
-      r[L.idx] = rv;
+      const float2 uu = ldn(G.u + idx), r2v = ldn(r2 + idx);
+      rv = make_float2(uu.x - cab * r2v.x, uu.y - cab * r2v.y);
+      stn(r + idx, rv);
     } else {
-      rv = r[L.idx];
+      rv = ldn(r + idx);
     }
-    const int64_t nb3 = G.n * G.bp;
     float2 z;
     if (a.iters == 0) {
       z = rv;
     } else {
-      const double hi = hi_of(a, L.inst);
-      const float th = float(0.5 * (hi + hi / a.ratio));
       float o0, o1;
-      dinv<float>(G.pblk[L.idx], G.pblk[nb3 + L.idx], G.pblk[2 * nb3 + L.idx], rv.x, rv.y, &o0,
-                  &o1);
+      dinv<float>(ldn(G.pblk + idx), ldn(G.pblk + nb3 + idx), ldn(G.pblk + 2 * nb3 + idx), rv.x,
+                  rv.y, &o0, &o1);
       z = make_float2(o0 / th, o1 / th);
-      if (a.iters > 1) G.D[L.idx] = z;
+      if (a.iters > 1) stn(G.D + idx, z);
     }
-    zbuf(G, a.z0)[L.idx] = z;
-    s2 = double(rv.x) * double(z.x) + double(rv.y) * double(z.y);
+    stn(z0 + idx, z);
+    s += double(rv.x) * double(z.x) + double(rv.y) * double(z.y);
   }
-  return s2;
+  return s;
 }
 
-__device__ double ph_cheb_step(const BatchArgs& a, int64_t vb, int mode, int j) {
-  const Lane L = lane_ctx(a, vb);
-  double s2 = 0.0;
+// Chebyshev step j over the block's rows; returns the thread's <r, z_j> sum.
+__device__ double rows_cheb_step(const BatchArgs& a, const Ctx& c, int mode, int j) {
+  double s = 0.0;
+  if (!c.lane_ok) return s;
+  const MinresScalars& S = a.S[c.inst];
+  const int done = S.done, it = mode == 2 ? S.it : 0;
+  if (done) return s;
+  const BGroup& G = *c.G;
   const bool last = (j == a.iters - 1);
-  const MinresScalars* S = L.valid ? &a.S[L.inst] : nullptr;
-  if (L.valid && !S->done) {
-    const BGroup& G = *L.G;
-    const int it = mode == 2 ? S->it : 0;
-    const float2* r = rbuf(G, it);
-    const float2* zin = zbuf(G, a.z0 + j - 1);
-    float2* zout = zbuf(G, a.z0 + j);
-    const Cheb c = cheb_coef(hi_of(a, L.inst), a.ratio, j);
-    const float2 acc = brow<float>(G, L.row, L.b, float(a.mu[L.inst]), zin);
-    const float2 zi = zin[L.idx];
-    const int64_t nb3 = G.n * G.bp;
-    const float b0 = G.pblk[L.idx], b1 = G.pblk[nb3 + L.idx], b2 = G.pblk[2 * nb3 + L.idx];
-    const float2 rv = r[L.idx];
+  const float2* r = rbuf(G, it);
+  const float2* zin = zbuf(G, a.z0 + j - 1);
+  float2* zout = zbuf(G, a.z0 + j);
+  const Cheb cc = cheb_coef(hi_of(a, c.inst), a.ratio, j);
+  const float c1 = float(cc.c1), c2 = float(cc.c2);
+  const float mu = float(a.mu[c.inst]);
+  const int64_t nb3 = G.n * G.bp;
+  for (int t = 0; t < a.fold; ++t) {
+    const int64_t row = c.row0 + int64_t(t) * c.rows_per_vb;
+    if (row >= G.n) break;
+    const int64_t idx = row * G.bp + c.b;
+    const float2 acc = brow<float>(G, row, c.b, mu, zin);
+    const float2 zi = zin[idx];
+    const float b0 = ldn(G.pblk + idx), b1 = ldn(G.pblk + nb3 + idx),
+                b2 = ldn(G.pblk + 2 * nb3 + idx);
+    const float2 rv = ldn(r + idx);
     const float res0 = rv.x - (acc.x + b0 * zi.x + b2 * zi.y);
     const float res1 = rv.y - (acc.y + b2 * zi.x + b1 * zi.y);
     float u0, u1;
     dinv<float>(b0, b1, b2, res0, res1, &u0, &u1);
-    float2 d = G.D[L.idx];
-    d.x = float(c.c1) * d.x + float(c.c2) * u0;
-    d.y = float(c.c1) * d.y + float(c.c2) * u1;
+    float2 d = ldn(G.D + idx);
+    d.x = c1 * d.x + c2 * u0;
+    d.y = c1 * d.y + c2 * u1;
     const float2 z = make_float2(zi.x + d.x, zi.y + d.y);
-    if (!last) G.D[L.idx] = d;
-    zout[L.idx] = z;
-    s2 = double(rv.x) * double(z.x) + double(rv.y) * double(z.y);
+    if (!last) stn(G.D + idx, d);
+    stn(zout + idx, z);
+    if (last) s += double(rv.x) * double(z.x) + double(rv.y) * double(z.y);
   }
-  return last ? s2 : 0.0;
+  return s;
 }
 
-__device__ __forceinline__ void w_update(const BGroup& G, const MinresScalars* S, int k,
-                                         int64_t idx, float2 vk) {
+// Pending update of iteration k: w_k = (v_k - oldeps w_{k-2} - delta w_{k-1}) / gamma,
+// x += phi w_k.
+struct WUpd {
+  float oe, de, dn;
+  double phi;
+};
+
+__device__ __forceinline__ void w_update(const BGroup& G, const WUpd& q, int k, int64_t idx,
+                                         float2 vk) {
   float2* wk = wbuf(G, k);
-  const float2 w1 = wbuf(G, k - 2)[idx], w2 = wbuf(G, k - 1)[idx];
-  const float oe = float(S->p_oldeps), de = float(S->p_delta), dn = float(S->p_denom);
-  const float2 w = make_float2((vk.x - oe * w1.x - de * w2.x) * dn,
-                               (vk.y - oe * w1.y - de * w2.y) * dn);
-  wk[idx] = w;
-  double2 xv = G.x[idx];
-  xv.x += S->p_phi * double(w.x);
-  xv.y += S->p_phi * double(w.y);
-  G.x[idx] = xv;
+  const float2 w1 = ldn(wbuf(G, k - 2) + idx), w2 = ldn(wbuf(G, k - 1) + idx);
+  const float2 w = make_float2((vk.x - q.oe * w1.x - q.de * w2.x) * q.dn,
+                               (vk.y - q.oe * w1.y - q.de * w2.y) * q.dn);
+  stn(wk + idx, w);
+  double2 xv = ldn(G.x + idx);
+  xv.x += q.phi * double(w.x);
+  xv.y += q.phi * double(w.y);
+  stn(G.x + idx, xv);
 }
 
-// v = y / beta, u = J v - (beta/beta_old) r1, previous w/x update, <v,u> partials.
-__device__ double ph_minres_a(const BatchArgs& a, int64_t vb) {
-  const Lane L = lane_ctx(a, vb);
+__device__ __forceinline__ WUpd wupd_of(const MinresScalars& S) {
+  WUpd q;
+  q.oe = float(S.p_oldeps);
+  q.de = float(S.p_delta);
+  q.dn = float(S.p_denom);
+  q.phi = S.p_phi;
+  return q;
+}
+
+// v = y / beta, u = J v - (beta/beta_old) r1, the previous w/x update, over
+// the block's rows; returns the thread's <v, u> sum.
+__device__ double rows_minres_a(const BatchArgs& a, const Ctx& c) {
   double s = 0.0;
-  const MinresScalars* S = L.valid ? &a.S[L.inst] : nullptr;
-  if (L.valid && !S->done) {
-    const BGroup& G = *L.G;
-    const int it = S->it;
-    const double beta = S->beta;
-    const float2 vold = G.v[L.idx];
-    if (S->pending) w_update(G, S, it, L.idx, vold);
-    const float2* y = zbuf(G, 0);
-    const float2 acc = brow<float>(G, L.row, L.b, float(a.mu[L.inst]), y);
-    const float2 yi = y[L.idx];
-    const int64_t nb3 = G.n * G.bp;
-    const float b0 = G.jblk[L.idx], b1 = G.jblk[nb3 + L.idx], b2 = G.jblk[2 * nb3 + L.idx];
-    const float ib = float(1.0 / beta);
+  if (!c.lane_ok) return s;
+  const MinresScalars& S = a.S[c.inst];
+  const int done = S.done, it = S.it, pending = S.pending;
+  const double beta = S.beta, oldb = S.oldb;
+  const WUpd q = wupd_of(S);
+  if (done) return s;
+  const BGroup& G = *c.G;
+  const float2* y = zbuf(G, 0);
+  const float2* r1 = rbuf(G, it);
+  const float ib = float(1.0 / beta);
+  const float cb = it >= 1 ? float(beta / oldb) : 0.0f;
+  const float mu = float(a.mu[c.inst]);
+  const int64_t nb3 = G.n * G.bp;
+  for (int t = 0; t < a.fold; ++t) {
+    const int64_t row = c.row0 + int64_t(t) * c.rows_per_vb;
+    if (row >= G.n) break;
+    const int64_t idx = row * G.bp + c.b;
+    if (pending) w_update(G, q, it, idx, ldn(G.v + idx));
+    const float2 acc = brow<float>(G, row, c.b, mu, y);
+    const float2 yi = y[idx];
+    const float b0 = ldn(G.jblk + idx), b1 = ldn(G.jblk + nb3 + idx),
+                b2 = ldn(G.jblk + 2 * nb3 + idx);
     float2 uu = make_float2((acc.x + b0 * yi.x + b2 * yi.y) * ib,
                             (acc.y + b2 * yi.x + b1 * yi.y) * ib);
     if (it >= 1) {
-      const float2 rv = rbuf(G, it)[L.idx];
-      const float c = float(beta / S->oldb);
-      uu.x -= c * rv.x;
-      uu.y -= c * rv.y;
+      const float2 rv = ldn(r1 + idx);
+      uu.x -= cb * rv.x;
+      uu.y -= cb * rv.y;
     }
     const float2 vn = make_float2(yi.x * ib, yi.y * ib);
-    G.v[L.idx] = vn;
-    G.u[L.idx] = uu;
-    s = double(vn.x) * double(uu.x) + double(vn.y) * double(uu.y);
+    stn(G.v + idx, vn);
+    stn(G.u + idx, uu);
+    s += double(vn.x) * double(uu.x) + double(vn.y) * double(uu.y);
   }
   return s;
 }
@@ -369,8 +485,8 @@ __device__ void ph_flush(const BatchArgs& a, int64_t vb) {
   const Lane L = lane_ctx(a, vb);
   if (!L.valid || !a.active[L.inst]) return;
   const BGroup& G = *L.G;
-  const MinresScalars* S = &a.S[L.inst];
-  if (S->pending) w_update(G, S, S->it, L.idx, G.v[L.idx]);
+  const MinresScalars& S = a.S[L.inst];
+  if (S.pending) w_update(G, wupd_of(S), S.it, L.idx, G.v[L.idx]);
   const double2 d = G.x[L.idx];
   double2 p = G.psi[L.idx];
   p.x += d.x;
@@ -413,21 +529,17 @@ __device__ __forceinline__ void blk_compute_f(const BatchArgs& a, int64_t rb) {
 }
 __device__ __forceinline__ void blk_cheb_first(const BatchArgs& a, int64_t rb, int mode) {
   const int64_t v0 = rb * a.fold;
-  double s = 0.0;
-  for (int r = 0; r < a.fold; ++r) s += ph_cheb_first(a, v0 + r, mode);
+  const double s = rows_cheb_first(a, make_ctx(a, v0), mode);
   if (a.iters <= 1) lane_partials(a, v0, s);
 }
 __device__ __forceinline__ void blk_cheb_step(const BatchArgs& a, int64_t rb, int mode, int j) {
   const int64_t v0 = rb * a.fold;
-  double s = 0.0;
-  for (int r = 0; r < a.fold; ++r) s += ph_cheb_step(a, v0 + r, mode, j);
+  const double s = rows_cheb_step(a, make_ctx(a, v0), mode, j);
   if (j == a.iters - 1) lane_partials(a, v0, s);
 }
 __device__ __forceinline__ void blk_minres_a(const BatchArgs& a, int64_t rb) {
   const int64_t v0 = rb * a.fold;
-  double s = 0.0;
-  for (int r = 0; r < a.fold; ++r) s += ph_minres_a(a, v0 + r);
-  lane_partials(a, v0, s);
+  lane_partials(a, v0, rows_minres_a(a, make_ctx(a, v0)));
 }
 
 // Launched blocks are remapped so that each XCD works on one contiguous

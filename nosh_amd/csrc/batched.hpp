@@ -18,8 +18,8 @@ struct BGroup {
   int64_t nblocks;  // blocks of this group
   const int32_t* slice_ptr;
   const int32_t* scol;
-  const float* a32;   // per padded slot: alpha (0 for padding)
-  const float* t32;   //                  a~ signed by direction
+  const int4* s32;    // per padded slot, FP32 path: {column, alpha, a~ signed by
+                      // direction, 0} (alpha 0 for padding), one 16-byte load
   const double* a64;
   const double* t64;
   const double* diag;    // per row: sum alpha
@@ -56,7 +56,7 @@ struct BatchArgs {
 struct GroupHost {
   MeshCache* mesh = nullptr;
   int B = 0;
-  DBuf<float> a32, t32;
+  DBuf<int4> s32;
   DBuf<double> a64, t64, diag, rowabs, ct, V;
   DBuf<double2> psi, f, x;
   DBuf<float> jblk, pblk;
@@ -103,8 +103,8 @@ struct Batch {
 // Per-slot (alpha, signed a~) in FP32 and FP64 plus row sums, from the edge
 // weights of the mesh (operators.hip).
 void launch_slot_fields(const MeshCache& m, const double* alpha, const double* atilde,
-                        float* a32, float* t32, double* a64, double* t64, double* diag,
-                        double* rowabs, hipStream_t s);
+                        int4* s32, double* a64, double* t64, double* diag, double* rowabs,
+                        hipStream_t s);
 // Reference-order instance vector -> lane b of an interleaved [n][bp] array, and back.
 void launch_scatter_lane(int64_t n, int bp, int b, const int32_t* perm, const double* src,
                          double* dst, hipStream_t s);

@@ -414,10 +414,11 @@ void launch_cv_weighted(const MeshCache& m, const double* a, const double* b, do
 
 namespace nosh {
 namespace {
-__global__ void k_slot_fields(const int32_t* __restrict__ sedge, const double* __restrict__ alpha,
+__global__ void k_slot_fields(const int32_t* __restrict__ sedge, const int32_t* __restrict__ scol,
+                              const double* __restrict__ alpha,
                               const double* __restrict__ atilde, int64_t nnz,
-                              float* __restrict__ a32, float* __restrict__ t32,
-                              double* __restrict__ a64, double* __restrict__ t64) {
+                              int4* __restrict__ s32, double* __restrict__ a64,
+                              double* __restrict__ t64) {
   const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= nnz) return;
   const int32_t code = sedge[j];
@@ -426,8 +427,7 @@ __global__ void k_slot_fields(const int32_t* __restrict__ sedge, const double* _
     al = alpha[code >> 1];
     at = (code & 1) ? -atilde[code >> 1] : atilde[code >> 1];
   }
-  a32[j] = float(al);
-  t32[j] = float(at);
+  s32[j] = make_int4(scol[j], __float_as_int(float(al)), __float_as_int(float(at)), 0);
   a64[j] = al;
   t64[j] = at;
 }
@@ -446,10 +446,10 @@ __global__ void k_gather_lane(int64_t n, int bp, int b, const int32_t* __restric
 }  // namespace
 
 void launch_slot_fields(const MeshCache& m, const double* alpha, const double* atilde,
-                        float* a32, float* t32, double* a64, double* t64, double* diag,
-                        double* rowabs, hipStream_t s) {
-  k_slot_fields<<<grid_for(m.nnz_pad, kBlock), kBlock, 0, s>>>(m.sedge.p, alpha, atilde,
-                                                               m.nnz_pad, a32, t32, a64, t64);
+                        int4* s32, double* a64, double* t64, double* diag, double* rowabs,
+                        hipStream_t s) {
+  k_slot_fields<<<grid_for(m.nnz_pad, kBlock), kBlock, 0, s>>>(
+      m.sedge.p, m.scol.p, alpha, atilde, m.nnz_pad, s32, a64, t64);
   k_keo_diag<<<grid_for(m.n, kBlock), kBlock, 0, s>>>(m.slice_ptr.p, m.sedge.p, alpha, m.n, diag,
                                                       rowabs);
 }
