@@ -197,6 +197,53 @@ __device__ __forceinline__ void brow_slots(const BGroup& G, int32_t j, int b, T 
 
 constexpr int kSlotBatch = 4;
 
+#ifndef NOSH_META_SHFL
+#define NOSH_META_SHFL 1
+#endif
+
+// Groups whose rows take their slot metadata by lane shuffles: every lane is
+// a real instance and the callers keep all lanes of a row in brow (lanes of
+// converged instances compute and discard).
+__device__ __forceinline__ bool meta_shfl(const BGroup& G) {
+  return NOSH_META_SHFL && G.B == G.bp && G.bp >= 4;
+}
+
+// FP32 rows of a full group (B == bp >= 4 lanes, all lanes active): lane q of
+// the row's lanes loads the metadata of slot k + q once and the others take
+// it by a lane shuffle, instead of every lane loading the same 16 bytes
+// (which costs the texture data path bp times the useful bytes).
+__device__ __forceinline__ void brow_shfl4(const BGroup& G, int32_t j, int32_t k, int32_t w,
+                                           int b, float mu, const float2* __restrict__ x,
+                                           float& ar, float& ai) {
+  const int lane = int(threadIdx.x & 63);
+  const int g0 = lane & ~(G.bp - 1);
+  int4 mine = make_int4(0, 0, 0, 0);
+  if (b < 4 && k + b < w) mine = G.s32[j + b * kSliceH];
+  int32_t c[4];
+  float av[4], tv[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    c[q] = __shfl(mine.x, g0 + q, 64);
+    av[q] = __int_as_float(__shfl(mine.y, g0 + q, 64));
+    tv[q] = __int_as_float(__shfl(mine.z, g0 + q, 64));
+  }
+  float2 xv[4];
+  const int bp = G.bp;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (k + q < w) xv[q] = x[int64_t(c[q]) * bp + b];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (k + q < w) {
+      float sn, cs;
+      sc(mu * tv[q], &sn, &cs);
+      const float vr = -av[q] * cs, vi = av[q] * sn;
+      ar += vr * xv[q].x - vi * xv[q].y;
+      ai += vr * xv[q].y + vi * xv[q].x;
+    }
+  }
+}
+
 template <class T>
 __device__ __forceinline__ typename Vt<T>::v2 brow(const BGroup& G, int64_t row, int b, T mu,
                                                     const typename Vt<T>::v2* __restrict__ x) {
@@ -206,6 +253,15 @@ __device__ __forceinline__ typename Vt<T>::v2 brow(const BGroup& G, int64_t row,
   T ar = T(0), ai = T(0);
   int32_t j = beg + int32_t(row & 63);
   int32_t k = 0;
+  if constexpr (std::is_same<T, float>::value) {
+    if (meta_shfl(G)) {
+      for (; k < w; k += 4, j += 4 * kSliceH) brow_shfl4(G, j, k, w, b, mu, x, ar, ai);
+      typename Vt<T>::v2 r;
+      r.x = ar;
+      r.y = ai;
+      return r;
+    }
+  }
   for (; k + kSlotBatch <= w; k += kSlotBatch, j += kSlotBatch * kSliceH)
     brow_slots<T, kSlotBatch>(G, j, b, mu, x, ar, ai);
   for (; k < w; ++k, j += kSliceH) brow_slots<T, 1>(G, j, b, mu, x, ar, ai);
@@ -377,8 +433,8 @@ __device__ double rows_cheb_step(const BatchArgs& a, const Ctx& c, int mode, int
   if (!c.lane_ok) return s;
   const MinresScalars& S = a.S[c.inst];
   const int done = S.done, it = mode == 2 ? S.it : 0;
-  if (done) return s;
   const BGroup& G = *c.G;
+  if (meta_shfl(G) ? __all(done) : done) return s;
   const bool last = (j == a.iters - 1);
   const float2* r = rbuf(G, it);
   const float2* zin = zbuf(G, a.z0 + j - 1);
@@ -392,6 +448,7 @@ __device__ double rows_cheb_step(const BatchArgs& a, const Ctx& c, int mode, int
     if (row >= G.n) break;
     const int64_t idx = row * G.bp + c.b;
     const float2 acc = brow<float>(G, row, c.b, mu, zin);
+    if (done) continue;
     const float2 zi = zin[idx];
     const float b0 = ldn(G.pblk + idx), b1 = ldn(G.pblk + nb3 + idx),
                 b2 = ldn(G.pblk + 2 * nb3 + idx);
@@ -449,8 +506,8 @@ __device__ double rows_minres_a(const BatchArgs& a, const Ctx& c) {
   const int done = S.done, it = S.it, pending = S.pending;
   const double beta = S.beta, oldb = S.oldb;
   const WUpd q = wupd_of(S);
-  if (done) return s;
   const BGroup& G = *c.G;
+  if (meta_shfl(G) ? __all(done) : done) return s;
   const float2* y = zbuf(G, 0);
   const float2* r1 = rbuf(G, it);
   const float ib = float(1.0 / beta);
@@ -461,8 +518,9 @@ __device__ double rows_minres_a(const BatchArgs& a, const Ctx& c) {
     const int64_t row = c.row0 + int64_t(t) * c.rows_per_vb;
     if (row >= G.n) break;
     const int64_t idx = row * G.bp + c.b;
-    if (pending) w_update(G, q, it, idx, ldn(G.v + idx));
+    if (pending && !done) w_update(G, q, it, idx, ldn(G.v + idx));
     const float2 acc = brow<float>(G, row, c.b, mu, y);
+    if (done) continue;
     const float2 yi = y[idx];
     const float b0 = ldn(G.jblk + idx), b1 = ldn(G.jblk + nb3 + idx),
                 b2 = ldn(G.jblk + 2 * nb3 + idx);

@@ -101,3 +101,42 @@ def test_persistent_mode_refuses_large_grids():
     psi0 = np.tile(np.stack([np.ones(c.shape[0]), np.zeros(c.shape[0])], 1).ravel(), 8)
     psi, st = batch.solve(psi0, tol=1e-10, max_iters=30, minres_rtol=1e-6, forcing=0)
     assert all(s["converged"] for s in st)
+
+
+def test_full_group_matches_padded_group():
+    """A full group (8 instances, 8 lanes: slot metadata shared by lane
+    shuffles) and a padded one (the first 7 of them, 8 lanes: every lane
+    loads its own) take the same Newton steps and reach the same solutions
+    for the common instances, up to the global phase (the two code paths may
+    contract FMAs differently, so not bitwise), and every solution passes
+    the oracle."""
+    import nosh_amd as nosh
+    c, t = mg.kuhn_cube(6, jitter=0.2, seed=2)
+    m = _model(c, t)
+    n = c.shape[0]
+    u = mg.splitmix64_uniform(5, 16).reshape(8, 2)
+    # Below the first vortex entry, where Newton from psi ~ 1 converges quickly.
+    mu, g = 0.25 * u[:, 0], 0.8 + 0.4 * u[:, 1]
+    psi0 = []
+    for k in range(8):
+        p = 0.1 * mg.random_state(n, 300 + k)
+        p[0::2] += 1.0
+        psi0.append(p)
+    opts = dict(tol=1e-10, max_iters=40, minres_rtol=1e-6, minres_max_iters=4000, cheb_iters=4,
+                cheb_ratio=100.0, forcing=0)
+    b8 = nosh.Batch([(m, 8)], mu, g)
+    b7 = nosh.Batch([(m, 7)], mu[:7], g[:7])
+    for b in (b8, b7):
+        b.set_tuning(256, 0, 4, 16, fold=2)
+    p8, s8 = b8.solve(np.concatenate(psi0), **opts)
+    p7, s7 = b7.solve(np.concatenate(psi0[:7]), **opts)
+    print("full group", [(s["newton_iters"], s["converged"]) for s in s8])
+    print("padded group", [(s["newton_iters"], s["converged"]) for s in s7])
+    assert all(s["converged"] for s in s8) and all(s["converged"] for s in s7)
+    assert [s["newton_iters"] for s in s8[:7]] == [s["newton_iters"] for s in s7]
+    # Rounding differences move the solution along the gauge direction i psi
+    # (J's null space), which F does not see: compare up to the global phase.
+    from conftest import phase_align
+    for a_, b_ in zip(b8.split(p8)[:7], b7.split(p7)):
+        assert np.abs(phase_align(a_, b_) - b_).max() <= 1e-9
+    _check_with_oracle([(c, t)], [8], mu, g, b8.split(p8))
