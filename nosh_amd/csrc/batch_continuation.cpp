@@ -13,7 +13,8 @@
 // branch starts from whichever of the two guesses has the smaller ||F||. A
 // branch whose Newton from the surrogate guess does not converge within the
 // budget, or whose Gibbs energy leaves its linear trend, is solved again
-// from psi_sec, then, if that fails too (a branch transition), from psi_k.
+// from psi_sec; at a branch transition, where Newton from the secant guess
+// may wander, psi_k is tried too (short tries of both, then full ones).
 //
 // This is the caller the reference delegates to LOCA (natural stepping,
 // executables/nosh-cont/nosh-cont.cpp:226-290,344 with examples/conf.xml:35-190);
@@ -53,6 +54,7 @@ int bcguard(F&& f) {
   }
 }
 
+constexpr int kShortNewton = 20;  // Newton steps of the first try from each guess
 constexpr int kRed = 256;       // threads of the per-lane reduction blocks
 constexpr int kRedBlocks = 512; // blocks of the per-lane reduction (grid-stride)
 
@@ -403,7 +405,7 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
         int nsur = 0;
         for (int i = 0; i < ni; ++i) {
           use_sur[i] = nf_sur[i] < nf_sec[i];
-          maxit[i] = use_sur[i] ? so->newton_budget : o.max_iters;
+          maxit[i] = use_sur[i] ? so->newton_budget : std::min(o.max_iters, kShortNewton);
           tried_sec[i] = !use_sur[i];
           nsur += use_sur[i];
         }
@@ -439,18 +441,24 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
       }
       // Secant guess for every branch not yet accepted, then psi_k.
       // Branches not yet accepted: Newton from the secant guess (unless a
-      // branch already started from it), then from psi_k.
-      for (int attempt = 0; attempt < 2; ++attempt) {
+      // branch already started from it), then from psi_k, first with at most
+      // kShortNewton steps each, then with the full limit. Near a branch
+      // transition Newton wanders from one of the two guesses and converges
+      // from the other; the short tries find out which at a fraction of the
+      // cost of a full failed run.
+      for (int attempt = 0; attempt < 4; ++attempt) {
+        const bool from_sec = (attempt % 2) == 0;
+        const int cap = attempt < 2 ? std::min(o.max_iters, kShortNewton) : o.max_iters;
         int todo = 0;
         for (int i = 0; i < ni; ++i) {
           skip[i] = ok[i] || (attempt == 0 && tried_sec[i]);
           todo += !skip[i];
-          maxit[i] = o.max_iters;
+          maxit[i] = cap;
         }
         if (todo == 0) continue;
         std::vector<int> take(ni);
         for (int i = 0; i < ni; ++i) take[i] = !skip[i];
-        take_into_psi(attempt == 0 ? &GroupCont::sec : &GroupCont::cur, take);
+        take_into_psi(from_sec ? &GroupCont::sec : &GroupCont::cur, take);
         const int it = newton_batch(bt, o, st.data(), s, skip.data(), maxit.data());
         int mx = 0;
         for (int i = 0; i < ni; ++i)

@@ -311,12 +311,24 @@ struct InferArgs {
   float* Y;
 };
 
+// The weights of a hidden layer are staged through LDS in chunks of TC
+// output tiles (32 TC rows of W) shared by the block's 4 waves, instead of
+// every wave reading the whole layer from L2; the inputs of layer 0 use the
+// same LDS region before the first chunk.
 template <int NT>
 __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
   constexpr int W = NT * 32;
-  constexpr int LD = W + 8;  // padded row (bf16) against LDS bank conflicts
+  constexpr int LD = W + 8;            // padded row (bf16) against LDS bank conflicts
+  constexpr int TC = NT < 4 ? NT : 4;  // output tiles per staged weight chunk
+  constexpr int NC = NT / TC;          // chunks per layer
+  constexpr int WROWS = 32 * TC;
+  constexpr int WBYTES = WROWS * LD * 2 > kInferRows * kInferMaxIn * 4
+                             ? WROWS * LD * 2
+                             : kInferRows * kInferMaxIn * 4;
   __shared__ __bf16 act[kInferRows][LD];
-  __shared__ float xs[kInferRows][kInferMaxIn];
+  __shared__ __attribute__((aligned(16))) unsigned char wbuf[WBYTES];
+  __bf16 (*wl)[LD] = reinterpret_cast<__bf16 (*)[LD]>(wbuf);
+  float (*xs)[kInferMaxIn] = reinterpret_cast<float (*)[kInferMaxIn]>(wbuf);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t r0 = int64_t(blockIdx.x) * kInferRows;
   const int rows = int(min<int64_t>(kInferRows, g.M - r0));
@@ -336,7 +348,6 @@ __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
       act[r][n] = __bf16(gelu(z));
     }
   }
-  __syncthreads();
   // Hidden layers: each wave owns rows 32 wave .. 32 wave + 31.
   const int row = wave * 32 + (lane & 31);
   const int kh = 8 * (lane >> 5);
@@ -346,17 +357,25 @@ __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
 #pragma unroll
     for (int t = 0; t < NT; ++t)
       for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch) {
+      __syncthreads();  // previous users of wl (xs, or the last chunk) are done
+      for (int v = tid; v < WROWS * (W / 8); v += 256) {
+        const int rr = v / (W / 8), cc = v - rr * (W / 8);
+        *reinterpret_cast<bf16x8*>(&wl[rr][cc * 8]) =
+            *reinterpret_cast<const bf16x8*>(Wl + int64_t(ch * WROWS + rr) * W + cc * 8);
+      }
+      __syncthreads();
 #pragma unroll 2
-    for (int kc = 0; kc < W / 16; ++kc) {
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&act[row][kc * 16 + kh]);
-      bf16x8 b[NT];
+      for (int kc = 0; kc < W / 16; ++kc) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&act[row][kc * 16 + kh]);
 #pragma unroll
-      for (int t = 0; t < NT; ++t)
-        b[t] = *reinterpret_cast<const bf16x8*>(Wl + int64_t(t * 32 + (lane & 31)) * W +
-                                                  kc * 16 + kh);
-#pragma unroll
-      for (int t = 0; t < NT; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[t], acc[t], 0, 0, 0);
+        for (int t = 0; t < TC; ++t) {
+          const bf16x8 b = *reinterpret_cast<const bf16x8*>(&wl[t * 32 + (lane & 31)][kc * 16 + kh]);
+          acc[ch * TC + t] =
+              __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[ch * TC + t], 0, 0, 0);
+        }
+      }
     }
     const float* bl = g.bias + g.boff[l];
 #pragma unroll
