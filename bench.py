@@ -84,6 +84,8 @@ def parse():
     p.add_argument("--cont-retrain", type=int, default=0,
                    help="Adam steps per continuation step after the first training")
     p.add_argument("--cont-batch", type=int, default=8192, help="surrogate minibatch rows")
+    p.add_argument("--cont-train-steps", type=int, default=150,
+                   help="Adam steps of the surrogate's first training")
     p.add_argument("--cont-plain-branches", type=int, default=1,
                    help="branches run with the plain FP64 continuation for the ratio")
     p.add_argument("--cont-tuning", default="",
@@ -213,7 +215,7 @@ def continuation_phase(args, model, n):
     t_tune = time.perf_counter() - t0
     say(f"continuation: autotune {t_tune:.2f}s -> {tuned}")
     sur = None if args.no_surrogate else dict(
-        width=256, depth=4, train_after=args.cont_train_after, train_steps=300,
+        width=256, depth=4, train_after=args.cont_train_after, train_steps=args.cont_train_steps,
         batch=args.cont_batch, lr=1e-3, newton_budget=6, retrain_steps=args.cont_retrain)
     if args.warmup > 0:
         t0 = time.perf_counter()

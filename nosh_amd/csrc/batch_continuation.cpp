@@ -9,8 +9,9 @@
 // With a surrogate, after `train_after` steps a per-vertex MLP (BF16 MFMA,
 // trained on the run's own solutions) seeds Newton with
 //   psi_guess = psi_sec + h^2 sigma Y(x, psi_k, h s, mu, g),   h s = psi_sec - psi_k,
-// sigma the RMS of the training targets (psi_{k+1} - psi_sec) / h^2; each
-// branch starts from whichever of the two guesses has the smaller ||F||. A
+// sigma the RMS of the training targets (psi_{k+1} - psi_sec) / h^2; a
+// branch starts from the surrogate guess when its ||F|| is at most half the
+// secant guess's, else from the secant guess. A
 // branch whose Newton from the surrogate guess does not converge within the
 // budget, or whose Gibbs energy leaves its linear trend, is solved again
 // from psi_sec; at a branch transition, where Newton from the secant guess
@@ -55,6 +56,11 @@ int bcguard(F&& f) {
 }
 
 constexpr int kShortNewton = 20;  // Newton steps of the first try from each guess
+// A surrogate guess is used only when its residual is at most this fraction
+// of the secant guess's: inexact Newton (Eisenstat-Walker forcing) takes the
+// same number of steps, and often more MINRES iterations, from a guess that
+// is only slightly better.
+constexpr double kSurrogateMargin = 0.5;
 constexpr int kRed = 256;       // threads of the per-lane reduction blocks
 constexpr int kRedBlocks = 512; // blocks of the per-lane reduction (grid-stride)
 
@@ -404,7 +410,7 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
         batch_residuals(bt, nf_sec, s);
         int nsur = 0;
         for (int i = 0; i < ni; ++i) {
-          use_sur[i] = nf_sur[i] < nf_sec[i];
+          use_sur[i] = nf_sur[i] < kSurrogateMargin * nf_sec[i];
           maxit[i] = use_sur[i] ? so->newton_budget : std::min(o.max_iters, kShortNewton);
           tried_sec[i] = !use_sur[i];
           nsur += use_sur[i];
