@@ -91,7 +91,7 @@ def parse():
     p.add_argument("--cont-tuning", default="",
                    help="block_threads,fold for the batched continuation (skips the autotuner)")
     p.add_argument("--no-surrogate", action="store_true")
-    p.add_argument("--workload", default="cont96", choices=["cont96", "batched", "cont"],
+    p.add_argument("--workload", default="cont96", choices=["cont96", "batched", "cont", "disk"],
                    help="cont96: the headline (continuation, side-by-side solves, roofline, "
                         "CPU baseline); cont: the continuation part only; batched: 128 uneven "
                         "random-start instances on jittered 16^3/24^3/32^3/48^3 cubes")
@@ -100,7 +100,7 @@ def parse():
                    help="batched: instances solved one by one with plain FP64 for the ratio")
     a = p.parse_args()
     if a.steps is None:
-        a.steps = 1 if a.workload == "batched" else 50
+        a.steps = {"batched": 1, "disk": 200}.get(a.workload, 50)
     if a.warmup is None:
         a.warmup = 1 if a.workload == "batched" else 2
     return a
@@ -504,6 +504,58 @@ def headline(args):
     print(json.dumps(out))
 
 
+def disk_workload(args):
+    """The disk workload: Delaunay disk of radius 5 (h = 0.0213, about 200 k
+    vertices, seed 3), V = -1, g = 1, mu from 0 to 2 in 200 steps from
+    psi = 1. The surrogate-seeded continuation (MLP trained on the first 40
+    solutions, tangent fallback) against the plain tangent continuation, both
+    with FP64 Newton-MINRES: Newton iterations and seconds per step."""
+    import torch
+    import nosh_amd as nosh
+    from nosh_amd import meshgen as mg
+    torch.cuda.init()
+    coords, cells = mg.disk(0.0213)
+    n = coords.shape[0]
+    mesh = nosh.Mesh(coords, cells)
+    mvp = nosh.VectorFieldExplicitValues(mesh, mg.magnetic_potential(coords), 0.0)
+    model = nosh.Nls(mesh, mvp, nosh.ScalarFieldConstant(mesh, -1.0), 1.0,
+                     nosh.ScalarFieldConstant(mesh, 1.0))
+    say(f"disk workload: n = {n}, {cells.shape[0]} triangles")
+    psi0 = np.zeros(2 * n)
+    psi0[0::2] = 1.0
+    steps = args.steps
+    kw = dict(tol=1e-10, max_iters=40, minres_rtol=1e-10, minres_max_iters=args.minres_maxit,
+              cheb_iters=args.cheb, cheb_ratio=args.ratio)
+    t0 = time.perf_counter()
+    _, rec_s, st = nosh.continuation_surrogate(
+        model, psi0, {"g": 1.0}, "mu", 0.0, 2.0, steps,
+        surrogate=dict(width=256, depth=4, train_after=40, train_steps=300, batch=16384,
+                       lr=1e-3, newton_budget=4), **kw)
+    t_s = time.perf_counter() - t0
+    say(f"disk: surrogate run {t_s:.1f}s, {st}")
+    t0 = time.perf_counter()
+    _, rec_p = nosh.continuation(model, psi0, {"g": 1.0}, "mu", 0.0, 2.0, steps, predictor=1,
+                                 **kw)
+    t_p = time.perf_counter() - t0
+    say(f"disk: plain run {t_p:.1f}s")
+    after = slice(41, None)
+    out = {"workload": f"Delaunay disk r = 5, n = {n}: mu 0 -> 2 in {steps} steps",
+           "surrogate": {"seconds": t_s, "steps_done": len(rec_s) - 1, "stats": st,
+                         "newton_total": sum(r["newton_iters"] for r in rec_s[1:]),
+                         "newton_after_training": sum(r["newton_iters"] for r in rec_s[after]),
+                         "seconds_after_training": sum(r["seconds"] for r in rec_s[after]),
+                         "seeded_steps": sum(r["predictor_used"] == 3 for r in rec_s)},
+           "plain": {"seconds": t_p, "steps_done": len(rec_p) - 1,
+                     "newton_total": sum(r["newton_iters"] for r in rec_p[1:]),
+                     "newton_after_training": sum(r["newton_iters"] for r in rec_p[after]),
+                     "seconds_after_training": sum(r["seconds"] for r in rec_p[after])},
+           "per_step_newton": {"surrogate": [r["newton_iters"] for r in rec_s],
+                               "plain": [r["newton_iters"] for r in rec_p]},
+           "per_step_seconds": {"surrogate": [round(r["seconds"], 4) for r in rec_s],
+                                "plain": [round(r["seconds"], 4) for r in rec_p]}}
+    print(json.dumps(out))
+
+
 def cont_workload(args):
     import torch
     torch.cuda.init()
@@ -620,6 +672,8 @@ def main():
         batched_workload(args)
     elif args.workload == "cont":
         cont_workload(args)
+    elif args.workload == "disk":
+        disk_workload(args)
     else:
         headline(args)
 

@@ -361,7 +361,8 @@ typedef struct {
   double seconds;     /* wall time of this step (predictor + corrector) */
   int predictor_used; /* 0 previous, 1 tangent, 2 secant, 3 surrogate */
   int fallback;       /* 1 if the surrogate guess was rejected for the tangent */
-  double pred_error;  /* ||psi_guess - psi_solution||_2 of the accepted guess */
+  double pred_error;  /* ||psi_surrogate_guess - psi_solution||_2 where the surrogate was
+                         evaluated, else equal to tan_error */
   double tan_error;   /* ||psi_tangent - psi_solution||_2 (surrogate runs) */
 } nosh_continuation_step;
 
@@ -379,11 +380,13 @@ int nosh_continuation_run(nosh_nls_t nls, double* psi, int nparams, const char* 
  * tangent run above, plus a per-vertex MLP that predicts the second-order
  * correction (psi_{k+1} - psi_k - h t_k) / h^2 from (x, y[, z], Re/Im psi,
  * h Re/Im t, mu). It trains on the run's own first train_after accepted
- * steps (train_steps Adam steps on random vertex minibatches), then seeds
- * Newton with psi_k + h t_k + h^2 MLP(...); the guess is rejected for the
- * plain tangent predictor when Newton needs more than newton_budget steps
- * or the converged Gibbs energy leaves the linear extrapolation of the last
- * two steps by more than max(energy_tol, 10 |E_k - E_{k-1}|). */
+ * steps (train_steps Adam steps on random vertex minibatches), then
+ * evaluates the guess psi_k + h t_k + h^2 MLP(...) at every step and starts
+ * Newton from it when its ||F|| is at most half the tangent guess's; the
+ * guess is rejected for the plain tangent predictor when Newton needs more
+ * than newton_budget steps or the converged Gibbs energy leaves the linear
+ * extrapolation of the last two steps by more than
+ * max(energy_tol, 10 |E_k - E_{k-1}|). */
 typedef struct {
   int width, depth;     /* 256, 4 */
   int train_after;      /* 40 */

@@ -733,6 +733,24 @@ static void nls_f_dev(nosh_nls_s* nls, const std::map<std::string, double>& p, c
   launch_compute_f(m, nls->keo->kv, nls->ct.p, V, g, x, f, nrm2, s);
 }
 
+}  // extern "C"
+
+namespace nosh {
+double residual_norm(nosh_nls_s* nls, const double* x_int,
+                     const std::map<std::string, double>& p, hipStream_t s) {
+  const int64_t n = nls->mesh->m->n;
+  if (nls->f.n != size_t(2 * n)) nls->f.alloc(2 * n);
+  if (nls->nrm2.n < 1) nls->nrm2.alloc(1);
+  nls_f_dev(nls, p, x_int, nls->f.p, nls->nrm2.p, s);
+  double nf2 = 0.0;
+  NOSH_HIP(hipMemcpyAsync(&nf2, nls->nrm2.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  NOSH_HIP(hipStreamSynchronize(s));
+  return std::sqrt(nf2);
+}
+}  // namespace nosh
+
+extern "C" {
+
 int nosh_nls_eval_f(nosh_nls_t nls, const double* x, int np, const char* const* names,
                     const double* values, double* f) {
   return guard([&] {

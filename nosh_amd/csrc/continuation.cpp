@@ -298,6 +298,8 @@ int nosh_continuation_run(nosh_nls_t nls, double* psi, int np, const char* const
 
 namespace {
 
+constexpr double kSurrogateMargin = 0.5;
+
 // X[i] = (x*cs, y*cs[, z*cs], Re psi, Im psi, h Re t, h Im t, mu) (internal order).
 __global__ void k_features(int64_t n, int dim, const int32_t* __restrict__ perm,
                            const double* __restrict__ coords, double cs,
@@ -507,13 +509,23 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int np, const c
       k_features<<<gn, 256, 0, s>>>(n, dim, m.perm.p, m.coords.p, cs,
                                     reinterpret_cast<const double2*>(w.psi_prev.p),
                                     reinterpret_cast<const double2*>(w.t.p), hs, p_cur, X.p);
-      bool ok = false;
+      bool ok = false, use_guess = false, probed = false;
       nosh_newton_stats st{};
       if (trained) {
         net.infer(X.p, n, Y.p, s);
         k_apply_guess<<<gn, 256, 0, s>>>(n, reinterpret_cast<const double2*>(w.psi_prev.p),
                                          reinterpret_cast<const double2*>(w.t.p), hs, Y.p,
                                          sigma, reinterpret_cast<double2*>(guess.p));
+        probed = true;
+        // Newton starts from the surrogate guess only when its residual is at
+        // most half the tangent guess's: a guess that is only slightly better
+        // saves no Newton step.
+        p[param] = target;
+        const double nf_g = residual_norm(nls, guess.p, p, s);
+        const double nf_t = residual_norm(nls, tanbuf.p, p, s);
+        use_guess = nf_g <= kSurrogateMargin * nf_t;
+      }
+      if (use_guess) {
         NOSH_HIP(hipMemcpyAsync(x, guess.p, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, s));
         newton_at(target, so->newton_budget, &st);
         r.newton_iters += st.newton_iters;
@@ -579,8 +591,7 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int np, const c
       }
       r.gibbs = gibbs_dev(nls, w, x, s);
       r.tan_error = diff_norm(2 * n, tanbuf.p, x, dpart, dscal, s);
-      r.pred_error = r.predictor_used == 3 ? diff_norm(2 * n, guess.p, x, dpart, dscal, s)
-                                           : r.tan_error;
+      r.pred_error = probed ? diff_norm(2 * n, guess.p, x, dpart, dscal, s) : r.tan_error;
       // Training pairs from the run's own solutions (whole tangent steps
       // only: sub-steps overwrite the tangent the features were built from),
       // kept in a ring of P steps.

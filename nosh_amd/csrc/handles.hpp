@@ -106,6 +106,9 @@ namespace nosh {
 void set_last_error(const std::string& msg);
 // Shared helpers of the C-ABI translation units (capi.cpp).
 double need(const std::map<std::string, double>& p, const char* key);
+// ||F(x)|| at the parameters p, x in internal order (uses nls->f, nls->nrm2).
+double residual_norm(nosh_nls_s* nls, const double* x_int,
+                     const std::map<std::string, double>& p, hipStream_t s);
 void compute_ct(MeshCache& m, nosh_sfield_s* t, DBuf<double>& ct,
                 const std::map<std::string, double>& p, hipStream_t s);
 void newton_run(nosh_nls_s* nls, double* psi_int, const std::map<std::string, double>& p,

@@ -120,6 +120,42 @@ def kuhn_cube(ncell: int, lo: float = -5.0, hi: float = 5.0, jitter: float = 0.0
     return coords, cells
 
 
+def disk(h: float, radius: float = 5.0, jitter: float = 0.2, seed: int = 3):
+    """Delaunay triangulation of a disk: jittered hexagonal lattice plus a boundary ring.
+
+    Lattice points (spacing h, rows h*sqrt(3)/2 apart) strictly inside radius
+    - h/2 are moved by U(-jitter*h, jitter*h) per coordinate from the
+    SplitMix64 stream of ``seed`` (two draws per point, in lattice order);
+    round(2 pi radius / h) points are placed evenly on the boundary circle;
+    scipy's Delaunay triangulation (Qhull) joins them. Vertex order: lattice
+    points row by row, then the boundary ring. h = 0.0213 gives about 200 k
+    vertices at radius 5.
+    """
+    from scipy.spatial import Delaunay
+    dy = h * np.sqrt(3.0) / 2.0
+    nrow = int(np.ceil(radius / dy)) + 1
+    pts = []
+    for r in range(-nrow, nrow + 1):
+        y = r * dy
+        x0 = 0.5 * h * (r & 1)
+        ncol = int(np.ceil(radius / h)) + 1
+        xs = x0 + h * np.arange(-ncol, ncol + 1)
+        keep = xs * xs + y * y < (radius - 0.5 * h) ** 2
+        pts.append(np.stack([xs[keep], np.full(keep.sum(), y)], axis=1))
+    lat = np.concatenate(pts)
+    u = splitmix64_uniform(seed, 2 * lat.shape[0]).reshape(-1, 2) * 2.0 - 1.0
+    lat = lat + jitter * h * u
+    nb = int(round(2.0 * np.pi * radius / h))
+    th = 2.0 * np.pi * np.arange(nb) / nb
+    ring = np.stack([radius * np.cos(th), radius * np.sin(th)], axis=1)
+    xy = np.concatenate([lat, ring])
+    tri = Delaunay(xy)
+    cells = np.ascontiguousarray(tri.simplices, dtype=np.int32)
+    coords = np.zeros((xy.shape[0], 3))
+    coords[:, :2] = xy
+    return coords, cells
+
+
 def magnetic_potential(coords: np.ndarray) -> np.ndarray:
     """The per-vertex tag A = 0.5*(-y, x, 0): constant curl B = e_z."""
     A = np.zeros_like(coords)

@@ -159,3 +159,35 @@ def test_jittered_control_volumes_sum_to_volume():
     # Per-tet circumcentric pyramids sum to the tet volume: checked through the
     # total, and positivity of the dual cells of this mildly jittered mesh.
     assert np.all(cv > 0)
+
+
+def test_disk_delaunay_mesh():
+    """The disk generator's triangulation (jittered hexagonal lattice + boundary
+    ring, Delaunay): a topological disk (V - E + F = 1) covering the inscribed
+    polygon; its edge coefficients satisfy the moment identity; Delaunay makes
+    every interior edge weight alpha = sum of (cot of opposite angle)/2 >= 0; the
+    control volumes sum to the polygon's area; numbering matches brute force."""
+    coords, cells = mg.disk(0.5)
+    n, T = coords.shape[0], cells.shape[0]
+    E = mg.count_edges(cells)
+    assert n - E + T == 1
+    m = O.Mesh(coords, cells)
+    e, ce = brute_edges(cells)
+    np.testing.assert_array_equal(m.edges(), e)
+    x = coords[cells][:, :, :2]
+    u0, w0 = x[:, 1] - x[:, 0], x[:, 2] - x[:, 0]
+    area = 0.5 * np.abs(u0[:, 0] * w0[:, 1] - u0[:, 1] * w0[:, 0])
+    nb = int(round(2 * np.pi * 5.0 / 0.5))
+    poly = 0.5 * nb * 25.0 * np.sin(2 * np.pi / nb)
+    assert abs(area.sum() - poly) < 1e-10
+    coef = m.cell_edge_coefficients()
+    vec = (coords[e[:, 0]] - coords[e[:, 1]])[:, :2]
+    M = np.einsum("tk,tki,tkj->tij", coef, vec[ce], vec[ce])
+    np.testing.assert_allclose(M, area[:, None, None] * np.eye(2), rtol=0, atol=1e-12)
+    alpha = np.zeros(E)
+    np.add.at(alpha, ce.ravel(), coef.ravel())
+    boundary = np.zeros(E, bool)
+    cnt = np.bincount(ce.ravel(), minlength=E)
+    boundary[cnt == 1] = True
+    assert np.all(alpha[~boundary] >= -1e-12)
+    assert abs(m.control_volumes().sum() - poly) < 1e-10
