@@ -425,7 +425,8 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int nparams,
  * mu_0 + span in `steps` equal steps; every step solves all branches with
  * the lockstep batched Newton (FP32 inner MINRES, FP64 outer) from a secant
  * predictor or, once trained, from the MLP surrogate guess (sopts != NULL),
- * with per-branch fallback to the secant guess and then to psi_k. psi
+ * with per-branch fallback to the secant guess, to psi_k, and to 2, 4, 8
+ * sub-steps of the branches that still fail. psi
  * (host, the instances' reference-order vectors concatenated) holds the
  * start states and receives the last accepted states. record has
  * steps + 1 entries (entry 0: the solve at mu_0). Statuses as for
@@ -439,8 +440,10 @@ typedef struct {
   int fallbacks;        /* branches re-solved after the surrogate guess */
   double max_norm_f;    /* largest final ||F|| over the branches */
   double seconds;
-  double pred_error;    /* mean over branches of ||guess - psi|| (the guess used first) */
+  double pred_error;    /* mean over branches of ||surrogate guess - psi|| where the
+                           surrogate was evaluated, else equal to sec_error */
   double sec_error;     /* mean over branches of ||secant guess - psi|| */
+  int halvings;         /* step splits needed by the branches no guess reached */
 } nosh_batch_continuation_step;
 
 int nosh_batch_continuation_run(nosh_batch_t batch, double* psi, double span, int steps,

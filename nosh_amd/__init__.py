@@ -779,7 +779,7 @@ class BatchContinuationStep(C.Structure):
     _fields_ = [("param", C.c_double), ("newton_iters", C.c_int), ("minres_iters", C.c_int),
                 ("converged", C.c_int), ("seeded", C.c_int), ("fallbacks", C.c_int),
                 ("max_norm_f", C.c_double), ("seconds", C.c_double), ("pred_error", C.c_double),
-                ("sec_error", C.c_double)]
+                ("sec_error", C.c_double), ("halvings", C.c_int)]
 
 
 def _load_mlp():

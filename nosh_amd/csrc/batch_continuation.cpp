@@ -56,6 +56,7 @@ int bcguard(F&& f) {
 }
 
 constexpr int kShortNewton = 20;  // Newton steps of the first try from each guess
+constexpr int kMaxHalvings = 3;   // step splits (2, 4, 8 sub-steps) for failed branches
 // A surrogate guess is used only when its residual is at most this fraction
 // of the secant guess's: inexact Newton (Eisenstat-Walker forcing) takes the
 // same number of steps, and often more MINRES iterations, from a guess that
@@ -131,6 +132,28 @@ __global__ void k_bc_take(int64_t n, int bp, const int* __restrict__ take,
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n * bp) return;
   if (take[i & (bp - 1)]) dst[i] = src[i];
+}
+
+// out[row][b] = a + r (a - c) for the lanes with take[b] != 0.
+__global__ void k_bc_lin_take(int64_t n, int bp, const int* __restrict__ take, double r,
+                              const double2* __restrict__ a, const double2* __restrict__ c,
+                              double2* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n * bp) return;
+  if (!take[i & (bp - 1)]) return;
+  const double2 x = a[i], y = c[i];
+  out[i] = make_double2(x.x + r * (x.x - y.x), x.y + r * (x.y - y.y));
+}
+
+// For the lanes with take[b] != 0: psi <- psi + (psi - before), before <- old psi.
+__global__ void k_bc_advance_take(int64_t n, int bp, const int* __restrict__ take,
+                                  double2* __restrict__ psi, double2* __restrict__ before) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n * bp) return;
+  if (!take[i & (bp - 1)]) return;
+  const double2 x = psi[i], y = before[i];
+  before[i] = x;
+  psi[i] = make_double2(2.0 * x.x - y.x, 2.0 * x.y - y.y);
 }
 
 // Per-lane sums over rows, deterministic (fixed grid, fixed order):
@@ -474,6 +497,62 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
         for (int i = 0; i < ni; ++i)
           if (!skip[i]) ok[i] = st[i].converged;
       }
+      // Branches still not converged (a transition none of the guesses
+      // reaches): the step is split into 2, 4, ... sub-steps for them alone
+      // (every instance has its own mu), from psi_k with a secant guess
+      // scaled to the sub-step, then the secant over the sub-steps.
+      for (int halv = 1; halv <= kMaxHalvings; ++halv) {
+        std::vector<int> sf(ni);
+        int nf = 0;
+        for (int i = 0; i < ni; ++i) nf += (sf[i] = !ok[i]);
+        if (nf == 0) break;
+        r.halvings = halv;
+        const int sub = 1 << halv;
+        const double hs = h / sub;
+        std::vector<int> skp(ni), full(ni, o.max_iters);
+        for (int i = 0; i < ni; ++i) skp[i] = !sf[i];
+        bool alive = true;
+        for (int q = 0; q < sub && alive; ++q) {
+          for (int i = 0; i < ni; ++i)
+            if (sf[i]) mu[i] = bt->mu_h[i] + (k - 1) * h + (q + 1) * hs;
+          set_mu();
+          for (size_t g = 0; g < ng; ++g) {
+            GroupHost& G = B.ghost[g];
+            std::vector<int>& t = gc[g].take_h;
+            t.assign(G.dev.bp, 0);
+            for (int b = 0; b < G.B; ++b) t[b] = sf[G.dev.inst0 + b];
+            gc[g].take.upload(t.data(), t.size(), s);
+            const int64_t m = G.mesh->n * G.dev.bp;
+            if (q == 0) {
+              // psi = psi_k + (hs / hprev)(psi_k - psi_{k-1}); guess buffer := psi_k.
+              k_bc_lin_take<<<grid_for(m, 256), 256, 0, s>>>(
+                  G.mesh->n, G.dev.bp, gc[g].take.p, hprev > 0.0 ? hs / hprev : 0.0,
+                  gc[g].cur.p, gc[g].prev.p, G.psi.p);
+              k_bc_take<<<grid_for(m, 256), 256, 0, s>>>(G.mesh->n, G.dev.bp, gc[g].take.p,
+                                                         gc[g].cur.p, gc[g].guess.p);
+            } else {
+              // psi <- psi + (psi - psi_before); guess buffer := the old psi.
+              k_bc_advance_take<<<grid_for(m, 256), 256, 0, s>>>(G.mesh->n, G.dev.bp,
+                                                                 gc[g].take.p, G.psi.p,
+                                                                 gc[g].guess.p);
+            }
+          }
+          const int it = newton_batch(bt, o, st.data(), s, skp.data(), full.data());
+          int mx = 0;
+          for (int i = 0; i < ni; ++i)
+            if (sf[i]) mx = std::max(mx, st[i].newton_iters);
+          r.newton_iters += mx;
+          r.minres_iters += it;
+          alive = true;
+          for (int i = 0; i < ni; ++i)
+            if (sf[i] && !st[i].converged) alive = false;
+        }
+        if (alive)
+          for (int i = 0; i < ni; ++i)
+            if (sf[i]) ok[i] = 1;
+        for (int i = 0; i < ni; ++i) mu[i] = bt->mu_h[i] + k * h;
+        set_mu();
+      }
       for (int i = 0; i < ni; ++i) {
         r.converged += ok[i];
         r.max_norm_f = std::max(r.max_norm_f, st[i].final_norm_f);
@@ -499,7 +578,7 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
       // Training pairs from whole secant steps, kept in a ring of P steps.
       // Steps whose Newton needed more than the budget (branch transitions)
       // are not stored: their targets dwarf the smooth ones.
-      if (so && hprev > 0.0 && (!trained || so->retrain_steps > 0) &&
+      if (so && hprev > 0.0 && (!trained || so->retrain_steps > 0) && r.halvings == 0 &&
           r.newton_iters <= so->newton_budget) {
         const int slot = pairs % P;
         for (size_t q = 0; q < ng; ++q) {

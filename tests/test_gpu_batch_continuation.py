@@ -90,3 +90,25 @@ def test_batch_continuation_surrogate_seeds_and_gains():
           "mean pred/secant error ratio", ratio)
     assert ratio < 0.9
     _oracle_check(c, t, MU0 + span, batch.split(psi))
+
+
+def test_batch_continuation_halves_steps_for_failing_branches():
+    """A large step (0 -> 0.1 in mu) with a 3-step Newton limit: the full step
+    fails from every guess, the driver splits it for the failing branches
+    (8 sub-steps here), and every branch ends on a solution of F = 0
+    (oracle)."""
+    import nosh_amd as nosh
+    c, t, model, _, psi0 = _setup()
+    mu0 = np.zeros(len(G))  # psi_0 = 1/sqrt(g) is exact at mu = 0
+    batch = nosh.Batch([(model, len(G))], mu0, G)
+    kw = dict(tol=1e-10, max_iters=3, minres_rtol=1e-12, minres_max_iters=4000, forcing=0)
+    span, steps = 0.1, 1
+    psi, rec, _ = batch.continuation(psi0, span, steps, **kw)
+    print([(r["newton_iters"], r["halvings"], r["converged"]) for r in rec])
+    assert len(rec) == steps + 1 and all(r["converged"] == len(G) for r in rec)
+    assert max(r["halvings"] for r in rec) >= 1
+    from oracle import oracle as O
+    om = O.Model(O.Mesh(c, t), mg.magnetic_potential(c))
+    for b, p in enumerate(batch.split(psi)):
+        om.set_params(G[b], span)
+        assert np.linalg.norm(om.f(p)) <= 1e-9, b
