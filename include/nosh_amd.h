@@ -452,6 +452,108 @@ int nosh_batch_continuation_run(nosh_batch_t batch, double* psi, double span, in
                                 nosh_batch_continuation_step* record, int* steps_done,
                                 nosh_surrogate_stats* stats);
 
+/* ------------------------------------------------ stability eigenvalues --
+ * Replaces the eigensolve of executables/nosh-eig/nosh-eig.cpp:212-333
+ * (Anasazi LOBPCG on the Hermitian Jacobian W = J(psi), preconditioned by
+ * W_prec when "prec" is set, block size 2, 10 eigenvalues) and LOCA's
+ * "Compute Eigenvalues" of examples/conf.xml:50-66: LOBPCG on the GPU with
+ * the keo_regularized Chebyshev preconditioner, the Rayleigh-Ritz problems
+ * solved on the host. J is real-symmetric on R^{2n}; at a solution it has
+ * the phase mode i psi in its null space (J(i psi) = i F(psi)), which
+ * deflate_gauge keeps out of the search space, so the reported eigenvalues
+ * are those that decide stability (a negative one: unstable). */
+typedef struct {
+  int nev;            /* eigenpairs wanted (1..16), default 4 */
+  int block_size;     /* LOBPCG block, >= nev (default nev + 2, at most 16) */
+  int max_iters;      /* default 500 */
+  double tol;         /* converged when ||J x - theta x|| <= tol * scale, scale the largest
+                         |Ritz value| seen (an estimate of ||J||); default 1e-8 */
+  int which;          /* 0 smallest algebraic (default), 1 largest algebraic */
+  int deflate_gauge;  /* 1 (default): search orthogonal to i psi */
+  int use_prec;       /* 1 (default): Chebyshev keo_regularized preconditioner */
+  int cheb_iters;     /* default 6 */
+  double cheb_ratio;  /* default 300 */
+  uint64_t seed;      /* SplitMix64 seed of the start block */
+} nosh_eigen_options;
+
+typedef struct {
+  int iters;          /* LOBPCG iterations */
+  int converged;      /* eigenpairs converged (of nev) */
+  int applies;        /* J applies */
+  double max_residual;/* largest ||J x - theta x|| / scale over the nev pairs */
+  double scale;       /* the ||J|| estimate used by the test */
+  double seconds;
+} nosh_eigen_stats;
+
+int nosh_eigen_default_options(nosh_eigen_options* opts);
+/* psi: 2n host values (reference order). evals: nev values, ascending for
+ * which = 0, descending for which = 1. evecs (nullable): nev x 2n host
+ * values, reference order, unit 2-norm. Returns NOSH_ERR_NOT_CONVERGED
+ * when fewer than nev pairs met tol in max_iters (evals then hold the
+ * current Ritz values). */
+int nosh_nls_jacobian_eigenpairs(nosh_nls_t nls, const double* psi, int nparams,
+                                 const char* const* names, const double* values,
+                                 const nosh_eigen_options* opts, double* evals, double* evecs,
+                                 nosh_eigen_stats* stats);
+
+/* ------------------------------------------ arc-length continuation -----
+ * Pseudo-arclength continuation, replacing LOCA's "Arc Length" stepper with
+ * the tangent predictor and the adaptive step the reference is driven by
+ * (examples/conf.xml:35-75; nosh-cont.cpp:226-290). Unknowns (psi, p) with
+ * the arclength metric theta^2 <a, b> + alpha beta; theta^2 is fixed at the
+ * first step so that psi and p contribute equally to the first tangent.
+ * Tangent: J z = -dF/dp, (psi', p') = (z, 1) / norm, oriented along the
+ * previous tangent (first: sign(p') = direction). Corrector: Newton on
+ * F(psi, p) = 0, theta^2 <psi', psi - psi_0> + p' (p - p_0) - ds = 0 by
+ * bordering (two MINRES solves per step: J a = -F, J b = -dF/dp). Step size
+ * (LOCA's adaptive rule): after a step that took k Newton steps,
+ * ds <- min(max_step, ds (1 + a ((N - k) / (N - 1))^2)), N = newton.max_iters,
+ * a = aggressiveness; a failed corrector halves ds (below min_step: stop).
+ * A step whose predictor leaves [min_value, max_value] is replaced by a
+ * natural-parameter step onto the bound, which ends the run. */
+typedef struct {
+  const char* parameter;  /* "mu" (default) or "g" */
+  double start;           /* the solve at p = start comes first */
+  double min_value, max_value;
+  int max_steps;
+  int direction;          /* +1 (default): p increases first, -1: decreases */
+  double initial_step, min_step, max_step, aggressiveness;
+  int predictor;          /* 1 tangent (default), 2 secant */
+  double border_rtol;     /* MINRES rtol of the tangent and dF/dp solves, default 1e-10 */
+  int eigenvalues;        /* > 0: that many (<= 8) stability eigenvalues per accepted
+                             step, deflated, smallest first */
+  nosh_newton_options newton;
+} nosh_arclength_options;
+
+typedef struct {
+  double param;           /* parameter of the accepted point */
+  double step;            /* arclength step ds that was accepted */
+  double param_dot;       /* parameter component p' of the unit tangent at this point (the
+                             secant through it with predictor 2; for a step onto a bound,
+                             the tangent its guess used) */
+  int newton_iters;       /* corrector steps (all tries) */
+  int minres_iters;       /* MINRES iterations (tangent + bordering, all tries) */
+  int failures;           /* step halvings before acceptance */
+  int converged;
+  double norm_f;
+  double gibbs;
+  double seconds;
+  int num_eigs;           /* eigenvalues stored below */
+  int num_negative;       /* eigenvalues < 0 among them (unstable directions) */
+  double eigs[8];
+} nosh_arclength_step;
+
+int nosh_arclength_default_options(nosh_arclength_options* opts);
+/* psi: 2n host values, the start state in, the last accepted state out.
+ * record: max_steps + 1 entries (entry 0 = the solve at start). states
+ * (nullable): (max_steps + 1) x 2n host values receiving every accepted
+ * state (reference order), the per-step solution output that
+ * continuation_data_saver writes. Returns NOSH_ERR_NOT_CONVERGED when the
+ * step size fell below min_step. */
+int nosh_arclength_run(nosh_nls_t nls, double* psi, int nparams, const char* const* names,
+                       const double* values, const nosh_arclength_options* opts,
+                       nosh_arclength_step* record, double* states, int* steps_done);
+
 #ifdef __cplusplus
 }
 #endif

@@ -916,3 +916,127 @@ def continuation_surrogate(model: Nls, psi0, params: Dict[str, float], parameter
         _check(rc)
     out = [rec[i].as_dict() for i in range(done.value)]
     return psi, out, {k: getattr(st, k) for k, _ in SurrogateStats._fields_}
+
+
+# ---------------------------------------------------------------- stability eigenvalues
+class EigenOptions(C.Structure):
+    _fields_ = [("nev", C.c_int), ("block_size", C.c_int), ("max_iters", C.c_int),
+                ("tol", C.c_double), ("which", C.c_int), ("deflate_gauge", C.c_int),
+                ("use_prec", C.c_int), ("cheb_iters", C.c_int), ("cheb_ratio", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class EigenStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("converged", C.c_int), ("applies", C.c_int),
+                ("max_residual", C.c_double), ("scale", C.c_double), ("seconds", C.c_double)]
+
+
+class ArclengthOptions(C.Structure):
+    _fields_ = [("parameter", C.c_char_p), ("start", C.c_double), ("min_value", C.c_double),
+                ("max_value", C.c_double), ("max_steps", C.c_int), ("direction", C.c_int),
+                ("initial_step", C.c_double), ("min_step", C.c_double),
+                ("max_step", C.c_double), ("aggressiveness", C.c_double),
+                ("predictor", C.c_int), ("border_rtol", C.c_double), ("eigenvalues", C.c_int),
+                ("newton", NewtonOptions)]
+
+
+class ArclengthStep(C.Structure):
+    _fields_ = [("param", C.c_double), ("step", C.c_double), ("param_dot", C.c_double),
+                ("newton_iters", C.c_int), ("minres_iters", C.c_int), ("failures", C.c_int),
+                ("converged", C.c_int), ("norm_f", C.c_double), ("gibbs", C.c_double),
+                ("seconds", C.c_double), ("num_eigs", C.c_int), ("num_negative", C.c_int),
+                ("eigs", C.c_double * 8)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "eigs"}
+        d["eigs"] = [self.eigs[i] for i in range(self.num_eigs)]
+        return d
+
+
+def _load_eigen():
+    lib = _load()
+    if getattr(lib, "_nosh_eig", False):
+        return lib
+    vp, i32 = C.c_void_p, C.c_int
+    cpp = C.POINTER(C.c_char_p)
+    sig = {
+        "nosh_eigen_default_options": (i32, [C.POINTER(EigenOptions)]),
+        "nosh_nls_jacobian_eigenpairs": (i32, [vp, vp, i32, cpp, vp, C.POINTER(EigenOptions),
+                                               vp, vp, C.POINTER(EigenStats)]),
+        "nosh_arclength_default_options": (i32, [C.POINTER(ArclengthOptions)]),
+        "nosh_arclength_run": (i32, [vp, vp, i32, cpp, vp, C.POINTER(ArclengthOptions),
+                                     C.POINTER(ArclengthStep), vp, C.POINTER(i32)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    lib._nosh_eig = True
+    return lib
+
+
+def jacobian_eigenpairs(model: Nls, psi, params: Dict[str, float], nev: int = 4,
+                        vectors: bool = False, **kw):
+    """Extreme eigenpairs of J(psi) (nosh-eig): returns (evals, evecs or None, stats).
+
+    Options as nosh_eigen_options (block_size, max_iters, tol, which,
+    deflate_gauge, use_prec, cheb_iters, cheb_ratio, seed). Raises
+    NoshError when fewer than nev pairs converge.
+    """
+    lib = _load_eigen()
+    o = EigenOptions()
+    _check(lib.nosh_eigen_default_options(C.byref(o)))
+    o.nev = nev
+    o.block_size = kw.pop("block_size", min(16, nev + 2))
+    for k, v in kw.items():
+        if k not in dict(EigenOptions._fields_):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    m = 2 * model.mesh.num_vertices
+    x = _vec(psi, m)
+    ev = np.zeros(nev)
+    vec = np.zeros((nev, m)) if vectors else None
+    st = EigenStats()
+    p = _Params(params)
+    _check(lib.nosh_nls_jacobian_eigenpairs(model._h, _ptr(x), *p.args, C.byref(o), _ptr(ev),
+                                            _ptr(vec) if vectors else None, C.byref(st)))
+    return ev, vec, {k: getattr(st, k) for k, _ in EigenStats._fields_}
+
+
+def arclength(model: Nls, psi0, params: Dict[str, float], parameter: str = "mu",
+              start: float = 0.0, max_steps: int = 20, states: bool = False,
+              newton: Optional[NewtonOptions] = None, newton_kw: Optional[dict] = None, **kw):
+    """Pseudo-arclength continuation (LOCA's "Arc Length" stepper).
+
+    Returns (final psi, step records[, states (steps_done x 2n)]). Keyword
+    options as nosh_arclength_options (min_value, max_value, direction,
+    initial_step, min_step, max_step, aggressiveness, predictor,
+    border_rtol, eigenvalues). A run that ends because the step size fell
+    below min_step returns the accepted records plus the failed one.
+    """
+    lib = _load_eigen()
+    o = ArclengthOptions()
+    _check(lib.nosh_arclength_default_options(C.byref(o)))
+    pname = parameter.encode()
+    o.parameter, o.start, o.max_steps = pname, start, max_steps
+    for k, v in kw.items():
+        if k not in dict(ArclengthOptions._fields_):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    o.newton = newton if newton is not None else default_newton_options(**(newton_kw or {}))
+    m = 2 * model.mesh.num_vertices
+    psi = np.array(_vec(psi0, m), copy=True)
+    rec = (ArclengthStep * (max_steps + 1))()
+    done = C.c_int()
+    st = np.zeros((max_steps + 1, m)) if states else None
+    p = _Params(params)
+    rc = lib.nosh_arclength_run(model._h, _ptr(psi), *p.args, C.byref(o), rec,
+                                _ptr(st) if states else None, C.byref(done))
+    if rc not in (_OK, _NOTCONV):
+        _check(rc)
+    out = [rec[i].as_dict() for i in range(done.value)]
+    if rc == _NOTCONV and done.value <= max_steps:
+        out.append(rec[done.value].as_dict())
+    if states:
+        return psi, out, st[:done.value]
+    return psi, out

@@ -56,6 +56,37 @@ def test_struct_layouts_match_header(tmp_path):
     assert b == ctypes.sizeof(nosh.NewtonStats)
 
 
+def test_every_struct_mirror_matches_header(tmp_path):
+    """Each ctypes mirror has the size and field offsets of its C struct."""
+    pairs = [("nosh_newton_options", nosh.NewtonOptions), ("nosh_newton_stats", nosh.NewtonStats),
+             ("nosh_batch_instance_stats", nosh.BatchInstanceStats),
+             ("nosh_continuation_options", nosh.ContinuationOptions),
+             ("nosh_continuation_step", nosh.ContinuationStep),
+             ("nosh_surrogate_options", nosh.SurrogateOptions),
+             ("nosh_surrogate_stats", nosh.SurrogateStats),
+             ("nosh_batch_continuation_step", nosh.BatchContinuationStep),
+             ("nosh_eigen_options", nosh.EigenOptions), ("nosh_eigen_stats", nosh.EigenStats),
+             ("nosh_arclength_options", nosh.ArclengthOptions),
+             ("nosh_arclength_step", nosh.ArclengthStep)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "nosh_amd.h"', 'int main(void){']
+    for cname, cls in pairs:
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    vals = iter(map(int, subprocess.run([str(exe)], capture_output=True, text=True,
+                                        check=True).stdout.split()))
+    for cname, cls in pairs:
+        assert next(vals) == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert next(vals) == getattr(cls, f).offset, (cname, f)
+
+
 def test_params_marshalling_sorted():
     p = nosh._Params({"mu": 0.5, "g": 2.0})
     assert p.count == 2
