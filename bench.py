@@ -91,16 +91,19 @@ def parse():
     p.add_argument("--cont-tuning", default="",
                    help="block_threads,fold for the batched continuation (skips the autotuner)")
     p.add_argument("--no-surrogate", action="store_true")
-    p.add_argument("--workload", default="cont96", choices=["cont96", "batched", "cont", "disk"],
+    p.add_argument("--workload", default="cont96",
+                   choices=["cont96", "batched", "cont", "disk", "stability"],
                    help="cont96: the headline (continuation, side-by-side solves, roofline, "
                         "CPU baseline); cont: the continuation part only; batched: 128 uneven "
-                        "random-start instances on jittered 16^3/24^3/32^3/48^3 cubes")
+                        "random-start instances on jittered 16^3/24^3/32^3/48^3 cubes; disk: "
+                        "surrogate continuation on the 200 k-vertex disk; stability: "
+                        "eigenvalues of J and arclength steps on the cube")
     p.add_argument("--instances", type=int, default=128)
     p.add_argument("--plain-sample", type=int, default=8,
                    help="batched: instances solved one by one with plain FP64 for the ratio")
     a = p.parse_args()
     if a.steps is None:
-        a.steps = {"batched": 1, "disk": 200}.get(a.workload, 50)
+        a.steps = {"batched": 1, "disk": 200, "stability": 12}.get(a.workload, 50)
     if a.warmup is None:
         a.warmup = 1 if a.workload == "batched" else 2
     return a
@@ -564,6 +567,66 @@ def cont_workload(args):
     print(json.dumps(continuation_phase(args, model, mesh.num_vertices)))
 
 
+def stability_workload(args):
+    """Stability eigenvalues and arc-length continuation on the Kuhn cube
+    (default 96^3, g = 1): the Meissner state at mu = 0.2 by natural
+    continuation; the 4 smallest eigenvalues of J with the phase mode
+    deflated (LOBPCG, Chebyshev preconditioner); then pseudo-arclength steps
+    from mu = 0.2 with those eigenvalues at every accepted point, against the
+    natural-parameter continuation's time per step (FP64 Newton-MINRES, with
+    Eisenstat-Walker forcing for the natural steps, where it is faster; the
+    arclength corrector solves exactly, which is faster and keeps it on the
+    branch at folds)."""
+    import torch
+    import nosh_amd as nosh
+    torch.cuda.init()
+    coords, cells, A, mesh, model, t_mesh = build_model(args.ncell)
+    n = mesh.num_vertices
+    say(f"stability workload: mesh {args.ncell}^3, n={n}")
+    kw_arc = dict(tol=1e-10, max_iters=20, minres_rtol=1e-10, minres_max_iters=args.minres_maxit,
+                  cheb_iters=args.cheb, cheb_ratio=args.ratio)
+    kw = dict(kw_arc, forcing=1, forcing_min=1e-6, forcing_max=0.1)
+    psi0 = np.zeros(2 * n)
+    psi0[0::2] = 1.0
+    t0 = time.perf_counter()
+    psi, rec_n = nosh.continuation(model, psi0, {"g": 1.0}, "mu", 0.0, 0.2, 10, predictor=1, **kw)
+    t_nat = time.perf_counter() - t0
+    assert all(r["converged"] for r in rec_n)
+    say(f"natural continuation 0 -> 0.2 in 10 steps: {t_nat:.2f}s")
+    eig = {}
+    for prec in (1, 0):
+        t0 = time.perf_counter()
+        ev, _, st = nosh.jacobian_eigenpairs(model, psi, {"g": 1.0, "mu": 0.2}, nev=4, tol=1e-8,
+                                             use_prec=prec, cheb_iters=args.cheb,
+                                             cheb_ratio=args.ratio, max_iters=2000)
+        st["wall"] = time.perf_counter() - t0
+        eig["prec" if prec else "noprec"] = {"evals": ev.tolist(), **st}
+        say(f"eigenpairs (prec={prec}): {ev}, {st}")
+    t0 = time.perf_counter()
+    _, rec_a = nosh.arclength(model, psi, {"g": 1.0, "mu": 0.2}, "mu", start=0.2,
+                              max_steps=args.steps, initial_step=0.02, max_step=0.05,
+                              eigenvalues=4, newton_kw=kw_arc)
+    t_arc = time.perf_counter() - t0
+    say(f"arclength {len(rec_a) - 1} steps: {t_arc:.2f}s")
+    t0 = time.perf_counter()
+    _, rec_b = nosh.arclength(model, psi, {"g": 1.0, "mu": 0.2}, "mu", start=0.2,
+                              max_steps=args.steps, initial_step=0.02, max_step=0.05,
+                              eigenvalues=0, newton_kw=kw_arc)
+    t_arc0 = time.perf_counter() - t0
+    out = {"workload": f"Kuhn-tet cube {args.ncell}^3, g = 1: stability eigenvalues at the "
+                       f"Meissner state mu = 0.2 and {args.steps} arclength steps from there",
+           "n_vertices": n,
+           "natural_continuation": {"steps": 10, "seconds": t_nat,
+                                    "newton": sum(r["newton_iters"] for r in rec_n[1:]),
+                                    "minres": sum(r["minres_iters"] for r in rec_n[1:])},
+           "eigenpairs": eig,
+           "arclength": {"steps": len(rec_a) - 1, "seconds_with_eigs": t_arc,
+                         "seconds_without_eigs": t_arc0,
+                         "records": [{k: (round(v, 6) if isinstance(v, float) else v)
+                                      for k, v in r.items()} for r in rec_a]}}
+    print(json.dumps(out))
+
+
 def batched_workload(args):
     """128 instances drawn round-robin over jittered Kuhn cubes 16^3, 24^3, 32^3,
     48^3 (interior vertices moved by U(-0.2h, 0.2h), seed 2); mu ~ U(0,1),
@@ -674,6 +737,8 @@ def main():
         cont_workload(args)
     elif args.workload == "disk":
         disk_workload(args)
+    elif args.workload == "stability":
+        stability_workload(args)
     else:
         headline(args)
 

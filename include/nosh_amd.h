@@ -505,7 +505,9 @@ int nosh_nls_jacobian_eigenpairs(nosh_nls_t nls, const double* psi, int nparams,
  * Tangent: J z = -dF/dp, (psi', p') = (z, 1) / norm, oriented along the
  * previous tangent (first: sign(p') = direction). Corrector: Newton on
  * F(psi, p) = 0, theta^2 <psi', psi - psi_0> + p' (p - p_0) - ds = 0 by
- * bordering (two MINRES solves per step: J a = -F, J b = -dF/dp). Step size
+ * bordering (two MINRES solves per Newton step: J a = -F to the Newton
+ * forcing term, J b = -dF/dp to border_rtol, since near a fold an inexact b
+ * lets the corrector leave the branch). Step size
  * (LOCA's adaptive rule): after a step that took k Newton steps,
  * ds <- min(max_step, ds (1 + a ((N - k) / (N - 1))^2)), N = newton.max_iters,
  * a = aggressiveness; a failed corrector halves ds (below min_step: stop).

@@ -344,6 +344,9 @@ int nosh_arclength_run(nosh_nls_t nls, double* psi, int np, const char* const* n
             // nls->f holds F(x, lam) (residual_norm).
             A.rebuild(x, lam);
             r.minres_iters += A.solve(nls->f.p, -1.0, eta, A.a.p);
+            // b to border_rtol whatever the forcing: near a fold (J nearly
+            // singular) an inexact b lets the corrector converge to another
+            // intersection of the arclength hyperplane with the branch.
             r.minres_iters += A.solve(A.fl.p, -1.0, o.border_rtol, A.b.p);
             const double* cols[3] = {A.a.p, A.b.p, x};
             double d[3];

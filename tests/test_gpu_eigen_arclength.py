@@ -174,3 +174,44 @@ def test_arclength_adaptive_steps_and_bound():
     for pred in (1, 2):
         d = np.linalg.norm(phase_align(finals[pred], ref) - ref) / np.linalg.norm(ref)
         assert d < 1e-9, (pred, d)
+
+
+def test_arclength_run_written_by_the_observers(tmp_path):
+    """An arclength run with eigenvalues, written as the reference's drivers
+    write it: the observer CSV (Gibbs energy and scaled norm from the library
+    agree with their formulas over the oracle's control volumes), the eigen
+    data CSV and one state file per step (read back bit-exact)."""
+    import nosh_amd as nosh
+    from nosh_amd import io as nio
+    from oracle import oracle as O
+    c, t, model, om = _setup(12)
+    n = c.shape[0]
+    psi0 = np.zeros(2 * n)
+    psi0[0::2] = 1.0
+    psi, rec, states = nosh.arclength(model, psi0, {"g": 1.0, "mu": 0.0}, "mu", start=0.0,
+                                      max_steps=5, states=True, initial_step=0.05,
+                                      max_step=0.05, eigenvalues=3, newton_kw=NEWTON)
+    assert len(rec) == 6 and all(r["converged"] for r in rec)
+    obs = nio.ContinuationObserver(str(tmp_path / "cont.dat"), "mu", model=model)
+    eig = nio.EigenDataSaver(str(tmp_path / "eig.dat"))
+    saver = nio.ContinuationDataSaver(c, t, str(tmp_path), "mu")
+    for r, x in zip(rec, states):
+        obs.observe(x, r["param"])
+        eig.save(r["eigs"])
+        saver.save_solution(x, r["param"])
+    obs.close()
+    eig.close()
+    cv = O.Mesh(c, t).control_volumes()
+    labels, rows = nio.read_csv(str(tmp_path / "cont.dat"))
+    assert labels[1] == "(1) mu" and rows.shape == (6, 4)
+    for k, (r, x) in enumerate(zip(rec, states)):
+        a2 = x[0::2] ** 2 + x[1::2] ** 2
+        assert rows[k, 0] == k and rows[k, 1] == pytest.approx(r["param"], rel=1e-14)
+        assert rows[k, 2] == pytest.approx(-np.sum(cv * a2 * a2) / cv.sum(), rel=1e-12)
+        assert rows[k, 2] == pytest.approx(r["gibbs"], rel=1e-12)
+        assert rows[k, 3] == pytest.approx(np.sqrt(np.sum(cv * a2) / cv.sum()), rel=1e-12)
+        p, fd = nio.read_state(str(tmp_path / f"out{k:04d}.vtu"))
+        assert np.array_equal(p, x) and fd["mu"] == r["param"]
+    labels, erows = nio.read_csv(str(tmp_path / "eig.dat"))
+    assert erows.shape == (6, 7)
+    assert np.allclose(erows[:, 4:], [r["eigs"] for r in rec], rtol=1e-14)
