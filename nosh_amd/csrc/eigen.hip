@@ -24,6 +24,7 @@
 #include <numeric>
 
 #include "eigen.hpp"
+#include "minres_device.hpp"
 
 namespace nosh {
 
@@ -40,12 +41,20 @@ constexpr int kDotThreads = 256;
 constexpr int kCombThreads = 256;
 
 // Partial sums of A_i . B_j over one row range for one 4 x 4 tile of (i, j).
-// grid.x: row blocks, grid.y: tiles (ti * ntj + tj).
+// One 1-D grid of nblk * ntiles workgroups; the logical index (XCD-remapped,
+// so that consecutive logical indices run on one XCD) is rb * ntiles + tile:
+// the tiles of one row range run side by side and re-read its columns from
+// that XCD's L2 rather than from HBM.
 __global__ __launch_bounds__(kDotThreads) void k_block_dots(ColsC A, int ma, ColsC B, int mb,
                                                             int64_t L2, int64_t per_block,
+                                                            int nblk,
                                                             double* __restrict__ partials) {
   const int ntj = (mb + 3) >> 2;
-  const int ti = blockIdx.y / ntj, tj = blockIdx.y % ntj;
+  const int ntiles = ((ma + 3) >> 2) * ntj;
+  const int64_t logical = xcd_block(true);
+  const int tile = int(logical % ntiles);
+  const int64_t rb = logical / ntiles;
+  const int ti = tile / ntj, tj = tile % ntj;
   const int na = min(4, ma - 4 * ti), nb = min(4, mb - 4 * tj);
   const double2* a[4];
   const double2* b[4];
@@ -57,7 +66,7 @@ __global__ __launch_bounds__(kDotThreads) void k_block_dots(ColsC A, int ma, Col
   double acc[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) acc[q] = 0.0;
-  const int64_t r0 = int64_t(blockIdx.x) * per_block;
+  const int64_t r0 = rb * per_block;
   const int64_t r1 = min(L2, r0 + per_block);
   for (int64_t r = r0 + threadIdx.x; r < r1; r += kDotThreads) {
     double2 av[4], bv[4];
@@ -71,12 +80,11 @@ __global__ __launch_bounds__(kDotThreads) void k_block_dots(ColsC A, int ma, Col
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[4 * i + j] += av[i].x * bv[j].x + av[i].y * bv[j].y;
   }
-  const int tile = blockIdx.y;
   __shared__ double lds[16 * 16];
   block_sum<16>(acc, lds);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) partials[(int64_t(tile) * 16 + q) * gridDim.x + blockIdx.x] = acc[q];
+    for (int q = 0; q < 16; ++q) partials[(int64_t(tile) * 16 + q) * nblk + rb] = acc[q];
   }
 }
 
@@ -92,11 +100,20 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_block_dots_final(
   if (threadIdx.x == 0 && i < ma && j < mb) out[i * mb + j] = v[0];
 }
 
+// Coefficients passed by value (kernel arguments) when m k fits.
+constexpr int kArgCoefs = 256;
+struct CoefArg {
+  double c[kArgCoefs];
+};
+
 // Y_j = beta Y_j + sum_c C[c k + j] S_c, two rows (one double2) per thread.
-__global__ __launch_bounds__(kCombThreads) void k_block_combine(ColsC S, int m,
-                                                                const double* __restrict__ C,
+// C is the by-value CA.c when BYVAL, else the device array Cd.
+template <bool BYVAL>
+__global__ __launch_bounds__(kCombThreads) void k_block_combine(ColsC S, int m, CoefArg CA,
+                                                                const double* __restrict__ Cd,
                                                                 ColsW Y, int k, double beta,
                                                                 int64_t L2) {
+  const double* C = BYVAL ? CA.c : Cd;
   const int64_t r = int64_t(blockIdx.x) * kCombThreads + threadIdx.x;
   if (r >= L2) return;
   double2 acc[16];
@@ -124,6 +141,154 @@ __global__ __launch_bounds__(kCombThreads) void k_block_combine(ColsC S, int m,
         o.y += beta * yo.y;
       }
       *y = o;
+    }
+  }
+}
+
+// ---- Multi-column operators: J or P's off-diagonal part read once for up to
+// kBV columns (each slot's column index and value shared by all of them).
+constexpr int kBV = 8;
+constexpr int kSB = 256;
+
+struct BlkOp {
+  int64_t n;
+  const int32_t* slice_ptr;
+  const int32_t* scol;
+  const double2* val;   // off-diagonal values of K (shared by J and P)
+  const double* blk;    // 3 n per-vertex blocks (J or P), diagonal of K included
+  const double* bound;  // Chebyshev bound from the rebuild
+  double upper, ratio;
+  int iters;
+};
+
+// Off-diagonal products of one row for nv <= NV columns.
+template <int NV>
+__device__ __forceinline__ void blk_row(const BlkOp& o, int64_t row, int nv, const ColsC& X,
+                                        double2 (&acc)[NV]) {
+  const int32_t sl = uniform(int32_t(row >> 6));
+  const int32_t b = uniform(o.slice_ptr[sl]);
+  const int32_t e = uniform(o.slice_ptr[sl + 1]);
+  const int lane = int(row & 63);
+#pragma unroll
+  for (int q = 0; q < NV; ++q) acc[q] = make_double2(0.0, 0.0);
+  for (int32_t j = b + lane; j < e; j += kSliceH) {
+    const int32_t c = __builtin_nontemporal_load(o.scol + j);
+    const double2 v = o.val[j];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      if (q < nv) {
+        const double2 xv = reinterpret_cast<const double2*>(X.p[q])[c];
+        acc[q].x += v.x * xv.x - v.y * xv.y;
+        acc[q].y += v.x * xv.y + v.y * xv.x;
+      }
+    }
+  }
+}
+
+// Y_q = Koff X_q + B X_q.
+template <int NV>
+__global__ __launch_bounds__(kSB) void k_blk_apply(BlkOp o, int nv, ColsC X, ColsW Y) {
+  const int64_t row = xcd_block(true) * kSB + threadIdx.x;
+  if (row >= o.n) return;
+  double2 acc[NV];
+  blk_row<NV>(o, row, nv, X, acc);
+  const double b0 = o.blk[row], b1 = o.blk[o.n + row], b2 = o.blk[2 * o.n + row];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    if (q < nv) {
+      const double2 xi = reinterpret_cast<const double2*>(X.p[q])[row];
+      reinterpret_cast<double2*>(Y.p[q])[row] =
+          make_double2(acc[q].x + b0 * xi.x + b2 * xi.y, acc[q].y + b2 * xi.x + b1 * xi.y);
+    }
+  }
+}
+
+// Chebyshev step 0 on P: Z_q = D^{-1} R_q / theta (and the direction D_q).
+template <int NV>
+__global__ __launch_bounds__(kSB) void k_blk_cheb_first(BlkOp o, int nv, ColsC R, ColsW Z,
+                                                        ColsW Dd) {
+  const int64_t row = int64_t(blockIdx.x) * kSB + threadIdx.x;
+  if (row >= o.n) return;
+  const double hi = cheb_hi(o.bound, o.upper);
+  const double th = 0.5 * (hi + hi / o.ratio);
+  const double b0 = o.blk[row], b1 = o.blk[o.n + row], b2 = o.blk[2 * o.n + row];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    if (q < nv) {
+      const double2 rv = reinterpret_cast<const double2*>(R.p[q])[row];
+      double z0, z1;
+      dinv<double>(b0, b1, b2, rv.x, rv.y, &z0, &z1);
+      const double2 z = make_double2(z0 / th, z1 / th);
+      reinterpret_cast<double2*>(Z.p[q])[row] = z;
+      if (o.iters > 1) reinterpret_cast<double2*>(Dd.p[q])[row] = z;
+    }
+  }
+}
+
+// Chebyshev step j >= 1: Zout = Zin + d, d = c1 d + c2 D^{-1}(R - P Zin).
+template <int NV>
+__global__ __launch_bounds__(kSB) void k_blk_cheb_step(BlkOp o, int nv, int j, ColsC R, ColsC Zin,
+                                                       ColsW Zout, ColsW Dd) {
+  const int64_t row = xcd_block(true) * kSB + threadIdx.x;
+  if (row >= o.n) return;
+  const Cheb c = cheb_coef(cheb_hi(o.bound, o.upper), o.ratio, j);
+  const bool last = j == o.iters - 1;
+  double2 acc[NV];
+  blk_row<NV>(o, row, nv, Zin, acc);
+  const double b0 = o.blk[row], b1 = o.blk[o.n + row], b2 = o.blk[2 * o.n + row];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    if (q < nv) {
+      const double2 zi = reinterpret_cast<const double2*>(Zin.p[q])[row];
+      const double2 rv = reinterpret_cast<const double2*>(R.p[q])[row];
+      const double res0 = rv.x - (acc[q].x + b0 * zi.x + b2 * zi.y);
+      const double res1 = rv.y - (acc[q].y + b2 * zi.x + b1 * zi.y);
+      double u0, u1;
+      dinv<double>(b0, b1, b2, res0, res1, &u0, &u1);
+      double2* dp = reinterpret_cast<double2*>(Dd.p[q]) + row;
+      double2 d = *dp;
+      d.x = c.c1 * d.x + c.c2 * u0;
+      d.y = c.c1 * d.y + c.c2 * u1;
+      if (!last) *dp = d;
+      reinterpret_cast<double2*>(Zout.p[q])[row] = make_double2(zi.x + d.x, zi.y + d.y);
+    }
+  }
+}
+
+template <class C, class V>
+C cols_of(const V& v, size_t from, int cnt) {
+  C c{};
+  for (int q = 0; q < cnt; ++q) c.p[q] = v[from + q];
+  return c;
+}
+
+// Y = J X (or P X) for any number of columns, kBV at a time.
+void blk_apply(const BlkOp& o, const std::vector<double*>& X, const std::vector<double*>& Y,
+               hipStream_t s) {
+  for (size_t f = 0; f < X.size(); f += kBV) {
+    const int nv = int(std::min<size_t>(kBV, X.size() - f));
+    k_blk_apply<kBV><<<grid_for(o.n, kSB), kSB, 0, s>>>(o, nv, cols_of<ColsC>(X, f, nv),
+                                                        cols_of<ColsW>(Y, f, nv));
+  }
+}
+
+// Z = Chebyshev(P) R for every column; T and Dd are scratch columns.
+void blk_cheb(const BlkOp& o, const std::vector<double*>& R, const std::vector<double*>& Z,
+              const std::vector<double*>& T, const std::vector<double*>& Dd, hipStream_t s) {
+  const unsigned nb = grid_for(o.n, kSB);
+  const int steps = o.iters < 1 ? 1 : o.iters;
+  for (size_t f = 0; f < R.size(); f += kBV) {
+    const int nv = int(std::min<size_t>(kBV, R.size() - f));
+    const ColsC r = cols_of<ColsC>(R, f, nv);
+    const ColsW d = cols_of<ColsW>(Dd, f, nv);
+    const ColsW zb[2] = {cols_of<ColsW>(Z, f, nv), cols_of<ColsW>(T, f, nv)};
+    int cur = (steps - 1) % 2;
+    k_blk_cheb_first<kBV><<<nb, kSB, 0, s>>>(o, nv, r, zb[cur], d);
+    for (int j = 1; j < steps; ++j) {
+      ColsC zin{};
+      for (int q = 0; q < nv; ++q) zin.p[q] = zb[cur].p[q];
+      k_blk_cheb_step<kBV><<<nb, kSB, 0, s>>>(o, nv, j, r, zin, zb[1 - cur], d);
+      cur = 1 - cur;
     }
   }
 }
@@ -195,8 +360,8 @@ void block_dots(const double* const* A, int ma, const double* const* B, int mb, 
   ColsC a{}, b{};
   for (int i = 0; i < ma; ++i) a.p[i] = A[i];
   for (int j = 0; j < mb; ++j) b.p[j] = B[j];
-  k_block_dots<<<dim3(unsigned(nblk), unsigned(ntiles)), kDotThreads, 0, s>>>(a, ma, b, mb, L2,
-                                                                             per, w.partials.p);
+  k_block_dots<<<unsigned(nblk * ntiles), kDotThreads, 0, s>>>(a, ma, b, mb, L2, per, int(nblk),
+                                                               w.partials.p);
   k_block_dots_final<<<unsigned(ntiles * 16), kFinalizeThreads, 0, s>>>(w.partials.p, int(nblk),
                                                                        ma, mb, w.out.p);
   NOSH_HIP(hipMemcpyAsync(out, w.out.p, sizeof(double) * ma * mb, hipMemcpyDeviceToHost, s));
@@ -210,17 +375,24 @@ void block_combine(const double* const* S, int m, const double* C, double* const
   for (int j = 0; j < k; ++j)
     for (int i = 0; i < m; ++i)
       NOSH_CHECK(Y[j] != S[i], kInvalid, "block_combine: output aliases an input");
-  if (w.coef.n < size_t(kMaxCols * 16)) w.coef.alloc(kMaxCols * 16);
-  // Synchronous upload: the host array may change right after this call.
-  NOSH_HIP(hipStreamSynchronize(s));
-  NOSH_HIP(hipMemcpy(w.coef.p, C, sizeof(double) * m * k, hipMemcpyHostToDevice));
   ColsC sc{};
   ColsW yc{};
   for (int i = 0; i < m; ++i) sc.p[i] = S[i];
   for (int j = 0; j < k; ++j) yc.p[j] = Y[j];
   const int64_t L2 = L / 2;
-  k_block_combine<<<grid_for(L2, kCombThreads), kCombThreads, 0, s>>>(sc, m, w.coef.p, yc, k,
-                                                                      beta, L2);
+  const unsigned grid = grid_for(L2, kCombThreads);
+  if (m * k <= kArgCoefs) {
+    CoefArg ca;
+    std::memcpy(ca.c, C, sizeof(double) * m * k);
+    k_block_combine<true><<<grid, kCombThreads, 0, s>>>(sc, m, ca, nullptr, yc, k, beta, L2);
+    return;
+  }
+  if (w.coef.n < size_t(kMaxCols * 16)) w.coef.alloc(kMaxCols * 16);
+  // Synchronous upload: the host array may change right after this call.
+  NOSH_HIP(hipStreamSynchronize(s));
+  NOSH_HIP(hipMemcpy(w.coef.p, C, sizeof(double) * m * k, hipMemcpyHostToDevice));
+  k_block_combine<false><<<grid, kCombThreads, 0, s>>>(sc, m, CoefArg{}, w.coef.p, yc, k, beta,
+                                                       L2);
 }
 
 void sym_eig(int n, std::vector<double> A, std::vector<double>& w, std::vector<double>& V) {
@@ -380,15 +552,14 @@ int lobpcg_jacobian(nosh_nls_s* nls, const double* psi_int, const std::map<std::
   if (nls->bound.n < 1) nls->bound.alloc(1);
   launch_rebuild_blocks(m, nls->keo->kv, nls->ct.p, V, g, psi_int, nls->jblk.p, nls->pblk.p,
                         nls->bound.p, s);
-  nls->w64.ensure(n);
-  const PrecView<double> P{nls->keo->kv.val.p, nls->pblk.p, nls->bound.p, 0.0, o.cheb_ratio,
-                           o.cheb_iters};
   int applies = 0;
+  BlkOp jop{n, m.slice_ptr.p, m.scol.p, nls->keo->kv.val.p, nls->jblk.p, nls->bound.p, 0.0,
+            o.cheb_ratio, o.cheb_iters};
+  BlkOp pop = jop;
+  pop.blk = nls->pblk.p;
   auto applyJ = [&](const std::vector<double*>& X, std::vector<double*>& Y) {
-    for (size_t j = 0; j < X.size(); ++j) {
-      launch_block_apply(m, nls->keo->kv, nls->jblk.p, X[j], Y[j], s);
-      ++applies;
-    }
+    blk_apply(jop, X, Y, s);
+    applies += int(X.size());
   };
   const bool prec = o.use_prec && o.cheb_iters > 0;
 
@@ -480,10 +651,10 @@ int lobpcg_jacobian(nosh_nls_s* nls, const double* psi_int, const std::map<std::
     fresh = false;
     // Preconditioned residuals.
     if (prec) {
-      std::vector<double*> Z = lb.pool.take(k);
-      for (int j = 0; j < k; ++j)
-        prec_apply<double>(m, P, nls->w64, reinterpret_cast<const double2*>(W[j]),
-                           reinterpret_cast<double2*>(Z[j]), s);
+      std::vector<double*> Z = lb.pool.take(k), T = lb.pool.take(k), Dd = lb.pool.take(k);
+      blk_cheb(pop, W, Z, T, Dd, s);
+      lb.pool.give(T);
+      lb.pool.give(Dd);
       lb.pool.give(W);
       W = Z;
     }
@@ -524,12 +695,29 @@ int lobpcg_jacobian(nosh_nls_s* nls, const double* psi_int, const std::map<std::
     }
     std::vector<double*> nX = lb.pool.take(k), nAX = lb.pool.take(k), nP = lb.pool.take(k),
                         nAP = lb.pool.take(k);
-    block_combine(S.data(), ms, Cx.data(), nX.data(), k, 0.0, L, lb.bw, s);
-    block_combine(AS.data(), ms, Cx.data(), nAX.data(), k, 0.0, L, lb.bw, s);
-    // New directions: the components of the new X along W and P.
-    block_combine(S.data() + k, ms - k, Cx.data() + size_t(k) * k, nP.data(), k, 0.0, L, lb.bw, s);
-    block_combine(AS.data() + k, ms - k, Cx.data() + size_t(k) * k, nAP.data(), k, 0.0, L, lb.bw,
-                  s);
+    // New X and new directions P (the components of the new X along W and
+    // P) in one pass over S, likewise AX and AP over AS.
+    if (2 * k <= 16) {
+      std::vector<double> C2(size_t(ms) * 2 * k, 0.0);
+      for (int a = 0; a < ms; ++a)
+        for (int j = 0; j < k; ++j) {
+          C2[size_t(a) * 2 * k + j] = Cx[size_t(a) * k + j];
+          if (a >= k) C2[size_t(a) * 2 * k + k + j] = Cx[size_t(a) * k + j];
+        }
+      std::vector<double*> Y(nX);
+      Y.insert(Y.end(), nP.begin(), nP.end());
+      block_combine(S.data(), ms, C2.data(), Y.data(), 2 * k, 0.0, L, lb.bw, s);
+      Y.assign(nAX.begin(), nAX.end());
+      Y.insert(Y.end(), nAP.begin(), nAP.end());
+      block_combine(AS.data(), ms, C2.data(), Y.data(), 2 * k, 0.0, L, lb.bw, s);
+    } else {
+      block_combine(S.data(), ms, Cx.data(), nX.data(), k, 0.0, L, lb.bw, s);
+      block_combine(AS.data(), ms, Cx.data(), nAX.data(), k, 0.0, L, lb.bw, s);
+      block_combine(S.data() + k, ms - k, Cx.data() + size_t(k) * k, nP.data(), k, 0.0, L,
+                    lb.bw, s);
+      block_combine(AS.data() + k, ms - k, Cx.data() + size_t(k) * k, nAP.data(), k, 0.0, L,
+                    lb.bw, s);
+    }
     lb.pool.give(X);
     lb.pool.give(AX);
     lb.pool.give(W);
