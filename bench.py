@@ -231,6 +231,20 @@ def continuation_phase(args, model, n):
     comb_rate = nb * steps / t_comb
     say(f"continuation: combined {t_comb:.2f}s for {nb}x{steps} solves -> {comb_rate:.3f}/s, "
         f"converged {conv}, surrogate {sst}")
+    # Roofline of the timed region's dominant kernel (the batched Chebyshev
+    # step), every branch active, at the final states, timed with HIP events
+    # on the library stream.
+    us_cs, bytes_cs = batch.time_cheb_step(50)
+    cheb_roof = {"bound": "hbm", "kernel": "kb_cheb_step (batched FP32 Chebyshev step, "
+                                           f"{nb} lanes, all active)",
+                 "achieved": round(bytes_cs / us_cs / 1e3, 1), "peak": 8000.0, "unit": "GB/s",
+                 "frac": round(bytes_cs / us_cs / 1e3 / 8000.0, 4),
+                 "us_per_launch": round(us_cs, 2), "bytes_per_launch": int(bytes_cs),
+                 "layout_note": "16 B per padded slot (packed column, alpha, a~; shared by the "
+                                "lanes) + 52 B per row and lane (z_in, r, D read+write, z_out, "
+                                "P block; FP32)"}
+    say(f"continuation: kb_cheb_step {us_cs:.1f} us, {bytes_cs / 1e6:.1f} MB -> "
+        f"{bytes_cs / us_cs / 1e3:.0f} GB/s")
     finals = batch.split(psi)
     # Error of the mixed-precision states against FP64: FP64 Newton from each
     # final state (same branch) and the distance to where it lands.
@@ -281,6 +295,7 @@ def continuation_phase(args, model, n):
                                            if seeded else None),
         "per_step_seconds": [round(r["seconds"], 4) for r in rec],
         "per_step_newton": [r["newton_iters"] for r in rec],
+        "dominant_kernel_roofline": cheb_roof,
         "plain_fp64_solves_per_s": plain_rate, "plain_sample": plain_rec,
         "speedup_vs_plain_fp64": comb_rate / plain_rate if plain_rate else None,
     }
@@ -500,6 +515,7 @@ def headline(args):
         "continuation": cont,
         "random_start_solves": solves,
         "roofline": roofline,
+        "roofline_timed_region": cont["dominant_kernel_roofline"],
         "surrogate_mfma": mfma,
         "mesh_build_s": round(t_mesh, 3),
         "cpu_baseline": cpu,

@@ -287,6 +287,15 @@ int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int fold, int m
 /* Times every (block_threads, fold, mode) candidate on one Newton step from
  * psi and keeps the fastest; with use_cache the choice is read from /
  * appended to $NOSH_AUTOTUNE_CACHE (default ~/.cache/nosh_amd_autotune.tsv). */
+/* Measurement (no counterpart in the reference): average duration of one
+ * batched Chebyshev step (the kernel the batched MINRES spends most time in),
+ * every instance active, rebuilt at the batch's current states, timed with
+ * HIP events on the library stream over reps launches; bytes_per_launch
+ * (nullable) receives its algorithmic bytes: 16 per padded slot (packed
+ * column, alpha, a~, shared by the lanes) + 52 per row and lane (z_in, r, D
+ * read and written, z_out, the P block; FP32). Resets the MINRES state. */
+int nosh_batch_time_cheb_step(nosh_batch_t batch, int reps, double* us_per_launch,
+                              double* bytes_per_launch);
 int nosh_batch_autotune(nosh_batch_t batch, const double* psi, const nosh_newton_options* opts,
                         int use_cache, int* best_threads, int* best_fold, int* best_mode);
 /* psi: all instances concatenated (each 2 n_k values, reference order);

@@ -663,6 +663,8 @@ def _load_batch():
         "nosh_batch_solve_device": (i32, [vp, vp, C.POINTER(NewtonOptions),
                                           C.POINTER(BatchInstanceStats), vp]),
         "nosh_batch_destroy": (i32, [vp]),
+        "nosh_batch_time_cheb_step": (i32, [vp, i32, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -698,6 +700,13 @@ class Batch(_Handle):
     @property
     def total_length(self) -> int:
         return sum(c * s for c, s in zip(self.counts, self.sizes))
+
+    def time_cheb_step(self, reps: int = 50):
+        """(us per launch, algorithmic bytes per launch) of the batched
+        Chebyshev step, all instances active, at the current states."""
+        us, by = C.c_double(), C.c_double()
+        _check(_load_batch().nosh_batch_time_cheb_step(self._h, reps, C.byref(us), C.byref(by)))
+        return us.value, by.value
 
     def split(self, psi):
         """Instance vectors of a concatenated state."""

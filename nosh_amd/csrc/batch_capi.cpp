@@ -339,6 +339,24 @@ int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int fold, int m
   });
 }
 
+int nosh_batch_time_cheb_step(nosh_batch_t batch, int reps, double* us_per_launch,
+                              double* bytes_per_launch) {
+  return bguard([&] {
+    NOSH_CHECK(batch && us_per_launch, kInvalid, "null argument");
+    hipStream_t s = default_stream();
+    *us_per_launch = batch->b.time_cheb_step(s, reps);
+    if (bytes_per_launch) {
+      // Per padded slot the packed {column, alpha, a~} (16 B, shared by the
+      // lanes); per row and lane z_in, r, D (read and written), z_out
+      // (8 B each, FP32 complex) and the three FP32 entries of the P block.
+      double b = 0.0;
+      for (auto& G : batch->b.ghost)
+        b += 16.0 * double(G.mesh->nnz_pad) + 52.0 * double(G.mesh->n) * G.dev.bp;
+      *bytes_per_launch = b;
+    }
+  });
+}
+
 int nosh_batch_solve_device(nosh_batch_t batch, double* psi, const nosh_newton_options* opts,
                             nosh_batch_instance_stats* stats, void* stream) {
   return bguard([&] {

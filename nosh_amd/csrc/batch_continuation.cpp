@@ -392,6 +392,11 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
       energies(e1);
       e0 = e1;
       copy_state(&GroupCont::cur, true);
+      // psi_{k-1} := psi_0: the first secant (ratio 0) then reads defined
+      // values (0 * (cur - prev) of stale memory could be 0 * NaN).
+      for (size_t q = 0; q < ng; ++q)
+        NOSH_HIP(hipMemcpyAsync(gc[q].prev.p, gc[q].cur.p, sizeof(double2) * gc[q].cur.n,
+                                hipMemcpyDeviceToDevice, s));
       done = 1;
     }
     const double h = span / steps;

@@ -244,6 +244,49 @@ __device__ __forceinline__ void brow_shfl4(const BGroup& G, int32_t j, int32_t k
   }
 }
 
+#ifndef NOSH_META_PIPE
+#define NOSH_META_PIPE 1
+#endif
+
+// brow_shfl4 over a whole row, software-pipelined: the metadata load of the
+// next 4 slots is issued right after the x gathers of the current 4, so the
+// two dependent memory latencies of consecutive batches overlap. Same
+// arithmetic in the same order as the unpipelined loop.
+__device__ __forceinline__ void brow_shfl_row(const BGroup& G, int32_t j, int32_t w, int b,
+                                              float mu, const float2* __restrict__ x, float& ar,
+                                              float& ai) {
+  const int lane = int(threadIdx.x & 63);
+  const int g0 = lane & ~(G.bp - 1);
+  const int bp = G.bp;
+  int4 mine = make_int4(0, 0, 0, 0);
+  if (b < 4 && b < w) mine = G.s32[j + b * kSliceH];
+  for (int32_t k = 0; k < w; k += 4, j += 4 * kSliceH) {
+    int32_t c[4];
+    float av[4], tv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      c[q] = __shfl(mine.x, g0 + q, 64);
+      av[q] = __int_as_float(__shfl(mine.y, g0 + q, 64));
+      tv[q] = __int_as_float(__shfl(mine.z, g0 + q, 64));
+    }
+    float2 xv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (k + q < w) xv[q] = x[int64_t(c[q]) * bp + b];
+    if (b < 4 && k + 4 + b < w) mine = G.s32[j + (4 + b) * kSliceH];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (k + q < w) {
+        float sn, cs;
+        sc(mu * tv[q], &sn, &cs);
+        const float vr = -av[q] * cs, vi = av[q] * sn;
+        ar += vr * xv[q].x - vi * xv[q].y;
+        ai += vr * xv[q].y + vi * xv[q].x;
+      }
+    }
+  }
+}
+
 template <class T>
 __device__ __forceinline__ typename Vt<T>::v2 brow(const BGroup& G, int64_t row, int b, T mu,
                                                     const typename Vt<T>::v2* __restrict__ x) {
@@ -255,7 +298,10 @@ __device__ __forceinline__ typename Vt<T>::v2 brow(const BGroup& G, int64_t row,
   int32_t k = 0;
   if constexpr (std::is_same<T, float>::value) {
     if (meta_shfl(G)) {
-      for (; k < w; k += 4, j += 4 * kSliceH) brow_shfl4(G, j, k, w, b, mu, x, ar, ai);
+      if (NOSH_META_PIPE)
+        brow_shfl_row(G, j, w, b, mu, x, ar, ai);
+      else
+        for (; k < w; k += 4, j += 4 * kSliceH) brow_shfl4(G, j, k, w, b, mu, x, ar, ai);
       typename Vt<T>::v2 r;
       r.x = ar;
       r.y = ai;
@@ -745,6 +791,38 @@ int Batch::minres(hipStream_t s, int maxit) {
     if (active_h[i]) total = std::max(total, Sh[i].it);
   (void)issued;
   return total;
+}
+
+double Batch::time_cheb_step(hipStream_t s, int reps) {
+  NOSH_CHECK(args.iters >= 2 && reps >= 1, kInvalid,
+             "timing a Chebyshev step needs degree >= 2 and reps >= 1");
+  const unsigned nb = unsigned(args.total_blocks / args.fold), bt = unsigned(block_threads);
+  std::vector<int> all(args.ninst, 1);
+  NOSH_HIP(hipMemcpyAsync(active.p, all.data(), sizeof(int) * args.ninst, hipMemcpyHostToDevice,
+                          s));
+  kb_reset<<<grid_for(args.ninst, 128), 128, 0, s>>>(args, 1);
+  kb_rebuild<<<nb, bt, 0, s>>>(args);
+  kb_setup<<<nb, bt, 0, s>>>(args);
+  kb_cheb_first<<<nb, bt, 0, s>>>(args, 1);
+  kb_cheb_step<<<nb, bt, 0, s>>>(args, 1, 1);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  NOSH_HIP(hipEventCreate(&e0));
+  NOSH_HIP(hipEventCreate(&e1));
+  NOSH_HIP(hipEventRecord(e0, s));
+  for (int q = 0; q < reps; ++q) kb_cheb_step<<<nb, bt, 0, s>>>(args, 1, 1);
+  NOSH_HIP(hipEventRecord(e1, s));
+  NOSH_HIP(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  NOSH_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // The instances are idle again.
+  std::vector<int> none(args.ninst, 0);
+  NOSH_HIP(hipMemcpyAsync(active.p, none.data(), sizeof(int) * args.ninst, hipMemcpyHostToDevice,
+                          s));
+  kb_reset<<<grid_for(args.ninst, 128), 128, 0, s>>>(args, 1);
+  NOSH_HIP(hipStreamSynchronize(s));
+  return 1e3 * double(ms) / reps;
 }
 
 void Batch::compute_norms(hipStream_t s, std::vector<double>& nrm) {

@@ -97,6 +97,10 @@ struct Batch {
   // applies psi += dx; returns the largest MINRES iteration count.
   int minres(hipStream_t s, int maxit);
   void compute_norms(hipStream_t s, std::vector<double>& nrm);
+  // Average duration (us) of one Chebyshev step launch (kb_cheb_step, j = 1)
+  // with every instance active, timed with events on s over reps launches
+  // after rebuilding at the current psi. Leaves the MINRES state reset.
+  double time_cheb_step(hipStream_t s, int reps);
   ~Batch();
 };
 

@@ -98,6 +98,14 @@ def test_batch_continuation_halves_steps_for_failing_branches():
     (8 sub-steps here), and every branch ends on a solution of F = 0
     (oracle)."""
     import nosh_amd as nosh
+    import torch
+    # Device memory freed with NaN in it: the driver's buffers must not read
+    # anything they did not write (the first secant once computed
+    # psi_k + 0 * (psi_k - stale) = NaN in reused memory).
+    junk = torch.full((64 << 20,), float("nan"), dtype=torch.float64, device="cuda")
+    del junk
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     c, t, model, _, psi0 = _setup()
     mu0 = np.zeros(len(G))  # psi_0 = 1/sqrt(g) is exact at mu = 0
     batch = nosh.Batch([(model, len(G))], mu0, G)

@@ -140,3 +140,24 @@ def test_full_group_matches_padded_group():
     for a_, b_ in zip(b8.split(p8)[:7], b7.split(p7)):
         assert np.abs(phase_align(a_, b_) - b_).max() <= 1e-9
     _check_with_oracle([(c, t)], [8], mu, g, b8.split(p8))
+
+
+def test_cheb_step_timing_leaves_the_batch_usable(setup):
+    """The measurement entry point times the batched Chebyshev step with every
+    instance active, reports its algorithmic bytes (16 per padded slot + 52
+    per row and lane), and a solve after it still converges (oracle)."""
+    import nosh_amd as nosh
+    meshes, counts, models, mu, g, psi0 = setup
+    batch = nosh.Batch(list(zip(models, counts)), mu, g)
+    opts = dict(tol=1e-10, max_iters=80, minres_rtol=1e-6, cheb_iters=4, cheb_ratio=100.0,
+                forcing=0)
+    batch.solve(psi0, **opts)
+    us, by = batch.time_cheb_step(20)
+    lanes = [1 << max(0, (c - 1).bit_length()) for c in counts]
+    expect = sum(16 * m.mesh.layout()["nnz_pad"] + 52 * m.mesh.num_vertices * bp
+                 for m, bp in zip(models, lanes))
+    print("kb_cheb_step", us, "us", by, "bytes")
+    assert us > 0 and by == expect
+    psi, st = batch.solve(psi0, **opts)
+    assert all(s["converged"] for s in st)
+    _check_with_oracle(meshes, counts, mu, g, batch.split(psi))
