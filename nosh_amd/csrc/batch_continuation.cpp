@@ -55,7 +55,9 @@ int bcguard(F&& f) {
   }
 }
 
-constexpr int kShortNewton = 20;  // Newton steps of the first try from each guess
+// Newton steps of the first try from each guess (12 measured slower at 96^3:
+// more branches reach the full-length tries).
+constexpr int kShortNewton = 20;
 constexpr int kMaxHalvings = 3;   // step splits (2, 4, 8 sub-steps) for failed branches
 // A surrogate guess is used only when its residual is at most this fraction
 // of the secant guess's: inexact Newton (Eisenstat-Walker forcing) takes the
@@ -437,6 +439,16 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
         copy_state(&GroupCont::sec, false);
         batch_residuals(bt, nf_sec, s);
         int nsur = 0;
+        if (progress) {
+          double rmin = 1e300, rmax = 0.0;
+          for (int i = 0; i < ni; ++i) {
+            const double q = nf_sur[i] / std::max(nf_sec[i], 1e-300);
+            rmin = std::min(rmin, q);
+            rmax = std::max(rmax, q);
+          }
+          std::fprintf(stderr, "[batch continuation] step %d probe: ||F(sur)||/||F(sec)|| in "
+                               "[%.3g, %.3g], ||F(sec)|| %.3g\n", k, rmin, rmax, nf_sec[0]);
+        }
         for (int i = 0; i < ni; ++i) {
           use_sur[i] = nf_sur[i] < kSurrogateMargin * nf_sec[i];
           maxit[i] = use_sur[i] ? so->newton_budget : std::min(o.max_iters, kShortNewton);
@@ -623,8 +635,9 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
       if (progress)
         std::fprintf(stderr,
                      "[batch continuation] step %d/%d dmu %.4f newton %d minres %d seeded %d "
-                     "%.3fs\n",
-                     k, steps, r.param, r.newton_iters, r.minres_iters, r.seeded, r.seconds);
+                     "%.3fs err(sur) %.3g err(sec) %.3g\n",
+                     k, steps, r.param, r.newton_iters, r.minres_iters, r.seeded, r.seconds,
+                     r.pred_error, r.sec_error);
     }
   });
   if (bt && psi && done > 0) {
