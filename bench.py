@@ -143,15 +143,19 @@ def cpu_baseline(coords, cells, A, mu, psi0, phi, counts, cheb, ratio, what):
             ts.append(time.perf_counter() - s)
         return float(np.median(ts))
 
-    t_f = med(lambda: om.f(psi0))
-    t_k = med(lambda: om.keo(phi))
+    t_f = med(lambda: om.f(psi0), 20)
+    t_k = med(lambda: om.keo(phi), 20)
 
     def rebuild():
         om.jac_rebuild(psi0)
         om.prec_rebuild(psi0)
 
     t_rb = med(rebuild, 3)
-    t_j = med(lambda: om.jac(phi))
+    t_j = med(lambda: om.jac(phi), 20)
+    threads = O.set_threads(0)
+    O.set_threads(1)
+    t_j1 = med(lambda: om.jac(phi), 20)
+    O.set_threads(threads)
     b = -om.f(psi0)
     k_sample = 20
     s = time.perf_counter()
@@ -160,13 +164,61 @@ def cpu_baseline(coords, cells, A, mu, psi0, phi, counts, cheb, ratio, what):
     steps = counts["newton_iters"]
     its = counts["minres_iters"]
     t_solve = (steps + 1) * t_f + steps * t_rb + its * t_it
-    sample = (f"oracle on {threads} threads, 96^3 cube: F {t_f*1e3:.1f} ms, K.phi "
-              f"{t_k*1e3:.1f} ms, J.phi {t_j*1e3:.1f} ms, rebuild {t_rb*1e3:.1f} ms, MINRES "
+    sample = (f"oracle on {threads} threads ({_cpu_model()}, nproc {os.cpu_count()}, "
+              f"OMP_NUM_THREADS {os.environ.get('OMP_NUM_THREADS', 'unset')}), 96^3 cube, "
+              f"medians of 20: F {t_f*1e3:.1f} ms, K.phi {t_k*1e3:.1f} ms, J.phi "
+              f"{t_j*1e3:.1f} ms (1 thread: {t_j1*1e3:.1f} ms), rebuild {t_rb*1e3:.1f} ms, MINRES "
               f"iteration {t_it*1e3:.1f} ms (mean of {k} with Chebyshev {cheb}); one solve = "
               f"{steps + 1:.2f} F + {steps:.2f} rebuilds + {its:.1f} MINRES iterations, the "
               f"{what}'s per-solve counts; setup {setup:.1f} s")
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": sample, "jphi_ms": t_j * 1e3, "seconds_per_solve": t_solve}
+            "sample": sample, "jphi_ms": t_j * 1e3, "jphi_ms_1thread": t_j1 * 1e3,
+            "f_ms": t_f * 1e3, "keo_ms": t_k * 1e3, "seconds_per_solve": t_solve,
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def config0_solves(args):
+    """The small plain case, 64x64 square of right triangles (mu = 0.1, g = 1,
+    seeded SplitMix64 psi_0): one full FP64 Newton-MINRES solve to
+    ||F|| <= 1e-10 on the GPU and with the oracle on the host, same settings."""
+    import nosh_amd as nosh
+    from nosh_amd import meshgen as mg
+    from oracle import oracle as O
+    c, t = mg.square_tri(64)
+    n = c.shape[0]
+    psi0 = mg.random_state(n, 0)
+    mesh = nosh.Mesh(c, t)
+    model = nosh.Nls(mesh, nosh.VectorFieldExplicitValues(mesh, mg.magnetic_potential(c), 0.0),
+                     nosh.ScalarFieldConstant(mesh, -1.0), 1.0,
+                     nosh.ScalarFieldConstant(mesh, 1.0))
+    kw = dict(tol=1e-10, max_iters=30, minres_rtol=1e-10, minres_max_iters=1000,
+              cheb_iters=args.cheb, cheb_ratio=args.ratio)
+    nosh.newton_solve(model, psi0, {"g": 1.0, "mu": 0.1}, **kw)  # warm-up
+    t0 = time.perf_counter()
+    pg, sg = nosh.newton_solve(model, psi0, {"g": 1.0, "mu": 0.1}, **kw)
+    t_gpu = time.perf_counter() - t0
+    om = O.Model(O.Mesh(c, t), mg.magnetic_potential(c))
+    t0 = time.perf_counter()
+    pc, it, nf, mh = om.newton(psi0, 1.0, 0.1, tol=1e-10, maxit=30, minres_rtol=1e-10,
+                               minres_maxit=1000, cheb_iters=args.cheb, ratio=args.ratio)
+    t_cpu = time.perf_counter() - t0
+    d = float(np.linalg.norm(_phase_align(pg, pc) - pc) / np.linalg.norm(pc))
+    return {"mesh": f"square 64x64 (n = {n})", "gpu_seconds": t_gpu, "cpu_seconds": t_cpu,
+            "cpu_threads": O.set_threads(0), "gpu_newton": sg["newton_iters"],
+            "gpu_minres": sg["minres_iters_total"], "gpu_final_norm_f": sg["final_norm_f"],
+            "cpu_newton": int(it), "cpu_minres": int(np.sum(mh)),
+            "cpu_final_norm_f": float(nf[-1]), "rel_diff_gpu_vs_cpu_solution": d}
 
 
 def _phase_align(psi, ref):
@@ -485,6 +537,10 @@ def headline(args):
                            {"newton_iters": ps["newton_iters"] / args.steps,
                             "minres_iters": ps["minres_iters"] / args.steps},
                            args.cheb, args.ratio, "plain FP64 continuation")
+    c0 = None if args.no_cpu else config0_solves(args)
+    if c0:
+        say(f"config 0 solve: GPU {c0['gpu_seconds']:.3f}s, CPU {c0['cpu_seconds']:.3f}s, "
+            f"rel. diff {c0['rel_diff_gpu_vs_cpu_solution']:.2e}")
     value = cont["solves_per_s"] if cont["converged"] else 0.0
     out = {
         "metric": METRIC,
@@ -519,6 +575,7 @@ def headline(args):
         "surrogate_mfma": mfma,
         "mesh_build_s": round(t_mesh, 3),
         "cpu_baseline": cpu,
+        "small_case_solve": c0,
     }
     print(json.dumps(out))
 

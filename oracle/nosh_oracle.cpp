@@ -41,6 +41,8 @@
 #include <unordered_map>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 struct V3 {
@@ -718,6 +720,13 @@ int minres(int64_t m, Op op, Prec prec, const double* b, double* x, double rtol,
 }  // namespace
 
 extern "C" {
+
+// OpenMP threads of the following calls (the CPU baseline times 1 and all);
+// returns the count now in effect.
+int orc_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
 
 // Chebyshev-Jacobi approximate inverse of keo_regularized: z = M^{-1} r.
 // hi <= 0 selects the automatic bound cheb_upper_bound(); lo = hi/ratio.

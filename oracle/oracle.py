@@ -55,6 +55,7 @@ def _load():
         "orc_jac_apply": (None, [vp, _dp, _dp]),
         "orc_prec_rebuild": (None, [vp, _dp]),
         "orc_prec_op_apply": (None, [vp, _dp, _dp]),
+        "orc_set_threads": (C.c_int, [C.c_int]),
         "orc_cheb_apply": (None, [vp, C.c_int, C.c_double, C.c_double, _dp, _dp]),
         "orc_cheb_bound": (C.c_double, [vp]),
         "orc_minres_jac": (C.c_int, [vp, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_double,
@@ -73,6 +74,11 @@ def _load():
 
 
 _lib = None
+
+
+def set_threads(n: int) -> int:
+    """OpenMP threads of the oracle's parallel loops (0: query); returns the count."""
+    return int(lib().orc_set_threads(int(n)))
 
 
 def lib():
