@@ -51,7 +51,7 @@ void alloc_group(GroupHost& G) {
   G.x.alloc(m);
   G.jblk.alloc(3 * m);
   G.pblk.alloc(3 * m);
-  for (auto* v : {&G.Y, &G.Z2, &G.v, &G.u, &G.R0, &G.R1, &G.D, &G.W0, &G.W1, &G.W2}) {
+  for (auto* v : {&G.Y, &G.Z2, &G.v, &G.v1, &G.u, &G.R0, &G.R1, &G.D, &G.W0, &G.W1, &G.W2}) {
     v->alloc(m);
     v->zero(default_stream());
   }
@@ -64,6 +64,7 @@ void alloc_group(GroupHost& G) {
   d.Y = G.Y.p;
   d.Z2 = G.Z2.p;
   d.v = G.v.p;
+  d.v1 = G.v1.p;
   d.u = G.u.p;
   d.R0 = G.R0.p;
   d.R1 = G.R1.p;

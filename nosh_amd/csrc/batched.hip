@@ -408,6 +408,7 @@ __device__ __forceinline__ float2* wbuf(const BGroup& G, int k) {
   return q == 0 ? G.W0 : (q == 1 ? G.W1 : G.W2);
 }
 __device__ __forceinline__ float2* zbuf(const BGroup& G, int k) { return (k & 1) ? G.Z2 : G.Y; }
+__device__ __forceinline__ float2* vbuf(const BGroup& G, int k) { return (k & 1) ? G.v1 : G.v; }
 
 // Per-thread context of a launched block. All fold row tiles of a block
 // belong to one group and a thread keeps its lane (instance) in all of them,
@@ -432,6 +433,35 @@ __device__ __forceinline__ Ctx make_ctx(const BatchArgs& a, int64_t v0) {
   return c;
 }
 
+// Pending update of iteration k: w_k = (v_k - oldeps w_{k-2} - delta w_{k-1}) / gamma,
+// x += phi w_k.
+struct WUpd {
+  float oe, de, dn;
+  double phi;
+};
+
+__device__ __forceinline__ void w_update(const BGroup& G, const WUpd& q, int k, int64_t idx,
+                                         float2 vk) {
+  float2* wk = wbuf(G, k);
+  const float2 w1 = ldn(wbuf(G, k - 2) + idx), w2 = ldn(wbuf(G, k - 1) + idx);
+  const float2 w = make_float2((vk.x - q.oe * w1.x - q.de * w2.x) * q.dn,
+                               (vk.y - q.oe * w1.y - q.de * w2.y) * q.dn);
+  stn(wk + idx, w);
+  double2 xv = ldn(G.x + idx);
+  xv.x += q.phi * double(w.x);
+  xv.y += q.phi * double(w.y);
+  stn(G.x + idx, xv);
+}
+
+__device__ __forceinline__ WUpd wupd_of(const MinresScalars& S) {
+  WUpd q;
+  q.oe = float(S.p_oldeps);
+  q.de = float(S.p_delta);
+  q.dn = float(S.p_denom);
+  q.phi = S.p_phi;
+  return q;
+}
+
 // Chebyshev step 0 over the block's rows. mode 1: r = R0 (MINRES start);
 // mode 2: r_new = u - (alfa/beta) r2. Returns the thread's <r, z> sum.
 __device__ double rows_cheb_first(const BatchArgs& a, const Ctx& c, int mode) {
@@ -442,6 +472,12 @@ __device__ double rows_cheb_first(const BatchArgs& a, const Ctx& c, int mode) {
   const double alfa = S.alfa, beta = S.beta;
   if (done) return s;
   const BGroup& G = *c.G;
+  // The previous iteration's pending update of w and x (Givens coefficients
+  // from its last reduction, v from its Lanczos step), kept out of the
+  // gathering kernel to spare its registers.
+  const bool pending = mode == 2 && S.pending;
+  const WUpd q = wupd_of(S);
+  const float2* vprev = vbuf(G, it - 1);
   float2* r = rbuf(G, it);
   const float2* r2 = rbuf(G, it + 1);
   const float cab = mode == 2 ? float(alfa / beta) : 0.0f;
@@ -453,6 +489,7 @@ __device__ double rows_cheb_first(const BatchArgs& a, const Ctx& c, int mode) {
     const int64_t row = c.row0 + int64_t(t) * c.rows_per_vb;
     if (row >= G.n) break;
     const int64_t idx = row * G.bp + c.b;
+    if (pending) w_update(G, q, it, idx, ldn(vprev + idx));
     float2 rv;
     if (mode == 2) {
       const float2 uu = ldn(G.u + idx), r2v = ldn(r2 + idx);
@@ -518,34 +555,6 @@ __device__ double rows_cheb_step(const BatchArgs& a, const Ctx& c, int mode, int
   return s;
 }
 
-// Pending update of iteration k: w_k = (v_k - oldeps w_{k-2} - delta w_{k-1}) / gamma,
-// x += phi w_k.
-struct WUpd {
-  float oe, de, dn;
-  double phi;
-};
-
-__device__ __forceinline__ void w_update(const BGroup& G, const WUpd& q, int k, int64_t idx,
-                                         float2 vk) {
-  float2* wk = wbuf(G, k);
-  const float2 w1 = ldn(wbuf(G, k - 2) + idx), w2 = ldn(wbuf(G, k - 1) + idx);
-  const float2 w = make_float2((vk.x - q.oe * w1.x - q.de * w2.x) * q.dn,
-                               (vk.y - q.oe * w1.y - q.de * w2.y) * q.dn);
-  stn(wk + idx, w);
-  double2 xv = ldn(G.x + idx);
-  xv.x += q.phi * double(w.x);
-  xv.y += q.phi * double(w.y);
-  stn(G.x + idx, xv);
-}
-
-__device__ __forceinline__ WUpd wupd_of(const MinresScalars& S) {
-  WUpd q;
-  q.oe = float(S.p_oldeps);
-  q.de = float(S.p_delta);
-  q.dn = float(S.p_denom);
-  q.phi = S.p_phi;
-  return q;
-}
 
 // v = y / beta, u = J v - (beta/beta_old) r1, the previous w/x update, over
 // the block's rows; returns the thread's <v, u> sum.
@@ -553,9 +562,8 @@ __device__ double rows_minres_a(const BatchArgs& a, const Ctx& c) {
   double s = 0.0;
   if (!c.lane_ok) return s;
   const MinresScalars& S = a.S[c.inst];
-  const int done = S.done, it = S.it, pending = S.pending;
+  const int done = S.done, it = S.it;
   const double beta = S.beta, oldb = S.oldb;
-  const WUpd q = wupd_of(S);
   const BGroup& G = *c.G;
   if (meta_shfl(G) ? __all(done) : done) return s;
   const float2* y = zbuf(G, 0);
@@ -568,7 +576,6 @@ __device__ double rows_minres_a(const BatchArgs& a, const Ctx& c) {
     const int64_t row = c.row0 + int64_t(t) * c.rows_per_vb;
     if (row >= G.n) break;
     const int64_t idx = row * G.bp + c.b;
-    if (pending && !done) w_update(G, q, it, idx, ldn(G.v + idx));
     const float2 acc = brow<float>(G, row, c.b, mu, y);
     if (done) continue;
     const float2 yi = y[idx];
@@ -582,7 +589,7 @@ __device__ double rows_minres_a(const BatchArgs& a, const Ctx& c) {
       uu.y -= cb * rv.y;
     }
     const float2 vn = make_float2(yi.x * ib, yi.y * ib);
-    stn(G.v + idx, vn);
+    stn(vbuf(G, it) + idx, vn);
     stn(G.u + idx, uu);
     s += double(vn.x) * double(uu.x) + double(vn.y) * double(uu.y);
   }
@@ -594,7 +601,7 @@ __device__ void ph_flush(const BatchArgs& a, int64_t vb) {
   if (!L.valid || !a.active[L.inst]) return;
   const BGroup& G = *L.G;
   const MinresScalars& S = a.S[L.inst];
-  if (S.pending) w_update(G, wupd_of(S), S.it, L.idx, G.v[L.idx]);
+  if (S.pending) w_update(G, wupd_of(S), S.it, L.idx, vbuf(G, S.it - 1)[L.idx]);
   const double2 d = G.x[L.idx];
   double2 p = G.psi[L.idx];
   p.x += d.x;
@@ -617,10 +624,9 @@ __device__ void ph_finalize(const BatchArgs& a, int inst, int what, double* out)
       out[inst] = v[0];
     else if (what == 1)
       init_update(S, v[0]);
-    else if (what == 2) {
-      S->alfa = v[0];
-      S->pending = 0;
-    } else
+    else if (what == 2)
+      S->alfa = v[0];  // the pending w/x update is applied by the next phase
+    else
       givens_update(S, v[0]);
   }
   __syncthreads();

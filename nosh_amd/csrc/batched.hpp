@@ -32,6 +32,7 @@ struct BGroup {
   float* jblk;           // 3 [n][bp]
   float* pblk;           // 3 [n][bp]
   float2 *Y, *Z2, *v, *u, *R0, *R1, *D, *W0, *W1, *W2;  // [n][bp] FP32 Krylov vectors
+  float2* v1;  // second Lanczos buffer: v of iteration k lives in (k odd ? v1 : v)
 };
 
 struct BatchArgs {
@@ -60,7 +61,7 @@ struct GroupHost {
   DBuf<double> a64, t64, diag, rowabs, ct, V;
   DBuf<double2> psi, f, x;
   DBuf<float> jblk, pblk;
-  DBuf<float2> Y, Z2, v, u, R0, R1, D, W0, W1, W2;
+  DBuf<float2> Y, Z2, v, v1, u, R0, R1, D, W0, W1, W2;
   BGroup dev{};
 };
 

@@ -215,3 +215,34 @@ def test_arclength_run_written_by_the_observers(tmp_path):
     labels, erows = nio.read_csv(str(tmp_path / "eig.dat"))
     assert erows.shape == (6, 7)
     assert np.allclose(erows[:, 4:], [r["eigs"] for r in rec], rtol=1e-14)
+
+
+def test_invalid_options_are_errors_not_crashes():
+    """Bad eigensolver / arclength options come back as errors (ValueError
+    for NOSH_ERR_INVALID) and leave the model usable; a bound crossed on the
+    first step ends the run on the bound."""
+    import nosh_amd as nosh
+    c, t, model, om = _setup(8)
+    n = c.shape[0]
+    psi0 = np.zeros(2 * n)
+    psi0[0::2] = 1.0
+    p = {"g": 1.0, "mu": 0.0}
+    for bad in (dict(nev=0), dict(nev=17), dict(nev=2, which=2), dict(nev=2, tol=0.0),
+                dict(nev=2, max_iters=0)):
+        with pytest.raises(ValueError):
+            nosh.jacobian_eigenpairs(model, psi0, p, **bad)
+    with pytest.raises(KeyError):
+        nosh.jacobian_eigenpairs(model, psi0, {"mu": 0.0}, nev=2)
+    for bad in (dict(max_steps=0), dict(initial_step=0.0), dict(min_value=1.0, max_value=0.0),
+                dict(predictor=3), dict(eigenvalues=9)):
+        with pytest.raises(ValueError):
+            nosh.arclength(model, psi0, p, "mu", start=0.0, **bad)
+    with pytest.raises(ValueError):
+        nosh.arclength(model, psi0, p, "V", start=0.0)
+    psi, rec = nosh.arclength(model, psi0, p, "mu", start=0.0, max_steps=5, initial_step=0.5,
+                              max_value=0.05, newton_kw=NEWTON)
+    assert len(rec) == 2 and rec[-1]["param"] == 0.05 and rec[-1]["converged"]
+    om.set_params(1.0, 0.05)
+    assert np.linalg.norm(om.f(psi)) <= 1e-9
+    ev, _, st = nosh.jacobian_eigenpairs(model, psi, {"g": 1.0, "mu": 0.05}, nev=2, tol=1e-9)
+    assert st["converged"] == 2 and ev[0] > 0
