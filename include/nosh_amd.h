@@ -434,8 +434,12 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int nparams,
  * mu_0 + span in `steps` equal steps; every step solves all branches with
  * the lockstep batched Newton (FP32 inner MINRES, FP64 outer) from a secant
  * predictor or, once trained, from the MLP surrogate guess (sopts != NULL),
- * with per-branch fallback to the secant guess, to psi_k, and to 2, 4, 8
- * sub-steps of the branches that still fail. psi
+ * with per-branch fallback to the secant guess and to psi_k (at most 20
+ * lockstep Newton steps each), then, one branch at a time, to the
+ * single-instance FP64 Newton-MINRES (inner solves to minres_rtol) from the
+ * secant guess, from psi_k, and to the FP64 tangent continuation over the
+ * step (up to 3 step halvings), and last to 2, 4, 8 batched sub-steps of the
+ * branches that still fail. psi
  * (host, the instances' reference-order vectors concatenated) holds the
  * start states and receives the last accepted states. record has
  * steps + 1 entries (entry 0: the solve at mu_0). Statuses as for
@@ -452,7 +456,8 @@ typedef struct {
   double pred_error;    /* mean over branches of ||surrogate guess - psi|| where the
                            surrogate was evaluated, else equal to sec_error */
   double sec_error;     /* mean over branches of ||secant guess - psi|| */
-  int halvings;         /* step splits needed by the branches no guess reached */
+  int halvings;         /* step halvings needed by the branches no guess reached (FP64
+                           fallback or batched sub-steps) */
 } nosh_batch_continuation_step;
 
 int nosh_batch_continuation_run(nosh_batch_t batch, double* psi, double span, int steps,

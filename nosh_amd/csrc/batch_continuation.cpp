@@ -15,7 +15,9 @@
 // branch whose Newton from the surrogate guess does not converge within the
 // budget, or whose Gibbs energy leaves its linear trend, is solved again
 // from psi_sec; at a branch transition, where Newton from the secant guess
-// may wander, psi_k is tried too (short tries of both, then full ones).
+// may wander, psi_k is tried too (short tries of both); a branch neither
+// brings to convergence is handed, alone, to the single-instance FP64
+// tangent continuation over the step, and last to batched sub-steps.
 //
 // This is the caller the reference delegates to LOCA (natural stepping,
 // executables/nosh-cont/nosh-cont.cpp:226-290,344 with examples/conf.xml:35-190);
@@ -58,6 +60,9 @@ int bcguard(F&& f) {
 // Newton steps of the first try from each guess (12 measured slower at 96^3:
 // more branches reach the full-length tries).
 constexpr int kShortNewton = 20;
+// Chebyshev degree of the single-instance FP64 fallback solves (the plain
+// solver's best).
+constexpr int kFallbackCheb = 6;
 constexpr int kMaxHalvings = 3;   // step splits (2, 4, 8 sub-steps) for failed branches
 // A surrogate guess is used only when its residual is at most this fraction
 // of the secant guess's: inexact Newton (Eisenstat-Walker forcing) takes the
@@ -145,6 +150,18 @@ __global__ void k_bc_lin_take(int64_t n, int bp, const int* __restrict__ take, d
   if (!take[i & (bp - 1)]) return;
   const double2 x = a[i], y = c[i];
   out[i] = make_double2(x.x + r * (x.x - y.x), x.y + r * (x.y - y.y));
+}
+
+// dst[row] = src[row * bp + b] (one lane of an interleaved state), and back.
+__global__ void k_bc_lane_get(int64_t n, int bp, int b, const double2* __restrict__ src,
+                              double2* __restrict__ dst) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i * bp + b];
+}
+__global__ void k_bc_lane_put(int64_t n, int bp, int b, const double2* __restrict__ src,
+                              double2* __restrict__ dst) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i * bp + b] = src[i];
 }
 
 // For the lanes with take[b] != 0: psi <- psi + (psi - before), before <- old psi.
@@ -236,6 +253,8 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
     const size_t ng = B.ghost.size();
     std::vector<GroupCont> gc(ng);
     DBuf<double> part(size_t(64) * kRedBlocks), lane_out(64);
+    DBuf<double> fb;  // one branch's state (reference order) for the single-instance fallback
+    std::vector<double> fb_h;
     std::vector<double> lane_h(64);
     int64_t xrows = 0;
     int dim = 2;
@@ -492,9 +511,9 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
       // transition Newton wanders from one of the two guesses and converges
       // from the other; the short tries find out which at a fraction of the
       // cost of a full failed run.
-      for (int attempt = 0; attempt < 4; ++attempt) {
+      for (int attempt = 0; attempt < 2; ++attempt) {
         const bool from_sec = (attempt % 2) == 0;
-        const int cap = attempt < 2 ? std::min(o.max_iters, kShortNewton) : o.max_iters;
+        const int cap = std::min(o.max_iters, kShortNewton);
         int todo = 0;
         for (int i = 0; i < ni; ++i) {
           skip[i] = ok[i] || (attempt == 0 && tried_sec[i]);
@@ -513,6 +532,100 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
         r.minres_iters += it;
         for (int i = 0; i < ni; ++i)
           if (!skip[i]) ok[i] = st[i].converged;
+      }
+      // Branches the short tries did not bring to convergence, one at a time
+      // with the single-instance FP64 solver: Newton-MINRES (inner solves to
+      // minres_rtol, no forcing) from the secant guess, then from psi_k, then
+      // the tangent continuation over the step with up to kMaxHalvings step
+      // halvings (the plain solver's way across a transition). A lockstep
+      // launch costs about as much for one active lane as for all of them
+      // (the lanes of a row share a wave), and FP64 inner solves cross a
+      // transition in fewer Newton steps than FP32 inexact ones: at 160^3
+      // this beat full-length lockstep tries even with all 8 branches here.
+      {
+        nosh_newton_options o64 = o;
+        o64.precision = 0;
+        o64.forcing = 0;
+        o64.cheb_iters = kFallbackCheb;
+        for (int i = 0; i < ni; ++i) {
+          if (ok[i]) continue;
+          const auto tf = std::chrono::steady_clock::now();
+          size_t g = 0;
+          while (g + 1 < ng && i >= B.ghost[g + 1].dev.inst0) ++g;
+          GroupHost& G = B.ghost[g];
+          const int b = i - G.dev.inst0;
+          const int64_t n = G.mesh->n;
+          if (fb.n < size_t(2 * n)) fb.alloc(2 * n);
+          std::map<std::string, double> p1{{"g", bt->g_h[i]}, {"mu", mu[i]}};
+          int how = 0, fnewton = 0, fminres = 0, fhalv = 0;
+          for (int tr = 0; tr < 2 && !ok[i]; ++tr) {
+            k_bc_lane_get<<<grid_for(n, 256), 256, 0, s>>>(
+                n, G.dev.bp, b, tr == 0 ? gc[g].sec.p : gc[g].cur.p,
+                reinterpret_cast<double2*>(fb.p));
+            nosh_newton_stats s64{};
+            newton_run(bt->models[g], fb.p, p1, o64, &s64, s);
+            fnewton += s64.newton_iters;
+            fminres += s64.minres_iters_total;
+            if (s64.converged) {
+              ok[i] = 1;
+              how = tr + 1;
+              st[i].final_norm_f = s64.final_norm_f;
+              k_bc_lane_put<<<grid_for(n, 256), 256, 0, s>>>(
+                  n, G.dev.bp, b, reinterpret_cast<const double2*>(fb.p), G.psi.p);
+            }
+          }
+          if (!ok[i]) {
+            fb_h.resize(2 * n);
+            launch_gather_lane(n, G.dev.bp, b, G.mesh->perm.p,
+                               reinterpret_cast<const double*>(gc[g].cur.p), fb.p, s);
+            fb.download(fb_h.data(), 2 * n, s);
+            NOSH_HIP(hipStreamSynchronize(s));
+            nosh_continuation_options co;
+            nosh_continuation_default_options(&co);
+            co.parameter = "mu";
+            co.start = bt->mu_h[i] + (k - 1) * h;
+            co.stop = mu[i];
+            co.steps = 1;
+            co.predictor = 1;
+            co.max_halvings = kMaxHalvings;
+            co.newton = o64;
+            nosh_continuation_step crec[2];
+            int cdone = 0;
+            const char* cn[] = {"g"};
+            const double cv[] = {bt->g_h[i]};
+            const int crc = nosh_continuation_run(bt->models[g], fb_h.data(), 1, cn, cv, &co,
+                                                  crec, &cdone);
+            for (int q = 0; q < std::min(cdone + 1, 2); ++q) {
+              fnewton += crec[q].newton_iters;
+              fminres += crec[q].minres_iters;
+            }
+            if (crc == NOSH_OK && cdone == 2) {
+              ok[i] = 1;
+              how = 3;
+              fhalv = crec[1].halvings;
+              r.halvings = std::max(r.halvings, fhalv);
+              st[i].final_norm_f = crec[1].norm_f;
+              fb.upload(fb_h.data(), 2 * n, s);
+              launch_scatter_lane(n, G.dev.bp, b, G.mesh->perm.p, fb.p,
+                                  reinterpret_cast<double*>(G.psi.p), s);
+            }
+          }
+          if (ok[i]) st[i].converged = 1;
+          r.newton_iters += fnewton;
+          r.minres_iters += fminres;
+          if (progress)
+            std::fprintf(stderr,
+                         "[batch continuation] step %d branch %d: FP64 fallback %s, newton %d "
+                         "minres %d halvings %d, %.3fs\n",
+                         k, i,
+                         how == 1 ? "converged from the secant guess"
+                                  : how == 2 ? "converged from psi_k"
+                                             : how == 3 ? "converged by tangent steps"
+                                                        : "failed",
+                         fnewton, fminres, fhalv,
+                         std::chrono::duration<double>(std::chrono::steady_clock::now() - tf)
+                             .count());
+        }
       }
       // Branches still not converged (a transition none of the guesses
       // reaches): the step is split into 2, 4, ... sub-steps for them alone
