@@ -93,10 +93,11 @@ def test_batch_continuation_surrogate_seeds_and_gains():
 
 
 def test_batch_continuation_halves_steps_for_failing_branches():
-    """A large step (0 -> 0.1 in mu) with a 3-step Newton limit: the full step
-    fails from every guess, the driver splits it for the failing branches
-    (8 sub-steps here), and every branch ends on a solution of F = 0
-    (oracle)."""
+    """A large step (0 -> 0.1 in mu) with a 3-step Newton limit: the step
+    fails from every guess in the lockstep tries and in the single-instance
+    FP64 Newton fallback, so the branches are split into halved sub-steps
+    (the FP64 tangent continuation, batched sub-steps after it), and every
+    branch ends on a solution of F = 0 (oracle)."""
     import nosh_amd as nosh
     import torch
     # Device memory freed with NaN in it: the driver's buffers must not read
