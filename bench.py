@@ -353,7 +353,7 @@ def continuation_phase(args, model, n):
     }
 
 
-def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev):
+def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev, A):
     """FP64 J(psi) phi timed on the library's launch stream (warm and cold)."""
     import torch
     E, n = mesh.num_edges, mesh.num_vertices
@@ -421,6 +421,37 @@ def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev):
         ev1.record(stream)
     ev1.synchronize()
     t_j32 = ev0.elapsed_time(ev1) / 1e3 / args.spmv_reps  # includes the block conversion
+    # K.phi and F(psi) (FP64, internal order), against the canonical bytes
+    # 40E + 44n and 40E + 52n.
+    import nosh_amd as nosh
+    others = {}
+    keo = nosh.Keo(mesh, nosh.ScalarFieldConstant(mesh, 1.0),
+                   nosh.VectorFieldExplicitValues(mesh, A, 0.0))
+    keo.set_parameters(params)
+
+    def timed(fn):
+        with torch.cuda.stream(stream):
+            for _ in range(3):
+                fn()
+            ev0.record(stream)
+            for _ in range(args.spmv_reps):
+                fn()
+            ev1.record(stream)
+        ev1.synchronize()
+        return ev0.elapsed_time(ev1) / 1e3 / args.spmv_reps
+
+    t_k = timed(lambda: keo.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream))
+    t_f = timed(lambda: model.eval_f_internal(x.data_ptr(), params, y.data_ptr(),
+                                              stream.cuda_stream))
+    for name, t_, b_ in (("keo_apply (K.phi FP64)", t_k, 40 * E + 44 * n),
+                         ("compute_f (F(psi) FP64)", t_f, 40 * E + 52 * n),
+                         ("k_block_apply<float> (J.phi FP32, with the block conversion)", t_j32,
+                          24 * E + 36 * n)):
+        others[name] = {"us_per_launch": round(t_ * 1e6, 2), "bytes_per_launch": b_,
+                        "achieved": round(b_ / t_ / 1e9, 1),
+                        "frac": round(b_ / t_ / 1e9 / HBM_PEAK_GBS, 4)}
+    say("K.phi {:.1f} us ({:.0f} GB/s), F {:.1f} us ({:.0f} GB/s)".format(
+        t_k * 1e6, (40 * E + 44 * n) / t_k / 1e9, t_f * 1e6, (40 * E + 52 * n) / t_f / 1e9))
     traffic, traffic_src = _profiled_traffic(args.ncell)
     say(f"J.phi {t_j*1e6:.1f} us -> {ach:.0f} GB/s ({ach/HBM_PEAK_GBS:.1%}); copy "
         f"{copy_gbs:.0f} GB/s; fp32 J.phi {t_j32*1e6:.1f} us")
@@ -437,7 +468,8 @@ def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev):
             "layout_note": "sliced ELLPACK (64-row slices) of the off-diagonal entries: "
                            "16 B value + 4 B column per padded slot, 4 B per slice, "
                            "24 B 2x2 block + 16 B x + 16 B y per vertex",
-            "nnz_pad": layout["nnz_pad"], "jphi_fp32_us": round(t_j32 * 1e6, 2)}
+            "nnz_pad": layout["nnz_pad"], "jphi_fp32_us": round(t_j32 * 1e6, 2),
+            "other_kernels": others}
 
 
 def surrogate_mfma(rows, stream, dev, nin=9, width=256, depth=4, reps=5):
@@ -523,7 +555,7 @@ def headline(args):
     psi0 = mg.random_state(n, 0)
     phi = mg.random_state(n, 1)
     stream = torch.cuda.Stream(device=dev)
-    roofline = jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev)
+    roofline = jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev, A)
     if args.roofline_only:
         print(json.dumps({"roofline": roofline}))
         return

@@ -203,6 +203,11 @@ int nosh_nls_eval_f(nosh_nls_t nls, const double* x, int nparams, const char* co
 int nosh_nls_eval_f_device(nosh_nls_t nls, const double* x, int nparams,
                            const char* const* names, const double* values, double* f,
                            void* stream);
+/* The same with x and f device vectors in internal order (no permutation),
+ * for device-resident callers that keep state in internal order. */
+int nosh_nls_eval_f_internal(nosh_nls_t nls, const double* x, int nparams,
+                             const char* const* names, const double* values, double* f,
+                             void* stream);
 /* Column `col` of DfDp, in get_p_names order (computeDFDP_,
  * src/model_evaluator_nls.cpp:630-695, including its use of the dK/dp of the
  * model's deriv_parameter for every column). */

@@ -145,6 +145,7 @@ def _load():
         "nosh_nls_param_nominal": (i32, [vp, i32, C.POINTER(d)]),
         "nosh_nls_eval_f": (i32, [vp, vp, i32, cpp, vp, vp]),
         "nosh_nls_eval_f_device": (i32, [vp, vp, i32, cpp, vp, vp, vp]),
+        "nosh_nls_eval_f_internal": (i32, [vp, vp, i32, cpp, vp, vp, vp]),
         "nosh_nls_eval_dfdp": (i32, [vp, vp, i32, cpp, vp, i32, vp]),
         "nosh_nls_create_w_op": (i32, [vp, pvp]),
         "nosh_nls_create_w_prec": (i32, [vp, pvp]),
@@ -374,6 +375,11 @@ class Keo(_Operator):
         _check(_lib.nosh_keo_apply_device(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr),
                                           C.c_void_p(stream)))
 
+    def apply_internal(self, x_ptr: int, y_ptr: int, stream: int = 0):
+        """y = K x on device vectors in internal order."""
+        _check(_lib.nosh_keo_apply_internal(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr),
+                                            C.c_void_p(stream)))
+
 
 class DkeoDP(Keo):
     """parameter_matrix::DkeoDP: dK/d(param_name)."""
@@ -491,6 +497,12 @@ class Nls(_Handle):
         p = _Params(params)
         _check(_lib.nosh_nls_eval_f(self._h, _ptr(x), *p.args, _ptr(f)))
         return f
+
+    def eval_f_internal(self, x_ptr: int, params: Dict[str, float], f_ptr: int, stream: int = 0):
+        """F on device vectors in internal order (no permutation)."""
+        p = _Params(params)
+        _check(_lib.nosh_nls_eval_f_internal(self._h, C.c_void_p(x_ptr), *p.args,
+                                             C.c_void_p(f_ptr), C.c_void_p(stream)))
 
     def eval_dfdp(self, x, params: Dict[str, float], column: int) -> np.ndarray:
         m = 2 * self.mesh.num_vertices

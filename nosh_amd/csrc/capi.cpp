@@ -779,6 +779,15 @@ int nosh_nls_eval_f_device(nosh_nls_t nls, const double* x, int np, const char* 
   });
 }
 
+int nosh_nls_eval_f_internal(nosh_nls_t nls, const double* x, int np, const char* const* names,
+                             const double* values, double* f, void* stream) {
+  return guard([&] {
+    NOSH_CHECK(nls && x && f, kInvalid, "null argument");
+    const auto p = to_map(np, names, values);
+    nls_f_dev(nls, p, x, f, nullptr, stream_of(stream));
+  });
+}
+
 int nosh_nls_eval_dfdp(nosh_nls_t nls, const double* x, int np, const char* const* names,
                        const double* values, int col, double* dfdp) {
   return guard([&] {

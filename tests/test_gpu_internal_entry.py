@@ -1,0 +1,43 @@
+"""GPU: the internal-order device entry points (F, K.phi) agree bitwise with
+the reference-order ones through the mesh permutation."""
+import numpy as np
+import pytest
+
+from nosh_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+def test_internal_order_entries_match_reference_order():
+    import torch
+    import nosh_amd as nosh
+    c, t = mg.kuhn_cube(6, jitter=0.2, seed=2)
+    mesh = nosh.Mesh(c, t)
+    A = mg.magnetic_potential(c)
+    mvp = nosh.VectorFieldExplicitValues(mesh, A, 0.0)
+    th = nosh.ScalarFieldConstant(mesh, 1.0)
+    model = nosh.Nls(mesh, mvp, nosh.ScalarFieldConstant(mesh, -1.0), 1.0, th)
+    n = c.shape[0]
+    psi = mg.random_state(n, 0)
+    p = {"g": 1.3, "mu": 0.4}
+    f_ref = model.eval_f(psi, p)
+    keo = nosh.Keo(mesh, th, nosh.VectorFieldExplicitValues(mesh, A, 0.0))
+    keo.set_parameters({"mu": 0.4})
+    k_ref = keo.apply(psi)
+    dev = torch.device("cuda:0")
+    x_ref = torch.from_numpy(psi).to(dev)
+    x = torch.empty_like(x_ref)
+    y = torch.empty_like(x_ref)
+    out = torch.empty_like(x_ref)
+    torch.cuda.synchronize()  # the copies above ran on torch's stream
+    mesh.to_internal(x_ref.data_ptr(), x.data_ptr())
+    model.eval_f_internal(x.data_ptr(), p, y.data_ptr())
+    mesh.to_reference(y.data_ptr(), out.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), f_ref)
+    keo.apply_internal(x.data_ptr(), y.data_ptr())
+    mesh.to_reference(y.data_ptr(), out.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), k_ref)
+    with pytest.raises(KeyError):
+        model.eval_f_internal(x.data_ptr(), {"mu": 0.4}, y.data_ptr())
