@@ -292,6 +292,8 @@ int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int fold, int m
 /* Times every (block_threads, fold, mode) candidate on one Newton step from
  * psi and keeps the fastest; with use_cache the choice is read from /
  * appended to $NOSH_AUTOTUNE_CACHE (default ~/.cache/nosh_amd_autotune.tsv). */
+int nosh_batch_autotune(nosh_batch_t batch, const double* psi, const nosh_newton_options* opts,
+                        int use_cache, int* best_threads, int* best_fold, int* best_mode);
 /* Measurement (no counterpart in the reference): average duration of one
  * batched Chebyshev step (the kernel the batched MINRES spends most time in),
  * every instance active, rebuilt at the batch's current states, timed with
@@ -301,10 +303,12 @@ int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int fold, int m
  * read and written, z_out, the P block; FP32). Resets the MINRES state. */
 int nosh_batch_time_cheb_step(nosh_batch_t batch, int reps, double* us_per_launch,
                               double* bytes_per_launch);
-int nosh_batch_autotune(nosh_batch_t batch, const double* psi, const nosh_newton_options* opts,
-                        int use_cache, int* best_threads, int* best_fold, int* best_mode);
 /* psi: all instances concatenated (each 2 n_k values, reference order);
- * overwritten by the solutions. stats: one record per instance (may be NULL). */
+ * overwritten by the solutions. stats: one record per instance (may be NULL).
+ * A batch of 16 or more instances compacts: once at most half of them are
+ * still iterating, those continue in a batch of their own (fewer lanes
+ * per row, hipGraph mode, the same block size and fold), which compacts in
+ * turn down to 4 instances; their Eisenstat-Walker forcing restarts there. */
 int nosh_batch_solve(nosh_batch_t batch, double* psi, const nosh_newton_options* opts,
                      nosh_batch_instance_stats* stats);
 int nosh_batch_solve_device(nosh_batch_t batch, double* psi, const nosh_newton_options* opts,

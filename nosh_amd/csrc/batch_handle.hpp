@@ -27,8 +27,12 @@ void gather_all(nosh_batch_s* h, double* dst_dev, hipStream_t s);
 // o.max_iters per instance. Returns the lockstep MINRES iterations (per
 // Newton step the largest count over the active instances, summed over the
 // steps).
+// stop_below > 0: return as soon as at most stop_below instances are still
+// iterating (they come back unconverged with newton_iters < their limit and
+// their current iterate in the state).
 int newton_batch(nosh_batch_s* h, const nosh_newton_options& o, nosh_batch_instance_stats* st,
-                 hipStream_t s, const int* skip = nullptr, const int* max_iters = nullptr);
+                 hipStream_t s, const int* skip = nullptr, const int* max_iters = nullptr,
+                 int stop_below = 0);
 
 // ||F(psi)|| of every instance at the batch's current state and mu, g.
 void batch_residuals(nosh_batch_s* h, std::vector<double>& norms, hipStream_t s);

@@ -161,3 +161,28 @@ def test_cheb_step_timing_leaves_the_batch_usable(setup):
     psi, st = batch.solve(psi0, **opts)
     assert all(s["converged"] for s in st)
     _check_with_oracle(meshes, counts, mu, g, batch.split(psi))
+
+
+def test_random_start_batch_compacts_and_converges(capfd, monkeypatch):
+    """24 random-start instances on two small meshes: Newton from random
+    phases is chaotic (tens of steps for some instances), so the batch
+    compacts the last quarter into a batch of their own; every instance ends
+    on an oracle solution and its statistics add up over the phases."""
+    import nosh_amd as nosh
+    monkeypatch.setenv("NOSH_PROGRESS", "1")
+    meshes = [mg.kuhn_cube(4, jitter=0.2, seed=2), mg.square_tri(12)]
+    counts = [12, 12]
+    models = [_model(c, t) for c, t in meshes]
+    u = mg.splitmix64_uniform(7, 2 * sum(counts)).reshape(-1, 2)
+    mu, g = u[:, 0], 0.5 + 1.5 * u[:, 1]
+    psi0 = np.concatenate([mg.random_state(c.shape[0], 300 + k)
+                           for (c, t), cnt in zip(meshes, counts) for k in range(cnt)])
+    batch = nosh.Batch(list(zip(models, counts)), mu, g)
+    psi, st = batch.solve(psi0, tol=1e-10, max_iters=120, minres_rtol=1e-6, cheb_iters=4,
+                          cheb_ratio=100.0, forcing=0)
+    err = capfd.readouterr().err
+    print([s["newton_iters"] for s in st])
+    assert "compacted 24 ->" in err
+    assert all(s["converged"] for s in st), st
+    assert all(s["newton_iters"] >= 1 and s["minres_iters"] >= s["newton_iters"] for s in st)
+    _check_with_oracle(meshes, counts, mu, g, batch.split(psi))
