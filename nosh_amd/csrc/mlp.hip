@@ -14,7 +14,8 @@
 // contiguous, so one 16-byte LDS read feeds each MFMA operand.
 // The n_in -> width and width -> n_out layers (K = 8 or N = 2) are FP32
 // dot products on the vector units. GELU is the tanh form
-//   gelu(z) = z/2 (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))).
+//   gelu(z) = z/2 (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))),
+// tanh from the exponential (tanh_fast).
 
 #include <cmath>
 #include <cstdint>
@@ -32,16 +33,24 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int kBM = 64, kBN = 128, kBK = 32, kPad = 8;
 constexpr int kLds = kBK + kPad;  // bf16 per LDS row
 
+// tanh(u) = 1 - 2 / (exp(2u) + 1) with the hardware exponential and
+// reciprocal (relative error ~1e-6, far below the BF16 rounding the
+// activations take next; exp(2u) = inf gives 1, 0 gives -1). The
+// library tanhf made GELU 45 % of the fused inference time.
+__device__ __forceinline__ float tanh_fast(float u) {
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__expf(2.0f * u) + 1.0f);
+}
+
 __device__ __forceinline__ float gelu(float z) {
   const float c = 0.7978845608028654f;  // sqrt(2/pi)
   const float u = c * (z + 0.044715f * z * z * z);
-  return 0.5f * z * (1.0f + tanhf(u));
+  return 0.5f * z * (1.0f + tanh_fast(u));
 }
 
 __device__ __forceinline__ float dgelu(float z) {
   const float c = 0.7978845608028654f;
   const float u = c * (z + 0.044715f * z * z * z);
-  const float t = tanhf(u);
+  const float t = tanh_fast(u);
   return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * z * z);
 }
 
