@@ -294,8 +294,11 @@ __device__ __forceinline__ void brow_shfl_row(const BGroup& G, int32_t j, int32_
 template <class T>
 __device__ __forceinline__ typename Vt<T>::v2 brow(const BGroup& G, int64_t row, int b, T mu,
                                                     const typename Vt<T>::v2* __restrict__ x) {
-  const int32_t s = int32_t(row >> 6);
-  const int32_t beg = G.slice_ptr[s], end = G.slice_ptr[s + 1];
+  // A wave's rows lie in one 64-row slice (it holds 64 / bp consecutive rows
+  // starting at a multiple of 64 / bp), so the slice bounds are wave-uniform
+  // and come through the scalar cache.
+  const int32_t s = uniform(int32_t(row >> 6));
+  const int32_t beg = ld_uniform(G.slice_ptr + s), end = ld_uniform(G.slice_ptr + s + 1);
   const int32_t w = (end - beg) / kSliceH;
   T ar = T(0), ai = T(0);
   int32_t j = beg + int32_t(row & 63);
@@ -402,20 +405,27 @@ __device__ __forceinline__ double hi_of(const BatchArgs& a, int inst) {
   return a.upper > 0.0 ? a.upper : 1.0 + a.bound[inst];
 }
 
-__device__ __forceinline__ float2* rbuf(const BGroup& G, int k) { return (k & 1) ? G.R1 : G.R0; }
+// Buffer selectors choose between pointer values (not between the record's
+// fields), so that a record held in registers stays there when the choice
+// differs between lanes.
+__device__ __forceinline__ float2* pick(bool c, float2* a, float2* b) { return c ? a : b; }
+__device__ __forceinline__ float2* rbuf(const BGroup& G, int k) { return pick(k & 1, G.R1, G.R0); }
 __device__ __forceinline__ float2* wbuf(const BGroup& G, int k) {
   const int q = ((k % 3) + 3) % 3;
-  return q == 0 ? G.W0 : (q == 1 ? G.W1 : G.W2);
+  return pick(q == 0, G.W0, pick(q == 1, G.W1, G.W2));
 }
-__device__ __forceinline__ float2* zbuf(const BGroup& G, int k) { return (k & 1) ? G.Z2 : G.Y; }
-__device__ __forceinline__ float2* vbuf(const BGroup& G, int k) { return (k & 1) ? G.v1 : G.v; }
+__device__ __forceinline__ float2* zbuf(const BGroup& G, int k) { return pick(k & 1, G.Z2, G.Y); }
+__device__ __forceinline__ float2* vbuf(const BGroup& G, int k) { return pick(k & 1, G.v1, G.v); }
 
 // Per-thread context of a launched block. All fold row tiles of a block
 // belong to one group and a thread keeps its lane (instance) in all of them,
 // so the group record and the instance's MINRES scalars are read once per
-// thread, not once per tile.
+// thread, not once per tile. The record is copied into registers by scalar
+// loads at kernel entry (its index made wave-uniform): read through a pointer
+// it would be reloaded by vector loads after every store of a row tile, which
+// the compiler cannot prove does not alias it, with a wait on each reload.
 struct Ctx {
-  const BGroup* G;
+  BGroup G;
   int b, inst, rows_per_vb;
   int64_t row0;  // the thread's row in the block's first tile
   bool lane_ok;  // a real instance (b < B)
@@ -423,13 +433,22 @@ struct Ctx {
 
 __device__ __forceinline__ Ctx make_ctx(const BatchArgs& a, int64_t v0) {
   Ctx c;
-  c.G = &a.groups[group_of(a, v0)];
-  const int bp = c.G->bp;
+  // Read through the constant address space: the record is never written by
+  // a kernel, and pointers loaded from constant memory are known to address
+  // global memory, so the row loops use global (not flat) loads.
+  const int gi = __builtin_amdgcn_readfirstlane(group_of(a, v0));
+#if __HIP_DEVICE_COMPILE__
+  typedef const __attribute__((address_space(4))) BGroup* ConstGroups;
+  c.G = ((ConstGroups)a.groups)[gi];
+#else
+  c.G = a.groups[gi];  // host pass only parses device functions
+#endif
+  const int bp = c.G.bp;
   c.b = threadIdx.x & (bp - 1);
-  c.inst = c.G->inst0 + c.b;
+  c.inst = c.G.inst0 + c.b;
   c.rows_per_vb = int(blockDim.x) / bp;
-  c.row0 = (v0 - c.G->block0) * c.rows_per_vb + threadIdx.x / bp;
-  c.lane_ok = c.b < c.G->B;
+  c.row0 = (v0 - c.G.block0) * c.rows_per_vb + threadIdx.x / bp;
+  c.lane_ok = c.b < c.G.B;
   return c;
 }
 
@@ -471,7 +490,7 @@ __device__ double rows_cheb_first(const BatchArgs& a, const Ctx& c, int mode) {
   const int done = S.done, it = mode == 2 ? S.it : 0;
   const double alfa = S.alfa, beta = S.beta;
   if (done) return s;
-  const BGroup& G = *c.G;
+  const BGroup& G = c.G;
   // The previous iteration's pending update of w and x (Givens coefficients
   // from its last reduction, v from its Lanczos step), kept out of the
   // gathering kernel to spare its registers.
@@ -520,7 +539,7 @@ __device__ double rows_cheb_step(const BatchArgs& a, const Ctx& c, int mode, int
   if (!c.lane_ok) return s;
   const MinresScalars& S = a.S[c.inst];
   const int done = S.done, it = mode == 2 ? S.it : 0;
-  const BGroup& G = *c.G;
+  const BGroup& G = c.G;
   if (meta_shfl(G) ? __all(done) : done) return s;
   const bool last = (j == a.iters - 1);
   const float2* r = rbuf(G, it);
@@ -564,7 +583,7 @@ __device__ double rows_minres_a(const BatchArgs& a, const Ctx& c) {
   const MinresScalars& S = a.S[c.inst];
   const int done = S.done, it = S.it;
   const double beta = S.beta, oldb = S.oldb;
-  const BGroup& G = *c.G;
+  const BGroup& G = c.G;
   if (meta_shfl(G) ? __all(done) : done) return s;
   const float2* y = zbuf(G, 0);
   const float2* r1 = rbuf(G, it);

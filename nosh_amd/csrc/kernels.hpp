@@ -24,6 +24,17 @@ __device__ __forceinline__ int64_t xcd_block(bool remap) {
 
 __device__ __forceinline__ int32_t uniform(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Load of a value that no kernel writes, at a wave-uniform address, through
+// the constant address space (a scalar load).
+__device__ __forceinline__ int32_t ld_uniform(const int32_t* p) {
+#if __HIP_DEVICE_COMPILE__
+  return *reinterpret_cast<const __attribute__((address_space(4))) int32_t*>(
+      reinterpret_cast<uintptr_t>(p));
+#else
+  return *p;  // host pass only parses device functions
+#endif
+}
+
 // Off-diagonal product of one internal row: sum_k val_k * x[col_k]
 // (complex multiply), in slot order.
 template <class T>
