@@ -292,9 +292,9 @@ def continuation_phase(args, model, n):
                  "achieved": round(bytes_cs / us_cs / 1e3, 1), "peak": 8000.0, "unit": "GB/s",
                  "frac": round(bytes_cs / us_cs / 1e3 / 8000.0, 4),
                  "us_per_launch": round(us_cs, 2), "bytes_per_launch": int(bytes_cs),
-                 "layout_note": "16 B per padded slot (packed column, alpha, a~; shared by the "
-                                "lanes) + 52 B per row and lane (z_in, r, D read+write, z_out, "
-                                "P block; FP32)"}
+                 "layout_note": "12 B per padded slot (packed column, alpha, a~; shared by the "
+                                "lanes) + 44 B per row and lane (z_in, r, z_{j-2} read and "
+                                "overwritten by z_j, P block; FP32), step j = 2"}
     say(f"continuation: kb_cheb_step {us_cs:.1f} us, {bytes_cs / 1e6:.1f} MB -> "
         f"{bytes_cs / us_cs / 1e3:.0f} GB/s")
     finals = batch.split(psi)

@@ -297,10 +297,11 @@ int nosh_batch_autotune(nosh_batch_t batch, const double* psi, const nosh_newton
 /* Measurement (no counterpart in the reference): average duration of one
  * batched Chebyshev step (the kernel the batched MINRES spends most time in),
  * every instance active, rebuilt at the batch's current states, timed with
- * HIP events on the library stream over reps launches; bytes_per_launch
- * (nullable) receives its algorithmic bytes: 16 per padded slot (packed
- * column, alpha, a~, shared by the lanes) + 52 per row and lane (z_in, r, D
- * read and written, z_out, the P block; FP32). Resets the MINRES state. */
+ * HIP events on the library stream over reps launches (step 2, or step 1 at
+ * Chebyshev degree 2); bytes_per_launch (nullable) receives its algorithmic
+ * bytes: 12 per padded slot (packed column, alpha, a~, shared by the lanes)
+ * + 44 per row and lane (z_in, r, z_{j-2} read and overwritten by z_j, the
+ * P block; FP32; 36 at step 1). Resets the MINRES state. */
 int nosh_batch_time_cheb_step(nosh_batch_t batch, int reps, double* us_per_launch,
                               double* bytes_per_launch);
 /* psi: all instances concatenated (each 2 n_k values, reference order);

@@ -51,7 +51,7 @@ void alloc_group(GroupHost& G) {
   G.x.alloc(m);
   G.jblk.alloc(3 * m);
   G.pblk.alloc(3 * m);
-  for (auto* v : {&G.Y, &G.Z2, &G.v, &G.v1, &G.u, &G.R0, &G.R1, &G.D, &G.W0, &G.W1, &G.W2}) {
+  for (auto* v : {&G.Y, &G.Z2, &G.v, &G.v1, &G.u, &G.R0, &G.R1, &G.W0, &G.W1, &G.W2}) {
     v->alloc(m);
     v->zero(default_stream());
   }
@@ -68,7 +68,6 @@ void alloc_group(GroupHost& G) {
   d.u = G.u.p;
   d.R0 = G.R0.p;
   d.R1 = G.R1.p;
-  d.D = G.D.p;
   d.W0 = G.W0.p;
   d.W1 = G.W1.p;
   d.W2 = G.W2.p;
@@ -355,12 +354,14 @@ int nosh_batch_time_cheb_step(nosh_batch_t batch, int reps, double* us_per_launc
     hipStream_t s = default_stream();
     *us_per_launch = batch->b.time_cheb_step(s, reps);
     if (bytes_per_launch) {
-      // Per padded slot the packed {column, alpha, a~} (16 B, shared by the
-      // lanes); per row and lane z_in, r, D (read and written), z_out
-      // (8 B each, FP32 complex) and the three FP32 entries of the P block.
+      // Per padded slot the packed {column, alpha, a~} (12 B, shared by the
+      // lanes); per row and lane z_in, r, z_{j-2} (read and overwritten by
+      // z_j; not read at j = 1) (8 B each, FP32 complex) and the three FP32
+      // entries of the P block.
+      const double row_bytes = batch->b.args.iters >= 3 ? 44.0 : 36.0;
       double b = 0.0;
       for (auto& G : batch->b.ghost)
-        b += 16.0 * double(G.mesh->nnz_pad) + 52.0 * double(G.mesh->n) * G.dev.bp;
+        b += 12.0 * double(G.mesh->nnz_pad) + row_bytes * double(G.mesh->n) * G.dev.bp;
       *bytes_per_launch = b;
     }
   });

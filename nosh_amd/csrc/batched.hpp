@@ -8,6 +8,14 @@
 
 namespace nosh {
 
+// Metadata of one padded slot for the FP32 row product, 12 bytes: internal
+// column, alpha and a~ signed by direction (alpha 0 for padding).
+struct SlotMeta {
+  int32_t col;
+  float alpha, at;
+};
+static_assert(sizeof(SlotMeta) == 12, "slot metadata is 12 packed bytes");
+
 // One mesh and the instances solved on it (device-side record).
 struct BGroup {
   int64_t n;        // rows (vertices)
@@ -18,8 +26,7 @@ struct BGroup {
   int64_t nblocks;  // blocks of this group
   const int32_t* slice_ptr;
   const int32_t* scol;
-  const int4* s32;    // per padded slot, FP32 path: {column, alpha, a~ signed by
-                      // direction, 0} (alpha 0 for padding), one 16-byte load
+  const SlotMeta* s32;  // per padded slot, FP32 path
   const double* a64;
   const double* t64;
   const double* diag;    // per row: sum alpha
@@ -31,7 +38,7 @@ struct BGroup {
   double2* x;            // [n][bp], MINRES solution accumulator
   float* jblk;           // 3 [n][bp]
   float* pblk;           // 3 [n][bp]
-  float2 *Y, *Z2, *v, *u, *R0, *R1, *D, *W0, *W1, *W2;  // [n][bp] FP32 Krylov vectors
+  float2 *Y, *Z2, *v, *u, *R0, *R1, *W0, *W1, *W2;  // [n][bp] FP32 Krylov vectors
   float2* v1;  // second Lanczos buffer: v of iteration k lives in (k odd ? v1 : v)
 };
 
@@ -57,11 +64,11 @@ struct BatchArgs {
 struct GroupHost {
   MeshCache* mesh = nullptr;
   int B = 0;
-  DBuf<int4> s32;
+  DBuf<SlotMeta> s32;
   DBuf<double> a64, t64, diag, rowabs, ct, V;
   DBuf<double2> psi, f, x;
   DBuf<float> jblk, pblk;
-  DBuf<float2> Y, Z2, v, v1, u, R0, R1, D, W0, W1, W2;
+  DBuf<float2> Y, Z2, v, v1, u, R0, R1, W0, W1, W2;
   BGroup dev{};
 };
 
@@ -98,8 +105,8 @@ struct Batch {
   // applies psi += dx; returns the largest MINRES iteration count.
   int minres(hipStream_t s, int maxit);
   void compute_norms(hipStream_t s, std::vector<double>& nrm);
-  // Average duration (us) of one Chebyshev step launch (kb_cheb_step, j = 1)
-  // with every instance active, timed with events on s over reps launches
+  // Average duration (us) of one Chebyshev step launch (kb_cheb_step,
+  // j = min(2, iters - 1)) with every instance active, timed with events on s over reps launches
   // after rebuilding at the current psi. Leaves the MINRES state reset.
   double time_cheb_step(hipStream_t s, int reps);
   ~Batch();
@@ -108,7 +115,7 @@ struct Batch {
 // Per-slot (alpha, signed a~) in FP32 and FP64 plus row sums, from the edge
 // weights of the mesh (operators.hip).
 void launch_slot_fields(const MeshCache& m, const double* alpha, const double* atilde,
-                        int4* s32, double* a64, double* t64, double* diag, double* rowabs,
+                        SlotMeta* s32, double* a64, double* t64, double* diag, double* rowabs,
                         hipStream_t s);
 // Reference-order instance vector -> lane b of an interleaved [n][bp] array, and back.
 void launch_scatter_lane(int64_t n, int bp, int b, const int32_t* perm, const double* src,

@@ -13,6 +13,7 @@
 //   jacobian_operator          src/jacobian_operator.cpp:38-200
 //   keo_regularized::rebuild   src/keo_regularized.cpp:181-264
 
+#include "batched.hpp"
 #include "kernels.hpp"
 
 namespace nosh {
@@ -417,7 +418,7 @@ namespace {
 __global__ void k_slot_fields(const int32_t* __restrict__ sedge, const int32_t* __restrict__ scol,
                               const double* __restrict__ alpha,
                               const double* __restrict__ atilde, int64_t nnz,
-                              int4* __restrict__ s32, double* __restrict__ a64,
+                              SlotMeta* __restrict__ s32, double* __restrict__ a64,
                               double* __restrict__ t64) {
   const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= nnz) return;
@@ -427,7 +428,11 @@ __global__ void k_slot_fields(const int32_t* __restrict__ sedge, const int32_t* 
     al = alpha[code >> 1];
     at = (code & 1) ? -atilde[code >> 1] : atilde[code >> 1];
   }
-  s32[j] = make_int4(scol[j], __float_as_int(float(al)), __float_as_int(float(at)), 0);
+  SlotMeta m;
+  m.col = scol[j];
+  m.alpha = float(al);
+  m.at = float(at);
+  s32[j] = m;
   a64[j] = al;
   t64[j] = at;
 }
@@ -446,7 +451,7 @@ __global__ void k_gather_lane(int64_t n, int bp, int b, const int32_t* __restric
 }  // namespace
 
 void launch_slot_fields(const MeshCache& m, const double* alpha, const double* atilde,
-                        int4* s32, double* a64, double* t64, double* diag, double* rowabs,
+                        SlotMeta* s32, double* a64, double* t64, double* diag, double* rowabs,
                         hipStream_t s) {
   k_slot_fields<<<grid_for(m.nnz_pad, kBlock), kBlock, 0, s>>>(
       m.sedge.p, m.scol.p, alpha, atilde, m.nnz_pad, s32, a64, t64);

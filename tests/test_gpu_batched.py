@@ -144,8 +144,9 @@ def test_full_group_matches_padded_group():
 
 def test_cheb_step_timing_leaves_the_batch_usable(setup):
     """The measurement entry point times the batched Chebyshev step with every
-    instance active, reports its algorithmic bytes (16 per padded slot + 52
-    per row and lane), and a solve after it still converges (oracle)."""
+    instance active, reports its algorithmic bytes (12 per padded slot + 44
+    per row and lane at Chebyshev degree 4), and a solve after it still
+    converges (oracle)."""
     import nosh_amd as nosh
     meshes, counts, models, mu, g, psi0 = setup
     batch = nosh.Batch(list(zip(models, counts)), mu, g)
@@ -154,7 +155,7 @@ def test_cheb_step_timing_leaves_the_batch_usable(setup):
     batch.solve(psi0, **opts)
     us, by = batch.time_cheb_step(20)
     lanes = [1 << max(0, (c - 1).bit_length()) for c in counts]
-    expect = sum(16 * m.mesh.layout()["nnz_pad"] + 52 * m.mesh.num_vertices * bp
+    expect = sum(12 * m.mesh.layout()["nnz_pad"] + 44 * m.mesh.num_vertices * bp
                  for m, bp in zip(models, lanes))
     print("kb_cheb_step", us, "us", by, "bytes")
     assert us > 0 and by == expect
