@@ -282,6 +282,7 @@ int nosh_batch_create(int ngroups, const nosh_nls_t* models, const int* counts, 
       d.slice_ptr = m.slice_ptr.p;
       d.scol = m.scol.p;
       d.s32 = G.s32.p;
+      d.kval = nullptr;
       d.a64 = G.a64.p;
       d.t64 = G.t64.p;
       d.diag = G.diag.p;
@@ -297,7 +298,7 @@ int nosh_batch_create(int ngroups, const nosh_nls_t* models, const int* counts, 
     B.args.ninst = inst;
     B.mu.alloc(inst);
     B.g.alloc(inst);
-    B.mu.upload(mu, inst, s);
+    B.set_mu(mu, s);
     B.g.upload(g, inst, s);
     h->mu_h.assign(mu, mu + inst);
     h->g_h.assign(g, g + inst);

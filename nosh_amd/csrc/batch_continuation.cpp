@@ -318,7 +318,7 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
       }
     };
     std::vector<double> mu(bt->mu_h), e_now, e1(ni), e0(ni);
-    auto set_mu = [&]() { B.mu.upload(mu.data(), ni, s); };
+    auto set_mu = [&]() { B.set_mu(mu.data(), s); };
 
     // Surrogate state.
     const int nin = dim + 6;
@@ -759,7 +759,7 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
     bt->stage.download(psi, bt->total_len, s);
     (void)hipStreamSynchronize(s);
     // The batch keeps its per-instance mu_0 for later solves.
-    bt->b.mu.upload(bt->mu_h.data(), bt->mu_h.size(), s);
+    bt->b.set_mu(bt->mu_h.data(), s);
     (void)hipStreamSynchronize(s);
   }
   if (steps_done) *steps_done = done;

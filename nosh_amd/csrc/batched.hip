@@ -291,6 +291,42 @@ __device__ __forceinline__ void brow_shfl_row(const BGroup& G, int32_t j, int32_
   }
 }
 
+// The shuffled, pipelined row product of brow_shfl_row over the group's
+// precomputed slot values (every active lane shares mu): no sine and cosine
+// per slot and lane. Same values, same order, same bits.
+template <int NS>
+__device__ __forceinline__ void brow_shfl_val(const BGroup& G, int32_t j, int32_t w, int b,
+                                              const float2* __restrict__ x, float& ar,
+                                              float& ai) {
+  const int lane = int(threadIdx.x & 63);
+  const int g0 = lane & ~(G.bp - 1);
+  const int bp = G.bp;
+  SlotVal mine = {0, 0.0f, 0.0f};
+  if (b < NS && b < w) mine = G.kval[j + b * kSliceH];
+  for (int32_t k = 0; k < w; k += NS, j += NS * kSliceH) {
+    int32_t c[NS];
+    float vr[NS], vi[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      c[q] = __shfl(mine.col, g0 + q, 64);
+      vr[q] = __shfl(mine.vr, g0 + q, 64);
+      vi[q] = __shfl(mine.vi, g0 + q, 64);
+    }
+    float2 xv[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+      if (k + q < w) xv[q] = x[int64_t(c[q]) * bp + b];
+    if (b < NS && k + NS + b < w) mine = G.kval[j + (NS + b) * kSliceH];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      if (k + q < w) {
+        ar += vr[q] * xv[q].x - vi[q] * xv[q].y;
+        ai += vr[q] * xv[q].y + vi[q] * xv[q].x;
+      }
+    }
+  }
+}
+
 template <class T>
 __device__ __forceinline__ typename Vt<T>::v2 brow(const BGroup& G, int64_t row, int b, T mu,
                                                     const typename Vt<T>::v2* __restrict__ x) {
@@ -305,7 +341,9 @@ __device__ __forceinline__ typename Vt<T>::v2 brow(const BGroup& G, int64_t row,
   int32_t k = 0;
   if constexpr (std::is_same<T, float>::value) {
     if (meta_shfl(G)) {
-      if (NOSH_META_PIPE)
+      if (G.kval)
+        brow_shfl_val<4>(G, j, w, b, x, ar, ai);
+      else if (NOSH_META_PIPE)
         brow_shfl_row<4>(G, j, w, b, mu, x, ar, ai);
       else
         for (; k < w; k += 4, j += 4 * kSliceH) brow_shfl4(G, j, k, w, b, mu, x, ar, ai);
@@ -723,6 +761,21 @@ __global__ void kb_reset(BatchArgs a, int maxit) {
   a.bound[i] = 0.0;
 }
 
+// Slot values at one mu for a whole group (see SlotVal).
+__global__ void kb_fill_kval(const SlotMeta* __restrict__ s32, int64_t nnz, float mu,
+                             SlotVal* __restrict__ kval) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= nnz) return;
+  const SlotMeta m = s32[j];
+  float sn, cs;
+  sc(mu * m.at, &sn, &cs);
+  SlotVal v;
+  v.col = m.col;
+  v.vr = -m.alpha * cs;
+  v.vi = m.alpha * sn;
+  kval[j] = v;
+}
+
 // Persistent MINRES: all iterations of the batch in one cooperative launch;
 // grid-wide barriers separate the phases; the loop ends when every instance
 // is done or after maxit iterations, a decision every block takes from the
@@ -789,10 +842,58 @@ int Batch::persistent_grid() {
   return int(std::min<int64_t>(cap, args.total_blocks / args.fold));
 }
 
+void Batch::set_mu(const double* m, hipStream_t s) {
+  mu_host.assign(m, m + args.ninst);
+  mu.upload(m, args.ninst, s);
+}
+
+void Batch::prepare_kval(hipStream_t s) {
+  bool changed = false;
+  for (auto& gh : ghost) {
+    BGroup& G = gh.dev;
+    bool shared = NOSH_META_SHFL && G.B == G.bp && G.bp >= 4;
+    bool any = false;
+    double m0 = 0.0;
+    for (int b = 0; shared && b < G.B; ++b) {
+      const int i = G.inst0 + b;
+      if (!active_h[i]) continue;
+      if (!any) {
+        m0 = mu_host[i];
+        any = true;
+      } else if (mu_host[i] != m0) {
+        shared = false;
+      }
+    }
+    const SlotVal* want = nullptr;
+    if (shared && any) {
+      if (!gh.kval.p) gh.kval.alloc(gh.mesh->nnz_pad);
+      if (!gh.kval_set || gh.kval_mu != m0) {
+        kb_fill_kval<<<grid_for(gh.mesh->nnz_pad, 256), 256, 0, s>>>(
+            G.s32, gh.mesh->nnz_pad, float(m0), gh.kval.p);
+        gh.kval_mu = m0;
+        gh.kval_set = true;
+      }
+      want = gh.kval.p;
+    }
+    if (G.kval != want) {
+      G.kval = want;
+      changed = true;
+    }
+  }
+  if (changed) {
+    records_host.clear();
+    for (auto& gh : ghost) records_host.push_back(gh.dev);
+    NOSH_HIP(hipMemcpyAsync(groups.p, records_host.data(), sizeof(BGroup) * records_host.size(),
+                            hipMemcpyHostToDevice, s));
+    NOSH_HIP(hipStreamSynchronize(s));
+  }
+}
+
 int Batch::minres(hipStream_t s, int maxit) {
   NOSH_CHECK(mode == 0 || args.total_blocks / args.fold <= kPersistentMaxBlocks, kUnsupported,
              "persistent grid too large");
   const unsigned nb = unsigned(args.total_blocks / args.fold), bt = unsigned(block_threads);
+  prepare_kval(s);
   kb_reset<<<grid_for(args.ninst, 128), 128, 0, s>>>(args, maxit);
   kb_rebuild<<<nb, bt, 0, s>>>(args);
   kb_setup<<<nb, bt, 0, s>>>(args);
@@ -837,6 +938,9 @@ double Batch::time_cheb_step(hipStream_t s, int reps) {
   std::vector<int> all(args.ninst, 1);
   NOSH_HIP(hipMemcpyAsync(active.p, all.data(), sizeof(int) * args.ninst, hipMemcpyHostToDevice,
                           s));
+  const std::vector<int> active_before = active_h;
+  active_h.assign(args.ninst, 1);
+  prepare_kval(s);
   kb_reset<<<grid_for(args.ninst, 128), 128, 0, s>>>(args, 1);
   kb_rebuild<<<nb, bt, 0, s>>>(args);
   kb_setup<<<nb, bt, 0, s>>>(args);
@@ -857,6 +961,7 @@ double Batch::time_cheb_step(hipStream_t s, int reps) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   // The instances are idle again.
+  active_h = active_before;
   std::vector<int> none(args.ninst, 0);
   NOSH_HIP(hipMemcpyAsync(active.p, none.data(), sizeof(int) * args.ninst, hipMemcpyHostToDevice,
                           s));

@@ -16,6 +16,16 @@ struct SlotMeta {
 };
 static_assert(sizeof(SlotMeta) == 12, "slot metadata is 12 packed bytes");
 
+// The FP32 off-diagonal value of one padded slot at a given mu, 12 bytes:
+// internal column and K = vr + i vi = -alpha e^{-i mu a~}, i.e.
+// vr = -alpha cos(mu a~), vi = alpha sin(mu a~) (the same FP32 operations as
+// the row product's own rebuild, so either path gives the same bits).
+struct SlotVal {
+  int32_t col;
+  float vr, vi;
+};
+static_assert(sizeof(SlotVal) == 12, "slot values are 12 packed bytes");
+
 // One mesh and the instances solved on it (device-side record).
 struct BGroup {
   int64_t n;        // rows (vertices)
@@ -27,6 +37,8 @@ struct BGroup {
   const int32_t* slice_ptr;
   const int32_t* scol;
   const SlotMeta* s32;  // per padded slot, FP32 path
+  const SlotVal* kval;  // per padded slot at the mu every active instance of the
+                        // group shares, or null (each lane rebuilds K at its mu)
   const double* a64;
   const double* t64;
   const double* diag;    // per row: sum alpha
@@ -65,6 +77,9 @@ struct GroupHost {
   MeshCache* mesh = nullptr;
   int B = 0;
   DBuf<SlotMeta> s32;
+  DBuf<SlotVal> kval;     // allocated on first use
+  double kval_mu = 0.0;   // the mu kval holds (valid when kval_set)
+  bool kval_set = false;
   DBuf<double> a64, t64, diag, rowabs, ct, V;
   DBuf<double2> psi, f, x;
   DBuf<float> jblk, pblk;
@@ -95,8 +110,16 @@ struct Batch {
   int graph_iters = 4;  // MINRES iterations per captured graph
   int poll = 16;        // iterations between host checks of the done flags
   hipGraphExec_t graph_exec = nullptr;
+  std::vector<double> mu_host;      // per instance, the values uploaded to mu
+  std::vector<BGroup> records_host;  // staging of the device group records
 
   void relayout(int threads, int fold);
+  // Uploads per-instance mu (and keeps the host copy).
+  void set_mu(const double* m, hipStream_t s);
+  // For every group whose active instances all share one mu (and whose rows
+  // take slot metadata by lane shuffles), fills kval at that mu (once per
+  // value) and points the group record at it; other groups rebuild K per lane.
+  void prepare_kval(hipStream_t s);
   void launch_phases_iteration(hipStream_t s);
   void build_graph(hipStream_t s);
   void drop_graph();

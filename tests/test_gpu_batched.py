@@ -142,6 +142,45 @@ def test_full_group_matches_padded_group():
     _check_with_oracle([(c, t)], [8], mu, g, b8.split(p8))
 
 
+def test_shared_mu_slot_values_match_per_lane_rebuild():
+    """When every instance of a full group shares mu, the row products read
+    the off-diagonal values precomputed once per slot instead of rebuilding
+    them per lane; both are the same FP32 operations, so instances 0-6 of a
+    group sharing mu (precomputed values) and of one whose instance 7 has
+    another mu (per-lane rebuild) take identical steps to identical bits."""
+    import nosh_amd as nosh
+    c, t = mg.kuhn_cube(6, jitter=0.2, seed=2)
+    m = _model(c, t)
+    n = c.shape[0]
+    u = mg.splitmix64_uniform(7, 8)
+    g = 0.8 + 0.4 * u
+    psi0 = []
+    for k in range(8):
+        p = 0.1 * mg.random_state(n, 500 + k)
+        p[0::2] += 1.0
+        psi0.append(p)
+    psi0 = np.concatenate(psi0)
+    opts = dict(tol=1e-10, max_iters=40, minres_rtol=1e-6, minres_max_iters=4000, cheb_iters=4,
+                cheb_ratio=100.0, forcing=0)
+    mu_shared = np.full(8, 0.2)
+    mu_mixed = mu_shared.copy()
+    mu_mixed[7] = 0.21
+    sols, stats = [], []
+    for mu in (mu_shared, mu_mixed):
+        b = nosh.Batch([(m, 8)], mu, g)
+        b.set_tuning(256, 0, 4, 16, fold=2)
+        p, st = b.solve(psi0, **opts)
+        assert all(s["converged"] for s in st)
+        sols.append(b.split(p))
+        stats.append(st)
+    for k in range(7):
+        assert stats[0][k]["newton_iters"] == stats[1][k]["newton_iters"]
+        assert stats[0][k]["minres_iters"] == stats[1][k]["minres_iters"]
+        assert np.array_equal(sols[0][k], sols[1][k]), k
+    _check_with_oracle([(c, t)], [8], mu_shared, g, sols[0])
+    _check_with_oracle([(c, t)], [8], mu_mixed, g, sols[1])
+
+
 def test_cheb_step_timing_leaves_the_batch_usable(setup):
     """The measurement entry point times the batched Chebyshev step with every
     instance active, reports its algorithmic bytes (12 per padded slot + 44
