@@ -50,6 +50,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "Newton-MINRES solves/sec at ||F||<=1e-10 vs plain FP64 GPU; SpMV HBM GB/s"
+# Iteration modes of the batched solver (nosh_batch_set_tuning).
+MODE_NAMES = ("hipGraph", "persistent", "hipGraph, 2 lanes per thread")
 
 
 def parse():
@@ -333,7 +335,7 @@ def continuation_phase(args, model, n):
         "solves_per_s": comb_rate, "seconds": t_comb, "converged": conv,
         "branches": nb, "steps": steps, "span": span, "g": [round(v, 3) for v in g],
         "autotuned": {"block_threads": tuned[0], "fold": tuned[2],
-                      "mode": ["hipGraph", "persistent"][tuned[1]], "seconds": round(t_tune, 2)},
+                      "mode": MODE_NAMES[tuned[1]], "seconds": round(t_tune, 2)},
         "lockstep_newton_iters": sum(r["newton_iters"] for r in rec),
         "lockstep_minres_iters": sum(r["minres_iters"] for r in rec),
         "max_norm_f": max(r["max_norm_f"] for r in rec),
@@ -817,7 +819,7 @@ def batched_workload(args):
            "config": {"workload": "batched uneven solves", "instances": ninst,
                       "meshes": [f"{s}^3 jittered" for s in sizes], "counts": counts,
                       "autotuned": {"block_threads": tuned[0], "fold": tuned[2],
-                                    "mode": ["hipGraph", "persistent"][tuned[1]]},
+                                    "mode": MODE_NAMES[tuned[1]]},
                       "autotune_s": round(t_tune, 2)},
            "converged": conv,
            "newton_iters_max": max(s["newton_iters"] for s in st),

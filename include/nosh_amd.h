@@ -285,7 +285,10 @@ int nosh_batch_create(int ngroups, const nosh_nls_t* models, const int* counts, 
  * mode 0: each MINRES iteration is a captured hipGraph replayed graph_iters
  * at a time, mode 1: one persistent cooperative kernel per MINRES solve
  * (only for grids of at most 64 launched blocks; larger ones get
- * NOSH_ERR_UNSUPPORTED and the batch stays in mode 0);
+ * NOSH_ERR_UNSUPPORTED and the batch stays in mode 0), mode 2: as mode 0
+ * with every thread of the gathering kernels taking 2 lanes of a row (every
+ * group needs at least 2 lanes, else NOSH_ERR_UNSUPPORTED and mode 0); the
+ * modes give the same results bit for bit;
  * poll: iterations between host checks of the done flags (mode 0). */
 int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int fold, int mode,
                           int graph_iters, int poll);
@@ -308,7 +311,8 @@ int nosh_batch_time_cheb_step(nosh_batch_t batch, int reps, double* us_per_launc
  * overwritten by the solutions. stats: one record per instance (may be NULL).
  * A batch of 16 or more instances compacts: once at most half of them are
  * still iterating, those continue in a batch of their own (fewer lanes
- * per row, hipGraph mode, the same block size and fold), which compacts in
+ * per row, hipGraph mode (2 lanes per thread if the batch uses them and the
+ * sub-batch has the lanes), the same block size and fold), which compacts in
  * turn down to 4 instances; their Eisenstat-Walker forcing restarts there. */
 int nosh_batch_solve(nosh_batch_t batch, double* psi, const nosh_newton_options* opts,
                      nosh_batch_instance_stats* stats);

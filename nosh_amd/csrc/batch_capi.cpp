@@ -331,7 +331,8 @@ int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int fold, int m
                    block_threads % maxbp == 0,
                kInvalid, "block_threads must be a multiple of 64 and of the lane count, <= 1024");
     NOSH_CHECK(fold >= 1 && fold <= 64, kInvalid, "fold must be in 1..64");
-    NOSH_CHECK(mode == 0 || mode == 1, kInvalid, "mode must be 0 (graph) or 1 (persistent)");
+    NOSH_CHECK(mode >= 0 && mode <= 2, kInvalid,
+               "mode must be 0 (graph), 1 (persistent) or 2 (graph, 2 lanes per thread)");
     NOSH_CHECK(graph_iters >= 1 && poll >= graph_iters, kInvalid, "bad graph_iters / poll");
     batch->b.graph_iters = graph_iters;
     batch->b.poll = poll;
@@ -339,12 +340,18 @@ int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int fold, int m
       batch->b.relayout(block_threads, fold);
     batch->b.drop_graph();
     batch->b.mode = 0;
-    NOSH_CHECK(mode == 0 ||
+    batch->b.lanes = 1;
+    NOSH_CHECK(mode != 1 ||
                    batch->b.args.total_blocks / batch->b.fold <= nosh::kPersistentMaxBlocks,
                kUnsupported,
                "persistent mode needs a grid of at most 64 workgroups at this block size; "
                "use mode 0 (hipGraph)");
-    batch->b.mode = mode;
+    int minbp = 64;
+    for (auto& G : batch->b.ghost) minbp = std::min(minbp, G.dev.bp);
+    NOSH_CHECK(mode != 2 || minbp >= nosh::kVecLanes, kUnsupported,
+               "mode 2 needs at least 2 lanes in every group; use mode 0 (hipGraph)");
+    batch->b.mode = mode == 1 ? 1 : 0;
+    batch->b.lanes = mode == 2 ? nosh::kVecLanes : 1;
   });
 }
 
@@ -444,7 +451,10 @@ void solve_compacting(nosh_batch_s* h, const nosh_newton_options& o, const int* 
     return;
   }
   std::unique_ptr<nosh_batch_s> owner(sub);
-  nosh_batch_set_tuning(sub, h->b.block_threads, h->b.fold, 0, h->b.graph_iters, h->b.poll);
+  // hipGraph mode, with 2 lanes per thread when the parent uses them and the
+  // sub-batch's groups have the lanes for it (else the call leaves mode 0).
+  nosh_batch_set_tuning(sub, h->b.block_threads, h->b.fold, h->b.lanes > 1 ? 2 : 0,
+                        h->b.graph_iters, h->b.poll);
   DBuf<double> buf(size_t(sub->total_len));
   int64_t so = 0;
   for (int i : rem) {
@@ -535,7 +545,7 @@ int nosh_batch_autotune(nosh_batch_t batch, const double* psi, const nosh_newton
       for (int threads : {128, 256, 512, 1024}) {
         if (threads % maxbp || threads < maxbp) continue;
         for (int fold : {1, 2, 4, 8}) {
-          for (int mode : {0, 1}) {
+          for (int mode : {0, 1, 2}) {
             if (nosh_batch_set_tuning(batch, threads, fold, mode, batch->b.graph_iters,
                                       batch->b.poll) != NOSH_OK)
               continue;

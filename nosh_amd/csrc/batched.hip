@@ -153,6 +153,73 @@ __device__ __forceinline__ void stn(double2* p, double2 v) {
 __device__ __forceinline__ void sc(float x, float* s, float* c) { __sincosf(x, s, c); }
 __device__ __forceinline__ void sc(double x, double* s, double* c) { sincos(x, s, c); }
 
+// The FP32 arithmetic of the batched row kernels, every multiply-add an
+// explicit fused operation and no other contraction, so that kernels which
+// map rows and lanes to threads differently round alike (left to itself the
+// compiler picks contractions per kernel).
+__device__ __forceinline__ void slot_value(float al, float at, float mu, float& vr, float& vi) {
+#pragma clang fp contract(off)
+  float sn, cs;
+  sc(mu * at, &sn, &cs);
+  vr = -al * cs;
+  vi = al * sn;
+}
+__device__ __forceinline__ void slot_value(double al, double at, double mu, double& vr,
+                                           double& vi) {
+  double sn, cs;
+  sc(mu * at, &sn, &cs);
+  vr = -al * cs;
+  vi = al * sn;
+}
+// a += (vr + i vi) x.
+__device__ __forceinline__ void cmac(float& ar, float& ai, float vr, float vi, float2 x) {
+#pragma clang fp contract(off)
+  ar += __builtin_fmaf(vr, x.x, -(vi * x.y));
+  ai += __builtin_fmaf(vi, x.x, vr * x.y);
+}
+__device__ __forceinline__ void cmac(double& ar, double& ai, double vr, double vi, double2 x) {
+  ar += vr * x.x - vi * x.y;
+  ai += vr * x.y + vi * x.x;
+}
+// acc + (b0 b2; b2 b1) z.
+__device__ __forceinline__ float2 blk_add(float2 acc, float b0, float b1, float b2, float2 z) {
+#pragma clang fp contract(off)
+  return make_float2(__builtin_fmaf(b2, z.y, __builtin_fmaf(b0, z.x, acc.x)),
+                     __builtin_fmaf(b1, z.y, __builtin_fmaf(b2, z.x, acc.y)));
+}
+// (a b; b c)^{-1} (x0, x1).
+__device__ __forceinline__ void dinv32(float a, float c, float b, float x0, float x1, float* o0,
+                                       float* o1) {
+#pragma clang fp contract(off)
+  const float det = __builtin_fmaf(a, c, -(b * b));
+  *o0 = __builtin_fmaf(c, x0, -(b * x1)) / det;
+  *o1 = __builtin_fmaf(a, x1, -(b * x0)) / det;
+}
+// One Chebyshev step of one (row, lane) from its row product acc:
+// z_j = z_{j-1} + c1 (z_{j-1} - z_{j-2}) + c2 P^{-1} (r - A z_{j-1}), z_{-1} = 0.
+__device__ __forceinline__ float2 cheb_z(float2 acc, float2 zi, float2 zpp, bool first, float2 rv,
+                                         float b0, float b1, float b2, float c1, float c2) {
+#pragma clang fp contract(off)
+  const float2 az = blk_add(acc, b0, b1, b2, zi);
+  float u0, u1;
+  dinv32(b0, b1, b2, rv.x - az.x, rv.y - az.y, &u0, &u1);
+  float2 d = first ? zi : make_float2(zi.x - zpp.x, zi.y - zpp.y);
+  d.x = __builtin_fmaf(c1, d.x, c2 * u0);
+  d.y = __builtin_fmaf(c1, d.y, c2 * u1);
+  return make_float2(zi.x + d.x, zi.y + d.y);
+}
+// u = (J y) / beta - cb r1 of one (row, lane) from its row product acc.
+__device__ __forceinline__ float2 lanczos_u(float2 acc, float2 yi, float b0, float b1, float b2,
+                                            float ib, float cb, float2 r1) {
+#pragma clang fp contract(off)
+  const float2 jy = blk_add(acc, b0, b1, b2, yi);
+  return make_float2(jy.x * ib - cb * r1.x, jy.y * ib - cb * r1.y);
+}
+__device__ __forceinline__ double dot2(float2 a, float2 b) {
+#pragma clang fp contract(off)
+  return __builtin_fma(double(a.x), double(b.x), double(a.y) * double(b.y));
+}
+
 // Off-diagonal product of one (row, instance) with values rebuilt in registers.
 // The slot count of a row is its slice's width, the same for the whole
 // wave; slots are taken 4 at a time so that their metadata loads, and then
@@ -187,11 +254,9 @@ __device__ __forceinline__ void brow_slots(const BGroup& G, int32_t j, int b, T 
   for (int q = 0; q < K; ++q) xv[q] = x[int64_t(c[q]) * bp + b];
 #pragma unroll
   for (int q = 0; q < K; ++q) {
-    T sn, cs;
-    sc(mu * tv[q], &sn, &cs);
-    const T vr = -av[q] * cs, vi = av[q] * sn;
-    ar += vr * xv[q].x - vi * xv[q].y;
-    ai += vr * xv[q].y + vi * xv[q].x;
+    T vr, vi;
+    slot_value(av[q], tv[q], mu, vr, vi);
+    cmac(ar, ai, vr, vi, xv[q]);
   }
 }
 
@@ -235,11 +300,9 @@ __device__ __forceinline__ void brow_shfl4(const BGroup& G, int32_t j, int32_t k
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     if (k + q < w) {
-      float sn, cs;
-      sc(mu * tv[q], &sn, &cs);
-      const float vr = -av[q] * cs, vi = av[q] * sn;
-      ar += vr * xv[q].x - vi * xv[q].y;
-      ai += vr * xv[q].y + vi * xv[q].x;
+      float vr, vi;
+      slot_value(av[q], tv[q], mu, vr, vi);
+      cmac(ar, ai, vr, vi, xv[q]);
     }
   }
 }
@@ -281,11 +344,9 @@ __device__ __forceinline__ void brow_shfl_row(const BGroup& G, int32_t j, int32_
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       if (k + q < w) {
-        float sn, cs;
-        sc(mu * tv[q], &sn, &cs);
-        const float vr = -av[q] * cs, vi = av[q] * sn;
-        ar += vr * xv[q].x - vi * xv[q].y;
-        ai += vr * xv[q].y + vi * xv[q].x;
+        float vr, vi;
+        slot_value(av[q], tv[q], mu, vr, vi);
+        cmac(ar, ai, vr, vi, xv[q]);
       }
     }
   }
@@ -319,10 +380,7 @@ __device__ __forceinline__ void brow_shfl_val(const BGroup& G, int32_t j, int32_
     if (b < NS && k + NS + b < w) mine = G.kval[j + (NS + b) * kSliceH];
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
-      if (k + q < w) {
-        ar += vr[q] * xv[q].x - vi[q] * xv[q].y;
-        ai += vr[q] * xv[q].y + vi[q] * xv[q].x;
-      }
+      if (k + q < w) cmac(ar, ai, vr[q], vi[q], xv[q]);
     }
   }
 }
@@ -469,18 +527,23 @@ struct Ctx {
   bool lane_ok;  // a real instance (b < B)
 };
 
-__device__ __forceinline__ Ctx make_ctx(const BatchArgs& a, int64_t v0) {
-  Ctx c;
-  // Read through the constant address space: the record is never written by
-  // a kernel, and pointers loaded from constant memory are known to address
-  // global memory, so the row loops use global (not flat) loads.
+// The record of the group holding virtual block v0, read through the
+// constant address space at a wave-uniform index: the record is never
+// written by a kernel, and pointers loaded from constant memory are known to
+// address global memory, so the row loops use global (not flat) loads.
+__device__ __forceinline__ BGroup load_group(const BatchArgs& a, int64_t v0) {
   const int gi = __builtin_amdgcn_readfirstlane(group_of(a, v0));
 #if __HIP_DEVICE_COMPILE__
   typedef const __attribute__((address_space(4))) BGroup* ConstGroups;
-  c.G = ((ConstGroups)a.groups)[gi];
+  return ((ConstGroups)a.groups)[gi];
 #else
-  c.G = a.groups[gi];  // host pass only parses device functions
+  return a.groups[gi];  // host pass only parses device functions
 #endif
+}
+
+__device__ __forceinline__ Ctx make_ctx(const BatchArgs& a, int64_t v0) {
+  Ctx c;
+  c.G = load_group(a, v0);
   const int bp = c.G.bp;
   c.b = threadIdx.x & (bp - 1);
   c.inst = c.G.inst0 + c.b;
@@ -560,12 +623,12 @@ __device__ double rows_cheb_first(const BatchArgs& a, const Ctx& c, int mode) {
       z = rv;
     } else {
       float o0, o1;
-      dinv<float>(ldn(G.pblk + idx), ldn(G.pblk + nb3 + idx), ldn(G.pblk + 2 * nb3 + idx), rv.x,
-                  rv.y, &o0, &o1);
+      dinv32(ldn(G.pblk + idx), ldn(G.pblk + nb3 + idx), ldn(G.pblk + 2 * nb3 + idx), rv.x, rv.y,
+             &o0, &o1);
       z = make_float2(o0 / th, o1 / th);
     }
     stn(z0 + idx, z);
-    s += double(rv.x) * double(z.x) + double(rv.y) * double(z.y);
+    s += dot2(rv, z);
   }
   return s;
 }
@@ -601,21 +664,10 @@ __device__ double rows_cheb_step(const BatchArgs& a, const Ctx& c, int mode, int
     const float b0 = ldn(G.pblk + idx), b1 = ldn(G.pblk + nb3 + idx),
                 b2 = ldn(G.pblk + 2 * nb3 + idx);
     const float2 rv = ldn(r + idx);
-    const float res0 = rv.x - (acc.x + b0 * zi.x + b2 * zi.y);
-    const float res1 = rv.y - (acc.y + b2 * zi.x + b1 * zi.y);
-    float u0, u1;
-    dinv<float>(b0, b1, b2, res0, res1, &u0, &u1);
-    float2 d = zi;
-    if (j > 1) {
-      const float2 zpp = ldn(zout + idx);
-      d.x -= zpp.x;
-      d.y -= zpp.y;
-    }
-    d.x = c1 * d.x + c2 * u0;
-    d.y = c1 * d.y + c2 * u1;
-    const float2 z = make_float2(zi.x + d.x, zi.y + d.y);
+    const float2 zpp = j > 1 ? ldn(zout + idx) : make_float2(0.f, 0.f);
+    const float2 z = cheb_z(acc, zi, zpp, j == 1, rv, b0, b1, b2, c1, c2);
     stn(zout + idx, z);
-    if (last) s += double(rv.x) * double(z.x) + double(rv.y) * double(z.y);
+    if (last) s += dot2(rv, z);
   }
   return s;
 }
@@ -646,19 +698,314 @@ __device__ double rows_minres_a(const BatchArgs& a, const Ctx& c) {
     const float2 yi = y[idx];
     const float b0 = ldn(G.jblk + idx), b1 = ldn(G.jblk + nb3 + idx),
                 b2 = ldn(G.jblk + 2 * nb3 + idx);
-    float2 uu = make_float2((acc.x + b0 * yi.x + b2 * yi.y) * ib,
-                            (acc.y + b2 * yi.x + b1 * yi.y) * ib);
-    if (it >= 1) {
-      const float2 rv = ldn(r1 + idx);
-      uu.x -= cb * rv.x;
-      uu.y -= cb * rv.y;
-    }
+    const float2 rv = it >= 1 ? ldn(r1 + idx) : make_float2(0.f, 0.f);
+    const float2 uu = lanczos_u(acc, yi, b0, b1, b2, ib, cb, rv);
     const float2 vn = make_float2(yi.x * ib, yi.y * ib);
     stn(vbuf(G, it) + idx, vn);
     stn(G.u + idx, uu);
-    s += double(vn.x) * double(uu.x) + double(vn.y) * double(uu.y);
+    s += dot2(vn, uu);
   }
   return s;
+}
+
+// ------------------------------------------------- L lanes per thread
+// Mapping of the gathering kernels in tuning mode 2 (L = kVecLanes): a thread
+// takes L consecutive lanes of one row (the bp / L threads of a row load the
+// same slot record; no lane shuffles) and gathers 8 L contiguous bytes of x
+// per slot, so a wave keeps L times the gathered bytes in flight of the
+// one-lane mapping for the same address and metadata registers. Launched
+// with block_threads / L threads over the same row tiles as the one-lane
+// kernels: same partial sums, same arithmetic per (row, lane), same bits.
+// At 96^3 x 8 lanes the Chebyshev step takes 114 us with L = 2 (60 VGPRs,
+// 8 waves per SIMD) and 115 us with L = 4 (92 VGPRs, 5 waves), against
+// 124 us for the one-lane kernel.
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <int L>
+__device__ __forceinline__ void load_lanes(const float2* __restrict__ p, float2 (&v)[L]) {
+#pragma unroll
+  for (int h = 0; h < L / 2; ++h) {
+    const float4 q = reinterpret_cast<const float4*>(p)[h];
+    v[2 * h] = make_float2(q.x, q.y);
+    v[2 * h + 1] = make_float2(q.z, q.w);
+  }
+}
+template <int L>
+__device__ __forceinline__ void load_lanes_nt(const float2* p, float2 (&v)[L]) {
+#pragma unroll
+  for (int h = 0; h < L / 2; ++h) {
+    const f4v q = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p) + h);
+    v[2 * h] = make_float2(q.x, q.y);
+    v[2 * h + 1] = make_float2(q.z, q.w);
+  }
+}
+template <int L>
+__device__ __forceinline__ void load_floats_nt(const float* p, float (&v)[L]) {
+  if constexpr (L == 2) {
+    const f2v q = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+    v[0] = q.x;
+    v[1] = q.y;
+  } else {
+#pragma unroll
+    for (int h = 0; h < L / 4; ++h) {
+      const f4v q = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p) + h);
+      v[4 * h] = q.x;
+      v[4 * h + 1] = q.y;
+      v[4 * h + 2] = q.z;
+      v[4 * h + 3] = q.w;
+    }
+  }
+}
+// Stores the lanes whose mask bit is set (all of them as 16-byte stores).
+template <int L>
+__device__ __forceinline__ void store_lanes_nt(float2* p, const float2 (&v)[L], bool all,
+                                               const bool (&act)[L]) {
+  if (all) {
+#pragma unroll
+    for (int h = 0; h < L / 2; ++h) {
+      f4v q;
+      q.x = v[2 * h].x;
+      q.y = v[2 * h].y;
+      q.z = v[2 * h + 1].x;
+      q.w = v[2 * h + 1].y;
+      __builtin_nontemporal_store(q, reinterpret_cast<f4v*>(p) + h);
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+      if (act[l]) stn(p + l, v[l]);
+  }
+}
+
+template <int L>
+struct VCtx {
+  BGroup G;
+  int T;      // threads per row (bp / L)
+  int b0;     // the thread's first lane
+  int R;      // rows per tile (block_threads / bp)
+  int64_t row0;  // the thread's row in the block's first tile
+};
+
+template <int L>
+__device__ __forceinline__ VCtx<L> make_vctx(const BatchArgs& a, int64_t v0) {
+  VCtx<L> c;
+  c.G = load_group(a, v0);
+  c.T = c.G.bp / L;
+  c.R = int(blockDim.x) / c.T;
+  c.b0 = int(threadIdx.x % c.T) * L;
+  c.row0 = (v0 - c.G.block0) * c.R + threadIdx.x / c.T;
+  return c;
+}
+
+// Row product of one row for L lanes: slots two at a time, the next two
+// slots' records loaded (clamped to the row's last slot) after the current
+// gathers, so that waiting for those gathers does not wait for them. A
+// clamped record's gather is loaded and not used.
+template <int L>
+__device__ __forceinline__ void brow_vec(const BGroup& G, int64_t row, int b0,
+                                         const float (&mu)[L], const float2* __restrict__ x,
+                                         float2 (&acc)[L]) {
+#pragma unroll
+  for (int l = 0; l < L; ++l) acc[l] = make_float2(0.f, 0.f);
+  const int32_t sl = uniform(int32_t(row >> 6));
+  const int32_t beg = ld_uniform(G.slice_ptr + sl), end = ld_uniform(G.slice_ptr + sl + 1);
+  const int32_t w = (end - beg) / kSliceH;
+  if (w <= 0) return;
+  const int32_t j0 = beg + int32_t(row & 63);
+  const int64_t bp = G.bp;
+  if (G.kval) {
+    SlotVal m0 = G.kval[j0], m1 = G.kval[j0 + min(1, w - 1) * kSliceH];
+    for (int32_t k = 0; k < w; k += 2) {
+      float2 x0[L], x1[L];
+      load_lanes<L>(x + m0.col * bp + b0, x0);
+      load_lanes<L>(x + m1.col * bp + b0, x1);
+      const SlotVal n0 = G.kval[j0 + min(k + 2, w - 1) * kSliceH];
+      const SlotVal n1 = G.kval[j0 + min(k + 3, w - 1) * kSliceH];
+#pragma unroll
+      for (int l = 0; l < L; ++l) cmac(acc[l].x, acc[l].y, m0.vr, m0.vi, x0[l]);
+      if (k + 1 < w) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) cmac(acc[l].x, acc[l].y, m1.vr, m1.vi, x1[l]);
+      }
+      m0 = n0;
+      m1 = n1;
+    }
+  } else {
+    SlotMeta m0 = G.s32[j0], m1 = G.s32[j0 + min(1, w - 1) * kSliceH];
+    for (int32_t k = 0; k < w; k += 2) {
+      float2 x0[L], x1[L];
+      load_lanes<L>(x + m0.col * bp + b0, x0);
+      load_lanes<L>(x + m1.col * bp + b0, x1);
+      const SlotMeta n0 = G.s32[j0 + min(k + 2, w - 1) * kSliceH];
+      const SlotMeta n1 = G.s32[j0 + min(k + 3, w - 1) * kSliceH];
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        float vr, vi;
+        slot_value(m0.alpha, m0.at, mu[l], vr, vi);
+        cmac(acc[l].x, acc[l].y, vr, vi, x0[l]);
+      }
+      if (k + 1 < w) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          float vr, vi;
+          slot_value(m1.alpha, m1.at, mu[l], vr, vi);
+          cmac(acc[l].x, acc[l].y, vr, vi, x1[l]);
+        }
+      }
+      m0 = n0;
+      m1 = n1;
+    }
+  }
+}
+
+// Per-instance partial sums of one launched block (see lane_partials), from
+// the L lanes of every thread.
+template <int L>
+__device__ __forceinline__ void lane_partials_v(const BatchArgs& a, int64_t v0, const VCtx<L>& c,
+                                                const double (&v)[L]) {
+  __shared__ double lds[1024];
+  const BGroup& G = c.G;
+  const int bp = G.bp;
+  const int rho = int(threadIdx.x) / c.T;
+#pragma unroll
+  for (int l = 0; l < L; ++l) lds[rho * bp + c.b0 + l] = v[l];
+  __syncthreads();
+  for (int b = int(threadIdx.x); b < bp && b < G.B; b += int(blockDim.x)) {
+    double s = 0.0;
+    for (int r = 0; r < c.R; ++r) s += lds[r * bp + b];
+    a.partials[int64_t(G.inst0 + b) * a.maxblocks + (v0 - G.block0) / a.fold] = s;
+  }
+  __syncthreads();
+}
+
+template <int L>
+__device__ void rows_cheb_step_v(const BatchArgs& a, const VCtx<L>& c, int mode, int j,
+                                 double (&s)[L]) {
+  const BGroup& G = c.G;
+  bool act[L];
+  float c1[L], c2[L], mu[L];
+  bool any = false, all = true;
+  int it = 0;
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    s[l] = 0.0;
+    act[l] = false;
+    c1[l] = c2[l] = mu[l] = 0.0f;
+    const int b = c.b0 + l;
+    if (b < G.B) {
+      const int inst = G.inst0 + b;
+      const MinresScalars& S = a.S[inst];
+      mu[l] = float(a.mu[inst]);
+      if (!S.done) {
+        act[l] = true;
+        any = true;
+        it = mode == 2 ? S.it : 0;  // the same for every active instance (lockstep)
+        const Cheb cc = cheb_coef(hi_of(a, inst), a.ratio, j);
+        c1[l] = float(cc.c1);
+        c2[l] = float(cc.c2);
+      }
+    }
+    all = all && act[l];
+  }
+  if (!any) return;
+  const bool last = (j == a.iters - 1);
+  const float2* r = rbuf(G, it);
+  const float2* zin = zbuf(G, a.z0 + j - 1);
+  float2* zout = zbuf(G, a.z0 + j);
+  const int64_t nb3 = G.n * G.bp;
+  for (int t = 0; t < a.fold; ++t) {
+    const int64_t row = c.row0 + int64_t(t) * c.R;
+    if (row >= G.n) break;
+    const int64_t idx0 = row * G.bp + c.b0;
+    float2 acc[L];
+    brow_vec<L>(G, row, c.b0, mu, zin, acc);
+    float2 zi[L], rv[L], zpp[L], z[L];
+    float b0[L], b1[L], b2[L];
+    load_lanes<L>(zin + idx0, zi);
+    load_lanes_nt<L>(r + idx0, rv);
+    if (j > 1) {
+      load_lanes_nt<L>(zout + idx0, zpp);
+    } else {
+#pragma unroll
+      for (int l = 0; l < L; ++l) zpp[l] = make_float2(0.f, 0.f);
+    }
+    load_floats_nt<L>(G.pblk + idx0, b0);
+    load_floats_nt<L>(G.pblk + nb3 + idx0, b1);
+    load_floats_nt<L>(G.pblk + 2 * nb3 + idx0, b2);
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+      z[l] = cheb_z(acc[l], zi[l], zpp[l], j == 1, rv[l], b0[l], b1[l], b2[l], c1[l], c2[l]);
+    store_lanes_nt<L>(zout + idx0, z, all, act);
+    if (last) {
+#pragma unroll
+      for (int l = 0; l < L; ++l)
+        if (act[l]) s[l] += dot2(rv[l], z[l]);
+    }
+  }
+}
+
+template <int L>
+__device__ void rows_minres_a_v(const BatchArgs& a, const VCtx<L>& c, double (&s)[L]) {
+  const BGroup& G = c.G;
+  bool act[L];
+  float ib[L], cb[L], mu[L];
+  bool any = false, all = true;
+  int it = 0;
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    s[l] = 0.0;
+    act[l] = false;
+    ib[l] = cb[l] = mu[l] = 0.0f;
+    const int b = c.b0 + l;
+    if (b < G.B) {
+      const int inst = G.inst0 + b;
+      const MinresScalars& S = a.S[inst];
+      mu[l] = float(a.mu[inst]);
+      if (!S.done) {
+        act[l] = true;
+        any = true;
+        it = S.it;  // the same for every active instance (lockstep)
+        ib[l] = float(1.0 / S.beta);
+        cb[l] = S.it >= 1 ? float(S.beta / S.oldb) : 0.0f;
+      }
+    }
+    all = all && act[l];
+  }
+  if (!any) return;
+  const float2* y = zbuf(G, 0);
+  const float2* r1 = rbuf(G, it);
+  float2* vout = vbuf(G, it);
+  const int64_t nb3 = G.n * G.bp;
+  for (int t = 0; t < a.fold; ++t) {
+    const int64_t row = c.row0 + int64_t(t) * c.R;
+    if (row >= G.n) break;
+    const int64_t idx0 = row * G.bp + c.b0;
+    float2 acc[L];
+    brow_vec<L>(G, row, c.b0, mu, y, acc);
+    float2 yi[L], rv[L], uu[L], vn[L];
+    float b0[L], b1[L], b2[L];
+    load_lanes<L>(y + idx0, yi);
+    if (it >= 1) {
+      load_lanes_nt<L>(r1 + idx0, rv);
+    } else {
+#pragma unroll
+      for (int l = 0; l < L; ++l) rv[l] = make_float2(0.f, 0.f);
+    }
+    load_floats_nt<L>(G.jblk + idx0, b0);
+    load_floats_nt<L>(G.jblk + nb3 + idx0, b1);
+    load_floats_nt<L>(G.jblk + 2 * nb3 + idx0, b2);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      uu[l] = lanczos_u(acc[l], yi[l], b0[l], b1[l], b2[l], ib[l], cb[l], rv[l]);
+      vn[l] = make_float2(yi[l].x * ib[l], yi[l].y * ib[l]);
+    }
+    store_lanes_nt<L>(vout + idx0, vn, all, act);
+    store_lanes_nt<L>(G.u + idx0, uu, all, act);
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+      if (act[l]) s[l] += dot2(vn[l], uu[l]);
+  }
 }
 
 __device__ void ph_flush(const BatchArgs& a, int64_t vb) {
@@ -739,6 +1086,22 @@ __global__ void kb_cheb_step(BatchArgs a, int mode, int j) {
   blk_cheb_step(a, xcd_block(true), mode, j);
 }
 __global__ void kb_minres_a(BatchArgs a) { blk_minres_a(a, xcd_block(true)); }
+template <int L>
+__global__ void kb_cheb_step_v(BatchArgs a, int mode, int j) {
+  const int64_t v0 = xcd_block(true) * a.fold;
+  const VCtx<L> c = make_vctx<L>(a, v0);
+  double s[L];
+  rows_cheb_step_v<L>(a, c, mode, j, s);
+  if (j == a.iters - 1) lane_partials_v<L>(a, v0, c, s);
+}
+template <int L>
+__global__ void kb_minres_a_v(BatchArgs a) {
+  const int64_t v0 = xcd_block(true) * a.fold;
+  const VCtx<L> c = make_vctx<L>(a, v0);
+  double s[L];
+  rows_minres_a_v<L>(a, c, s);
+  lane_partials_v<L>(a, v0, c, s);
+}
 __global__ void kb_flush(BatchArgs a) {
   const int64_t rb = xcd_block(true);
   for (int r = 0; r < a.fold; ++r) ph_flush(a, rb * a.fold + r);
@@ -807,12 +1170,28 @@ __global__ void kb_minres_persistent(BatchArgs a, int maxit) {
 }  // namespace
 
 // ------------------------------------------------------------------ host
+void Batch::launch_cheb_step(hipStream_t s, int mode_, int j) {
+  const unsigned nb = unsigned(args.total_blocks / args.fold), bt = unsigned(block_threads);
+  if (lanes == kVecLanes)
+    kb_cheb_step_v<kVecLanes><<<nb, bt / kVecLanes, 0, s>>>(args, mode_, j);
+  else
+    kb_cheb_step<<<nb, bt, 0, s>>>(args, mode_, j);
+}
+
+void Batch::launch_minres_a(hipStream_t s) {
+  const unsigned nb = unsigned(args.total_blocks / args.fold), bt = unsigned(block_threads);
+  if (lanes == kVecLanes)
+    kb_minres_a_v<kVecLanes><<<nb, bt / kVecLanes, 0, s>>>(args);
+  else
+    kb_minres_a<<<nb, bt, 0, s>>>(args);
+}
+
 void Batch::launch_phases_iteration(hipStream_t s) {
   const unsigned nb = unsigned(args.total_blocks / args.fold), bt = unsigned(block_threads);
-  kb_minres_a<<<nb, bt, 0, s>>>(args);
+  launch_minres_a(s);
   kb_finalize<<<args.ninst, 256, 0, s>>>(args, 2, nullptr);
   kb_cheb_first<<<nb, bt, 0, s>>>(args, 2);
-  for (int j = 1; j < args.iters; ++j) kb_cheb_step<<<nb, bt, 0, s>>>(args, 2, j);
+  for (int j = 1; j < args.iters; ++j) launch_cheb_step(s, 2, j);
   kb_finalize<<<args.ninst, 256, 0, s>>>(args, 3, nullptr);
 }
 
@@ -851,7 +1230,9 @@ void Batch::prepare_kval(hipStream_t s) {
   bool changed = false;
   for (auto& gh : ghost) {
     BGroup& G = gh.dev;
-    bool shared = NOSH_META_SHFL && G.B == G.bp && G.bp >= 4;
+    // Read by the shuffled one-lane row products and by every row product of
+    // the L-lane mapping.
+    bool shared = lanes > 1 || (NOSH_META_SHFL && G.B == G.bp && G.bp >= 4);
     bool any = false;
     double m0 = 0.0;
     for (int b = 0; shared && b < G.B; ++b) {
@@ -899,7 +1280,7 @@ int Batch::minres(hipStream_t s, int maxit) {
   kb_setup<<<nb, bt, 0, s>>>(args);
   // y = M^{-1} r0 and beta1 for every instance.
   kb_cheb_first<<<nb, bt, 0, s>>>(args, 1);
-  for (int j = 1; j < args.iters; ++j) kb_cheb_step<<<nb, bt, 0, s>>>(args, 1, j);
+  for (int j = 1; j < args.iters; ++j) launch_cheb_step(s, 1, j);
   kb_finalize<<<args.ninst, 256, 0, s>>>(args, 1, nullptr);
   int issued = 0;
   if (mode == 1) {
@@ -945,7 +1326,7 @@ double Batch::time_cheb_step(hipStream_t s, int reps) {
   kb_rebuild<<<nb, bt, 0, s>>>(args);
   kb_setup<<<nb, bt, 0, s>>>(args);
   kb_cheb_first<<<nb, bt, 0, s>>>(args, 1);
-  kb_cheb_step<<<nb, bt, 0, s>>>(args, 1, 1);
+  launch_cheb_step(s, 1, 1);
   // A step past the first (it reads z_{j-2}); repeated, it rewrites z_{j-2}
   // in place with bounded values (|c1| < 1).
   const int j = std::min(2, args.iters - 1);
@@ -953,7 +1334,7 @@ double Batch::time_cheb_step(hipStream_t s, int reps) {
   NOSH_HIP(hipEventCreate(&e0));
   NOSH_HIP(hipEventCreate(&e1));
   NOSH_HIP(hipEventRecord(e0, s));
-  for (int q = 0; q < reps; ++q) kb_cheb_step<<<nb, bt, 0, s>>>(args, 1, j);
+  for (int q = 0; q < reps; ++q) launch_cheb_step(s, 1, j);
   NOSH_HIP(hipEventRecord(e1, s));
   NOSH_HIP(hipEventSynchronize(e1));
   float ms = 0.0f;

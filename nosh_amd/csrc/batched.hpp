@@ -95,6 +95,9 @@ struct GroupHost {
 // (hipGraph) mode serves them.
 constexpr int64_t kPersistentMaxBlocks = 64;
 
+// Lanes per thread of the gathering kernels in tuning mode 2.
+constexpr int kVecLanes = 2;
+
 struct Batch {
   std::vector<GroupHost> ghost;
   DBuf<BGroup> groups;
@@ -107,6 +110,8 @@ struct Batch {
   int block_threads = 256;
   int fold = 1;         // virtual blocks per launched block (fewer reduction partials)
   int mode = 0;         // 0: hipGraph-captured iterations, 1: persistent cooperative kernel
+  int lanes = 1;        // lanes per thread of the gathering kernels in hipGraph mode
+                        // (1 or kVecLanes)
   int graph_iters = 4;  // MINRES iterations per captured graph
   int poll = 16;        // iterations between host checks of the done flags
   hipGraphExec_t graph_exec = nullptr;
@@ -117,10 +122,13 @@ struct Batch {
   // Uploads per-instance mu (and keeps the host copy).
   void set_mu(const double* m, hipStream_t s);
   // For every group whose active instances all share one mu (and whose rows
-  // take slot metadata by lane shuffles), fills kval at that mu (once per
-  // value) and points the group record at it; other groups rebuild K per lane.
+  // take slot metadata by lane shuffles, or with lanes > 1), fills kval at
+  // that mu (once per value) and points the group record at it; other groups
+  // rebuild K per lane.
   void prepare_kval(hipStream_t s);
   void launch_phases_iteration(hipStream_t s);
+  void launch_cheb_step(hipStream_t s, int mode, int j);
+  void launch_minres_a(hipStream_t s);
   void build_graph(hipStream_t s);
   void drop_graph();
   int persistent_grid();

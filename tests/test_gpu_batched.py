@@ -73,13 +73,50 @@ def test_batched_graph_and_persistent_agree(setup, threads, fold):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("threads,fold", [(128, 1), (256, 4)])
+def test_two_lanes_per_thread_match_one(setup, threads, fold):
+    """Mode 2 (every thread of the gathering kernels takes 2 lanes of a row)
+    against mode 0 (one lane per thread) on groups of 3 (padded to 4 lanes),
+    5 (padded to 8) and 2 instances: the same arithmetic per row and lane and
+    the same partial sums, so bitwise identical; a batch with a 1-lane group
+    refuses mode 2 and stays in mode 0."""
+    import nosh_amd as nosh
+    meshes, _, models, mu, g, _ = setup
+    counts = [3, 5, 2]
+    u = mg.splitmix64_uniform(11, 2 * sum(counts)).reshape(-1, 2)
+    mu, g = 0.3 * u[:, 0], 0.6 + 0.8 * u[:, 1]
+    psi0 = []
+    for (c, t), cnt in zip(meshes, counts):
+        for q in range(cnt):
+            p = 0.1 * mg.random_state(c.shape[0], 700 + len(psi0))
+            p[0::2] += 1.0
+            psi0.append(p)
+    psi0 = np.concatenate(psi0)
+    batch = nosh.Batch(list(zip(models, counts)), mu, g)
+    opts = dict(tol=1e-10, max_iters=60, minres_rtol=1e-6, minres_max_iters=4000, cheb_iters=4,
+                cheb_ratio=100.0, forcing=0)
+    out = {}
+    for mode in (0, 2):
+        print(f"[batched threads={threads} fold={fold} mode={mode}]", flush=True)
+        batch.set_tuning(threads, mode, 4, 16, fold=fold)
+        psi, st = batch.solve(psi0, **opts)
+        assert all(s["converged"] for s in st), st
+        out[mode] = (psi, [(s["newton_iters"], s["minres_iters"]) for s in st])
+    assert out[0][1] == out[2][1]
+    assert np.array_equal(out[0][0], out[2][0])
+    _check_with_oracle(meshes, counts, mu, g, batch.split(out[2][0]))
+    b1 = nosh.Batch(list(zip(models, [3, 5, 1])), mu[:9], g[:9])
+    with pytest.raises(nosh.NoshError):
+        b1.set_tuning(256, 2, 4, 16)
+
+
 def test_batched_autotune(setup, tmp_path, monkeypatch):
     import nosh_amd as nosh
     monkeypatch.setenv("NOSH_AUTOTUNE_CACHE", str(tmp_path / "tune.tsv"))
     meshes, counts, models, mu, g, psi0 = setup
     batch = nosh.Batch(list(zip(models, counts)), mu, g)
     t, m, f = batch.autotune(psi0, cheb_iters=4, cheb_ratio=100.0)
-    assert t in (128, 256, 512, 1024) and m in (0, 1) and f in (1, 2, 4, 8)
+    assert t in (128, 256, 512, 1024) and m in (0, 1, 2) and f in (1, 2, 4, 8)
     assert (tmp_path / "tune.tsv").read_text().strip().endswith(f"{t} {f} {m}")
     assert batch.autotune(psi0, cheb_iters=4, cheb_ratio=100.0) == (t, m, f)  # from the cache
     psi, st = batch.solve(psi0, tol=1e-10, max_iters=80, minres_rtol=1e-6, cheb_iters=4,
