@@ -91,7 +91,8 @@ def parse():
     p.add_argument("--cont-plain-branches", type=int, default=1,
                    help="branches run with the plain FP64 continuation for the ratio")
     p.add_argument("--cont-tuning", default="",
-                   help="block_threads,fold for the batched continuation (skips the autotuner)")
+                   help="block_threads,fold[,mode] for the batched continuation (skips the "
+                        "autotuner; mode 0 by default)")
     p.add_argument("--no-surrogate", action="store_true")
     p.add_argument("--workload", default="cont96",
                    choices=["cont96", "batched", "cont", "disk", "stability"],
@@ -264,9 +265,10 @@ def continuation_phase(args, model, n):
     batch = nosh.Batch([(model, nb)], mu0, g)
     t0 = time.perf_counter()
     if args.cont_tuning:
-        bt, fold = (int(v) for v in args.cont_tuning.split(","))
-        batch.set_tuning(bt, 0, 4, 16, fold=fold)
-        tuned = (bt, 0, fold)
+        v = [int(x) for x in args.cont_tuning.split(",")]
+        bt, fold, mode = v[0], v[1], (v[2] if len(v) > 2 else 0)
+        batch.set_tuning(bt, mode, 4, 16, fold=fold)
+        tuned = (bt, mode, fold)
     else:
         tuned = batch.autotune(psi0, use_cache=False, **kw)
     t_tune = time.perf_counter() - t0
@@ -289,7 +291,8 @@ def continuation_phase(args, model, n):
     # step), every branch active, at the final states, timed with HIP events
     # on the library stream.
     us_cs, bytes_cs = batch.time_cheb_step(50)
-    cheb_roof = {"bound": "hbm", "kernel": "kb_cheb_step (batched FP32 Chebyshev step, "
+    kname = "kb_cheb_step_v<2>" if tuned[1] == 2 else "kb_cheb_step"
+    cheb_roof = {"bound": "hbm", "kernel": f"{kname} (batched FP32 Chebyshev step, "
                                            f"{nb} lanes, all active)",
                  "achieved": round(bytes_cs / us_cs / 1e3, 1), "peak": 8000.0, "unit": "GB/s",
                  "frac": round(bytes_cs / us_cs / 1e3 / 8000.0, 4),
@@ -297,7 +300,7 @@ def continuation_phase(args, model, n):
                  "layout_note": "12 B per padded slot (packed column, alpha, a~; shared by the "
                                 "lanes) + 44 B per row and lane (z_in, r, z_{j-2} read and "
                                 "overwritten by z_j, P block; FP32), step j = 2"}
-    say(f"continuation: kb_cheb_step {us_cs:.1f} us, {bytes_cs / 1e6:.1f} MB -> "
+    say(f"continuation: {kname} {us_cs:.1f} us, {bytes_cs / 1e6:.1f} MB -> "
         f"{bytes_cs / us_cs / 1e3:.0f} GB/s")
     finals = batch.split(psi)
     # Error of the mixed-precision states against FP64: FP64 Newton from each

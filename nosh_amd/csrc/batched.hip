@@ -523,9 +523,30 @@ __device__ __forceinline__ float2* vbuf(const BGroup& G, int k) { return pick(k 
 struct Ctx {
   BGroup G;
   int b, inst, rows_per_vb;
-  int64_t row0;  // the thread's row in the block's first tile
+  int64_t row0;     // the thread's row in the block's first tile
+  int64_t tstride;  // rows from one of the block's tiles to the next
   bool lane_ok;  // a real instance (b < B)
 };
+
+// Row tiles of launched blocks. A group's launched blocks take its tiles
+// strided by their count (block k: tiles k, k + nl, k + 2 nl, ... of the
+// group, nl = nblocks / fold), so that the blocks resident at one time work
+// on one band of rows, whose gathers fit an XCD's L2, rather than each on its
+// own stretch of fold consecutive tiles (with fold 8 and 512 threads that put
+// an XCD's whole eighth of a 96^3 mesh in flight and the gathers missed L2).
+// v0 = rb * fold of launched block rb; first_tile is the block's index k in
+// its group, which also numbers its partial sums.
+__device__ __forceinline__ int64_t first_tile(const BatchArgs& a, const BGroup& G, int64_t v0) {
+  return (v0 - G.block0) / a.fold;
+}
+__device__ __forceinline__ int64_t tile_stride(const BatchArgs& a, const BGroup& G) {
+  return G.nblocks / a.fold;
+}
+// Virtual block (tile) t of launched block rb.
+__device__ __forceinline__ int64_t tile_vb(const BatchArgs& a, int64_t rb, int t) {
+  const BGroup& G = a.groups[group_of(a, rb * a.fold)];
+  return G.block0 + first_tile(a, G, rb * a.fold) + int64_t(t) * tile_stride(a, G);
+}
 
 // The record of the group holding virtual block v0, read through the
 // constant address space at a wave-uniform index: the record is never
@@ -548,7 +569,8 @@ __device__ __forceinline__ Ctx make_ctx(const BatchArgs& a, int64_t v0) {
   c.b = threadIdx.x & (bp - 1);
   c.inst = c.G.inst0 + c.b;
   c.rows_per_vb = int(blockDim.x) / bp;
-  c.row0 = (v0 - c.G.block0) * c.rows_per_vb + threadIdx.x / bp;
+  c.row0 = first_tile(a, c.G, v0) * c.rows_per_vb + threadIdx.x / bp;
+  c.tstride = tile_stride(a, c.G) * c.rows_per_vb;
   c.lane_ok = c.b < c.G.B;
   return c;
 }
@@ -606,7 +628,7 @@ __device__ double rows_cheb_first(const BatchArgs& a, const Ctx& c, int mode) {
   const int64_t nb3 = G.n * G.bp;
   float2* z0 = zbuf(G, a.z0);
   for (int t = 0; t < a.fold; ++t) {
-    const int64_t row = c.row0 + int64_t(t) * c.rows_per_vb;
+    const int64_t row = c.row0 + int64_t(t) * c.tstride;
     if (row >= G.n) break;
     const int64_t idx = row * G.bp + c.b;
     if (pending) w_update(G, q, it, idx, ldn(vprev + idx));
@@ -655,7 +677,7 @@ __device__ double rows_cheb_step(const BatchArgs& a, const Ctx& c, int mode, int
   const float mu = float(a.mu[c.inst]);
   const int64_t nb3 = G.n * G.bp;
   for (int t = 0; t < a.fold; ++t) {
-    const int64_t row = c.row0 + int64_t(t) * c.rows_per_vb;
+    const int64_t row = c.row0 + int64_t(t) * c.tstride;
     if (row >= G.n) break;
     const int64_t idx = row * G.bp + c.b;
     const float2 acc = brow<float>(G, row, c.b, mu, zin);
@@ -690,7 +712,7 @@ __device__ double rows_minres_a(const BatchArgs& a, const Ctx& c) {
   const float mu = float(a.mu[c.inst]);
   const int64_t nb3 = G.n * G.bp;
   for (int t = 0; t < a.fold; ++t) {
-    const int64_t row = c.row0 + int64_t(t) * c.rows_per_vb;
+    const int64_t row = c.row0 + int64_t(t) * c.tstride;
     if (row >= G.n) break;
     const int64_t idx = row * G.bp + c.b;
     const float2 acc = brow<float>(G, row, c.b, mu, y);
@@ -784,7 +806,8 @@ struct VCtx {
   int T;      // threads per row (bp / L)
   int b0;     // the thread's first lane
   int R;      // rows per tile (block_threads / bp)
-  int64_t row0;  // the thread's row in the block's first tile
+  int64_t row0;     // the thread's row in the block's first tile
+  int64_t tstride;  // rows from one of the block's tiles to the next
 };
 
 template <int L>
@@ -794,7 +817,8 @@ __device__ __forceinline__ VCtx<L> make_vctx(const BatchArgs& a, int64_t v0) {
   c.T = c.G.bp / L;
   c.R = int(blockDim.x) / c.T;
   c.b0 = int(threadIdx.x % c.T) * L;
-  c.row0 = (v0 - c.G.block0) * c.R + threadIdx.x / c.T;
+  c.row0 = first_tile(a, c.G, v0) * c.R + threadIdx.x / c.T;
+  c.tstride = tile_stride(a, c.G) * c.R;
   return c;
 }
 
@@ -915,7 +939,7 @@ __device__ void rows_cheb_step_v(const BatchArgs& a, const VCtx<L>& c, int mode,
   float2* zout = zbuf(G, a.z0 + j);
   const int64_t nb3 = G.n * G.bp;
   for (int t = 0; t < a.fold; ++t) {
-    const int64_t row = c.row0 + int64_t(t) * c.R;
+    const int64_t row = c.row0 + int64_t(t) * c.tstride;
     if (row >= G.n) break;
     const int64_t idx0 = row * G.bp + c.b0;
     float2 acc[L];
@@ -978,7 +1002,7 @@ __device__ void rows_minres_a_v(const BatchArgs& a, const VCtx<L>& c, double (&s
   float2* vout = vbuf(G, it);
   const int64_t nb3 = G.n * G.bp;
   for (int t = 0; t < a.fold; ++t) {
-    const int64_t row = c.row0 + int64_t(t) * c.R;
+    const int64_t row = c.row0 + int64_t(t) * c.tstride;
     if (row >= G.n) break;
     const int64_t idx0 = row * G.bp + c.b0;
     float2 acc[L];
@@ -1050,7 +1074,7 @@ __device__ void ph_finalize(const BatchArgs& a, int inst, int what, double* out)
 __device__ __forceinline__ void blk_compute_f(const BatchArgs& a, int64_t rb) {
   const int64_t v0 = rb * a.fold;
   double s = 0.0;
-  for (int r = 0; r < a.fold; ++r) s += ph_compute_f(a, v0 + r);
+  for (int r = 0; r < a.fold; ++r) s += ph_compute_f(a, tile_vb(a, rb, r));
   lane_partials(a, v0, s);
 }
 __device__ __forceinline__ void blk_cheb_first(const BatchArgs& a, int64_t rb, int mode) {
@@ -1073,11 +1097,11 @@ __device__ __forceinline__ void blk_minres_a(const BatchArgs& a, int64_t rb) {
 __global__ void kb_compute_f(BatchArgs a) { blk_compute_f(a, xcd_block(true)); }
 __global__ void kb_rebuild(BatchArgs a) {
   const int64_t rb = xcd_block(true);
-  for (int r = 0; r < a.fold; ++r) ph_rebuild(a, rb * a.fold + r);
+  for (int r = 0; r < a.fold; ++r) ph_rebuild(a, tile_vb(a, rb, r));
 }
 __global__ void kb_setup(BatchArgs a) {
   const int64_t rb = xcd_block(true);
-  for (int r = 0; r < a.fold; ++r) ph_setup(a, rb * a.fold + r);
+  for (int r = 0; r < a.fold; ++r) ph_setup(a, tile_vb(a, rb, r));
 }
 __global__ void kb_cheb_first(BatchArgs a, int mode) {
   blk_cheb_first(a, xcd_block(true), mode);
@@ -1104,7 +1128,7 @@ __global__ void kb_minres_a_v(BatchArgs a) {
 }
 __global__ void kb_flush(BatchArgs a) {
   const int64_t rb = xcd_block(true);
-  for (int r = 0; r < a.fold; ++r) ph_flush(a, rb * a.fold + r);
+  for (int r = 0; r < a.fold; ++r) ph_flush(a, tile_vb(a, rb, r));
 }
 __global__ void kb_finalize(BatchArgs a, int what, double* out) {
   ph_finalize(a, blockIdx.x, what, out);
