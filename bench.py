@@ -68,7 +68,7 @@ def parse():
     p.add_argument("--solve-reps", type=int, default=2,
                    help="random-start solves timed per variant")
     p.add_argument("--cheb", type=int, default=6, help="Chebyshev degree, plain FP64 solver")
-    p.add_argument("--cont-cheb", type=int, default=4, help="Chebyshev degree, batched solver")
+    p.add_argument("--cont-cheb", type=int, default=5, help="Chebyshev degree, batched solver")
     p.add_argument("--ratio", type=float, default=300.0)
     p.add_argument("--cont-ratio", type=float, default=300.0,
                    help="Chebyshev interval ratio, batched solver")

@@ -40,7 +40,7 @@ namespace nosh {
 namespace {
 
 struct Lane {
-  const BGroup* G;
+  BGroup G;  // in registers (see load_group)
   int64_t row;
   int b;
   int inst;
@@ -54,16 +54,29 @@ __device__ __forceinline__ int group_of(const BatchArgs& a, int64_t vb) {
   return gi;
 }
 
+// The record of the group holding virtual block v0, read through the
+// constant address space at a wave-uniform index: the record is never
+// written by a kernel, and pointers loaded from constant memory are known to
+// address global memory, so the row loops use global (not flat) loads.
+__device__ __forceinline__ BGroup load_group(const BatchArgs& a, int64_t v0) {
+  const int gi = __builtin_amdgcn_readfirstlane(group_of(a, v0));
+#if __HIP_DEVICE_COMPILE__
+  typedef const __attribute__((address_space(4))) BGroup* ConstGroups;
+  return ((ConstGroups)a.groups)[gi];
+#else
+  return a.groups[gi];  // host pass only parses device functions
+#endif
+}
+
 __device__ __forceinline__ Lane lane_ctx(const BatchArgs& a, int64_t vb) {
   Lane L;
-  const int gi = group_of(a, vb);
-  L.G = &a.groups[gi];
-  const int bp = L.G->bp;
+  L.G = load_group(a, vb);
+  const int bp = L.G.bp;
   L.b = threadIdx.x & (bp - 1);
-  L.row = (vb - L.G->block0) * (blockDim.x / bp) + threadIdx.x / bp;
-  L.inst = L.G->inst0 + L.b;
+  L.row = (vb - L.G.block0) * (blockDim.x / bp) + threadIdx.x / bp;
+  L.inst = L.G.inst0 + L.b;
   L.idx = L.row * bp + L.b;
-  L.valid = L.row < L.G->n && L.b < L.G->B;
+  L.valid = L.row < L.G.n && L.b < L.G.B;
   return L;
 }
 
@@ -273,51 +286,15 @@ __device__ __forceinline__ bool meta_shfl(const BGroup& G) {
   return NOSH_META_SHFL && G.B == G.bp && G.bp >= 4;
 }
 
-// FP32 rows of a full group (B == bp >= 4 lanes, all lanes active): lane q of
+// FP32 rows of a full group (B == bp >= 4 lanes, all lanes computing): the
+// shuffled row product over a whole row, NS slots per batch. Lane q < NS of
 // the row's lanes loads the metadata of slot k + q once and the others take
 // it by a lane shuffle, instead of every lane loading the same 12 bytes
-// (which costs the texture data path bp times the useful bytes).
-__device__ __forceinline__ void brow_shfl4(const BGroup& G, int32_t j, int32_t k, int32_t w,
-                                           int b, float mu, const float2* __restrict__ x,
-                                           float& ar, float& ai) {
-  const int lane = int(threadIdx.x & 63);
-  const int g0 = lane & ~(G.bp - 1);
-  SlotMeta mine = {0, 0.0f, 0.0f};
-  if (b < 4 && k + b < w) mine = G.s32[j + b * kSliceH];
-  int32_t c[4];
-  float av[4], tv[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    c[q] = __shfl(mine.col, g0 + q, 64);
-    av[q] = __shfl(mine.alpha, g0 + q, 64);
-    tv[q] = __shfl(mine.at, g0 + q, 64);
-  }
-  float2 xv[4];
-  const int bp = G.bp;
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-    if (k + q < w) xv[q] = x[int64_t(c[q]) * bp + b];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    if (k + q < w) {
-      float vr, vi;
-      slot_value(av[q], tv[q], mu, vr, vi);
-      cmac(ar, ai, vr, vi, xv[q]);
-    }
-  }
-}
-
-#ifndef NOSH_META_PIPE
-#define NOSH_META_PIPE 1
-#endif
-
-// The shuffled row product over a whole row, NS slots per batch (lanes
-// b < NS of the row each load one slot's metadata; NS <= bp; 8 measured
-// slower than 4 at 96^3 x 8 lanes: 178 vs 164 us, occupancy 6 instead of 8),
-// and software-pipelined: the metadata load of the next batch is issued right
-// after the x gathers of the current one, so the two dependent memory
-// latencies of consecutive batches overlap. Same arithmetic in the same
-// order as the unpipelined loop.
+// (NS <= bp; 8 measured slower than 4 at 96^3 x 8 lanes: 178 vs 164 us,
+// occupancy 6 instead of 8). Software-pipelined: the metadata load of the
+// next batch is issued right after the x gathers of the current one, so the
+// two dependent memory latencies of consecutive batches overlap (169 -> 164
+// us against the unpipelined loop, same arithmetic in the same order).
 template <int NS>
 __device__ __forceinline__ void brow_shfl_row(const BGroup& G, int32_t j, int32_t w, int b,
                                               float mu, const float2* __restrict__ x, float& ar,
@@ -401,10 +378,8 @@ __device__ __forceinline__ typename Vt<T>::v2 brow(const BGroup& G, int64_t row,
     if (meta_shfl(G)) {
       if (G.kval)
         brow_shfl_val<4>(G, j, w, b, x, ar, ai);
-      else if (NOSH_META_PIPE)
-        brow_shfl_row<4>(G, j, w, b, mu, x, ar, ai);
       else
-        for (; k < w; k += 4, j += 4 * kSliceH) brow_shfl4(G, j, k, w, b, mu, x, ar, ai);
+        brow_shfl_row<4>(G, j, w, b, mu, x, ar, ai);
       typename Vt<T>::v2 r;
       r.x = ar;
       r.y = ai;
@@ -428,7 +403,7 @@ __device__ double ph_compute_f(const BatchArgs& a, int64_t vb) {
   const Lane L = lane_ctx(a, vb);
   double s2 = 0.0;
   if (L.valid) {
-    const BGroup& G = *L.G;
+    const BGroup& G = L.G;
     const double mu = a.mu[L.inst], g = a.g[L.inst];
     const double2 acc = brow<double>(G, L.row, L.b, mu, G.psi);
     const double2 xi = G.psi[L.idx];
@@ -446,7 +421,7 @@ __device__ void ph_rebuild(const BatchArgs& a, int64_t vb) {
   __shared__ double rmax[1024];
   rmax[threadIdx.x] = 0.0;
   if (L.valid) {
-  const BGroup& G = *L.G;
+  const BGroup& G = L.G;
   const int64_t nb3 = G.n * G.bp;
   const double g = a.g[L.inst];
   const double2 p = G.psi[L.idx];
@@ -473,11 +448,11 @@ __device__ void ph_rebuild(const BatchArgs& a, int64_t vb) {
   }
   __syncthreads();
   // One atomic per (block, instance): the max over the block's rows.
-  const int bp = L.G->bp;
-  if (threadIdx.x < bp && threadIdx.x < L.G->B) {
+  const int bp = L.G.bp;
+  if (threadIdx.x < bp && threadIdx.x < L.G.B) {
     double m = 0.0;
     for (int r = 0; r < int(blockDim.x) / bp; ++r) m = fmax(m, rmax[r * bp + threadIdx.x]);
-    atomicMax(reinterpret_cast<unsigned long long*>(a.bound) + L.G->inst0 + threadIdx.x,
+    atomicMax(reinterpret_cast<unsigned long long*>(a.bound) + L.G.inst0 + threadIdx.x,
               static_cast<unsigned long long>(__double_as_longlong(m)));
   }
   // rmax is rewritten by the next row tile of a folded block.
@@ -487,7 +462,7 @@ __device__ void ph_rebuild(const BatchArgs& a, int64_t vb) {
 __device__ void ph_setup(const BatchArgs& a, int64_t vb) {
   const Lane L = lane_ctx(a, vb);
   if (!L.valid) return;
-  const BGroup& G = *L.G;
+  const BGroup& G = L.G;
   const double2 fv = G.f[L.idx];
   const float2 r = make_float2(float(-fv.x), float(-fv.y));
   G.R0[L.idx] = r;
@@ -546,20 +521,6 @@ __device__ __forceinline__ int64_t tile_stride(const BatchArgs& a, const BGroup&
 __device__ __forceinline__ int64_t tile_vb(const BatchArgs& a, int64_t rb, int t) {
   const BGroup& G = a.groups[group_of(a, rb * a.fold)];
   return G.block0 + first_tile(a, G, rb * a.fold) + int64_t(t) * tile_stride(a, G);
-}
-
-// The record of the group holding virtual block v0, read through the
-// constant address space at a wave-uniform index: the record is never
-// written by a kernel, and pointers loaded from constant memory are known to
-// address global memory, so the row loops use global (not flat) loads.
-__device__ __forceinline__ BGroup load_group(const BatchArgs& a, int64_t v0) {
-  const int gi = __builtin_amdgcn_readfirstlane(group_of(a, v0));
-#if __HIP_DEVICE_COMPILE__
-  typedef const __attribute__((address_space(4))) BGroup* ConstGroups;
-  return ((ConstGroups)a.groups)[gi];
-#else
-  return a.groups[gi];  // host pass only parses device functions
-#endif
 }
 
 __device__ __forceinline__ Ctx make_ctx(const BatchArgs& a, int64_t v0) {
@@ -1035,7 +996,7 @@ __device__ void rows_minres_a_v(const BatchArgs& a, const VCtx<L>& c, double (&s
 __device__ void ph_flush(const BatchArgs& a, int64_t vb) {
   const Lane L = lane_ctx(a, vb);
   if (!L.valid || !a.active[L.inst]) return;
-  const BGroup& G = *L.G;
+  const BGroup& G = L.G;
   const MinresScalars& S = a.S[L.inst];
   if (S.pending) w_update(G, wupd_of(S), S.it, L.idx, vbuf(G, S.it - 1)[L.idx]);
   const double2 d = G.x[L.idx];
