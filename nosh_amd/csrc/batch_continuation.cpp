@@ -378,18 +378,24 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
         sigma = std::sqrt(ss / double(th.size()));
         if (!(sigma > 0.0)) sigma = 1.0;
       }
+      // Every step's minibatch drawn up front (the same draws, in the same
+      // order, as one step at a time) and uploaded once, and the loss read
+      // back only after the last step: the steps then queue on the stream
+      // without a host round trip each.
       std::uniform_int_distribution<int64_t> pick(0, int64_t(P) * xrows - 1);
-      double lval = 0.0;
+      const size_t nbat = size_t(so->batch);
+      idx_h.resize(nbat * size_t(std::max(nsteps, 1)));
+      for (auto& v : idx_h) v = pick(rng);
+      if (idx.n < idx_h.size()) idx.alloc(idx_h.size());
+      idx.upload(idx_h.data(), idx_h.size(), s);
       for (int it = 0; it < nsteps; ++it) {
-        for (auto& v : idx_h) v = pick(rng);
-        idx.upload(idx_h.data(), idx_h.size(), s);
-        launch_gather_rows(Xs.p, idx.p, so->batch, nin, 1.0f, xb.p, s);
-        launch_gather_rows(Ts.p, idx.p, so->batch, 2, float(1.0 / sigma), tb.p, s);
-        lval = net->gradients(xb.p, tb.p, so->batch, s);
+        const int64_t* bi = idx.p + size_t(it) * nbat;
+        launch_gather_rows(Xs.p, bi, so->batch, nin, 1.0f, xb.p, s);
+        launch_gather_rows(Ts.p, bi, so->batch, 2, float(1.0 / sigma), tb.p, s);
+        net->gradients(xb.p, tb.p, so->batch, s, false);
         net->adam(so->lr, s);
       }
-      NOSH_HIP(hipStreamSynchronize(s));
-      return lval;
+      return nsteps > 0 ? net->last_loss(s) : 0.0;
     };
 
     // Step 0: solve every branch at its mu_0.

@@ -19,6 +19,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <limits>
 #include <vector>
 
 #include "kernels.hpp"
@@ -562,7 +563,14 @@ void Mlp::infer(const float* X, int64_t M, float* Y, hipStream_t s) {
     k_mlp_infer<2><<<nb, 256, 0, s>>>(g);
 }
 
-double Mlp::gradients(const float* X, const float* T, int64_t M, hipStream_t s) {
+double Mlp::last_loss(hipStream_t s) {
+  double lh = 0.0;
+  NOSH_HIP(hipMemcpyAsync(&lh, loss.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  NOSH_HIP(hipStreamSynchronize(s));
+  return lh;
+}
+
+double Mlp::gradients(const float* X, const float* T, int64_t M, hipStream_t s, bool read_loss) {
   ensure(M);
   forward(X, M, y.p, s);
   const int L = int(dims.size()) - 1;
@@ -570,12 +578,20 @@ double Mlp::gradients(const float* X, const float* T, int64_t M, hipStream_t s) 
   const unsigned nbl = g1(len);
   k_loss<<<nbl, 256, 0, s>>>(y.p, T, len, 1.0f / float(len), dy.p, partials.p);
   k_finalize1<<<1, 256, 0, s>>>(partials.p, int(nbl), 1.0 / double(len), loss.p);
-  // Row chunks for the deterministic reductions over the batch.
-  const int64_t chunk = 1024;
+  // Row chunks for the deterministic reductions over the batch (column sums
+  // and the small first/last-layer weight gradients): 32 rows (at most 4096
+  // chunks), so that a batch of 8 k rows gives 256 chunks of parallel work
+  // (1024-row chunks took 240 us per reduction at 8 k rows, 1.6 of the 2 ms
+  // of an Adam step).
+  const int64_t chunk = std::max<int64_t>(32, (M + 4095) / 4096);
   const int64_t nchunk = (M + chunk - 1) / chunk;
+  // Split-K of the hidden layers' weight-gradient GEMMs: 1024 rows a split.
+  const int64_t nsplit = (M + 1023) / 1024;
   // Scratch for the partial sums: split-K GEMM partials (<= 64 x width^2),
-  // per-chunk column sums and small weight gradients (nchunk x width x 64).
-  const int64_t need = std::max<int64_t>(64, nchunk) * int64_t(width) * std::max(width, 64);
+  // per-chunk column sums (nchunk x width) and small weight gradients
+  // (nchunk x width x max(n_in, n_out)).
+  const int64_t need = std::max<int64_t>(64 * int64_t(width) * width,
+                                         nchunk * int64_t(width) * std::max(nin, nout));
   if (red.n < size_t(need)) red.alloc(need);
   auto wg = [&](int l) { return grad.p + woff[l]; };
   auto bg = [&](int l) { return grad.p + nweights + boff[l]; };
@@ -597,7 +613,7 @@ double Mlp::gradients(const float* X, const float* T, int64_t M, hipStream_t s) 
   __bf16* dzn = dza.p;
   for (int l = L - 2; l >= 1; --l) {
     const int64_t N = dims[l + 1], K = dims[l];
-    const int splits = int(std::min<int64_t>(nchunk, 64));
+    const int splits = int(std::min<int64_t>(nsplit, 64));
     // C[n][k] = sum_m dz[m][n] h[m][k]: A(n,m) = dz[m*N + n], B(m,k) = h[m*K + k].
     gemm(dz, 1, N, hbuf[l - 1].p, K, 1, N, K, M, 2, nullptr, nullptr, red.p, nullptr, splits,
          s);
@@ -624,10 +640,7 @@ double Mlp::gradients(const float* X, const float* T, int64_t M, hipStream_t s) 
     k_colsum_chunks<float><<<gb, 256, 0, s>>>(zbuf[0].p, M, N, chunk, red.p);
     k_sum_splits<<<g1(N), 256, 0, s>>>(red.p, int(nchunk), N, bg(0));
   }
-  double lh = 0.0;
-  NOSH_HIP(hipMemcpyAsync(&lh, loss.p, sizeof(double), hipMemcpyDeviceToHost, s));
-  NOSH_HIP(hipStreamSynchronize(s));
-  return lh;
+  return read_loss ? last_loss(s) : std::numeric_limits<double>::quiet_NaN();
 }
 
 void Mlp::adam(float lr, hipStream_t s) {

@@ -37,8 +37,11 @@ struct Mlp {
   // width 64, 128 or 256 and n_in <= 32, else forward() in row chunks.
   void infer(const float* X, int64_t M, float* Y, hipStream_t s);
   // Fills grad (same layout as the parameters) for the mean squared error
-  // of the outputs against T; returns the loss.
-  double gradients(const float* X, const float* T, int64_t M, hipStream_t s);
+  // of the outputs against T; returns the loss (with read_loss false it
+  // returns NaN without waiting for the stream, and last_loss reads it later).
+  double gradients(const float* X, const float* T, int64_t M, hipStream_t s,
+                   bool read_loss = true);
+  double last_loss(hipStream_t s);
   void adam(float lr, hipStream_t s);
   void sync_bf16(hipStream_t s);
 };
