@@ -370,12 +370,7 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
     // call also fixes the output scale sigma (RMS of the targets).
     auto train = [&](int nsteps) {
       if (!trained) {
-        std::vector<float> th(size_t(P) * xrows * 2);
-        Ts.download(th.data(), th.size(), s);
-        NOSH_HIP(hipStreamSynchronize(s));
-        double ss = 0.0;
-        for (float v : th) ss += double(v) * v;
-        sigma = std::sqrt(ss / double(th.size()));
+        sigma = rms_f32(Ts.p, int64_t(P) * xrows * 2, s);
         if (!(sigma > 0.0)) sigma = 1.0;
       }
       // Every step's minibatch drawn up front (the same draws, in the same

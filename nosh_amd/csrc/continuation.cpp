@@ -611,12 +611,7 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int np, const c
         ++pairs;
         if (!trained && pairs == P) {
           const auto tt = std::chrono::steady_clock::now();
-          std::vector<float> th(int64_t(P) * n * 2);
-          Ts.download(th.data(), th.size(), s);
-          NOSH_HIP(hipStreamSynchronize(s));
-          double ss = 0.0;
-          for (float v : th) ss += double(v) * v;
-          sigma = std::sqrt(ss / double(th.size()));
+          sigma = rms_f32(Ts.p, int64_t(P) * n * 2, s);
           if (!(sigma > 0.0)) sigma = 1.0;
           sst.final_loss = train_steps(so->train_steps);
           trained = true;

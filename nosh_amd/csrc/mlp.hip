@@ -666,7 +666,30 @@ __global__ void k_gather_rows(const float* __restrict__ src, const int64_t* __re
   const int64_t r = i / w;
   dst[i] = src[idx[r] * w + (i - r * w)] * scale;
 }
+
+__global__ void k_sumsq_f32(const float* __restrict__ x, int64_t n, double* __restrict__ part) {
+  double v[1] = {0.0};
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double t = x[i];
+    v[0] += t * t;
+  }
+  block_partials<1>(v, part);
+}
 }  // namespace
+
+double rms_f32(const float* x, int64_t n, hipStream_t s) {
+  if (n <= 0) return 0.0;
+  const int nb = int(std::min<int64_t>(1024, grid_for(n, 256)));
+  DBuf<double> part{size_t(nb)};
+  k_sumsq_f32<<<nb, 256, 0, s>>>(x, n, part.p);
+  std::vector<double> h(static_cast<size_t>(nb));
+  part.download(h.data(), h.size(), s);
+  NOSH_HIP(hipStreamSynchronize(s));
+  double ss = 0.0;
+  for (double v : h) ss += v;
+  return std::sqrt(ss / double(n));
+}
 
 void launch_gather_rows(const float* src, const int64_t* idx, int64_t B, int w, float scale,
                         float* dst, hipStream_t s) {

@@ -46,6 +46,9 @@ struct Mlp {
   void sync_bf16(hipStream_t s);
 };
 
+// Root mean square of n FP32 values on the device (deterministic: block
+// partial sums in FP64 added in block order on the host).
+double rms_f32(const float* x, int64_t n, hipStream_t s);
 // dst[r][:] = scale * src[idx[r]][:] for r < B rows of width w (minibatch gather).
 void launch_gather_rows(const float* src, const int64_t* idx, int64_t B, int w, float scale,
                         float* dst, hipStream_t s);
