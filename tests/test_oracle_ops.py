@@ -235,6 +235,46 @@ def test_chebyshev_matches_polynomial():
         assert n2 == r.size
 
 
+def test_three_term_chebyshev_equals_closed_form():
+    """The batched kernels' three-term form of the Chebyshev preconditioner
+    (no direction vector: z_0 = D^-1 r / theta, z_j = z_{j-1} + c1 (z_{j-1} -
+    z_{j-2}) + c2 D^-1 (r - P z_{j-1}) with z_{-1} = 0, c1 = rho_j rho_{j-1},
+    c2 = 2 rho_j / delta, rho_0 = 1/sigma, rho_j = 1/(2 sigma - rho_{j-1}))
+    restated here step by step, against the closed-form Chebyshev polynomial
+    of D^-1 P and the oracle's direction-vector form."""
+    coords, cells = mg.square_tri(4)
+    m = O.Mesh(coords, cells)
+    md = O.Model(m, mg.magnetic_potential(coords))
+    md.set_params(1.0, 0.3)
+    md.prec_rebuild(mg.random_state(m.n, 0))
+    P = md.dense(2)
+    D = np.zeros_like(P)
+    for k in range(m.n):
+        D[2 * k:2 * k + 2, 2 * k:2 * k + 2] = P[2 * k:2 * k + 2, 2 * k:2 * k + 2]
+    Dinv = np.linalg.inv(D)
+    w, Q = np.linalg.eigh(D)
+    Dm12 = Q @ np.diag(w ** -0.5) @ Q.T
+    lam, V = np.linalg.eigh(Dm12 @ P @ Dm12)
+    hi = md.cheb_bound()
+    r = mg.random_state(m.n, 5)
+    for iters, ratio in ((2, 30.0), (4, 300.0), (5, 300.0)):
+        lo = hi / ratio
+        theta, delta = (hi + lo) / 2, (hi - lo) / 2
+        sigma = theta / delta
+        z_prev, z = np.zeros_like(r), Dinv @ r / theta
+        rho = 1.0 / sigma
+        for _ in range(1, iters):
+            rho_new = 1.0 / (2.0 * sigma - rho)
+            c1, c2 = rho_new * rho, 2.0 * rho_new / delta
+            z_prev, z = z, z + c1 * (z - z_prev) + c2 * (Dinv @ (r - P @ z))
+            rho = rho_new
+        Tk = np.polynomial.chebyshev.Chebyshev.basis(iters)
+        R = Tk((theta - lam) / delta) / Tk(theta / delta)
+        expect = Dm12 @ (V @ np.diag((1 - R) / lam) @ V.T) @ Dm12 @ r
+        assert rel_err(z, expect) < 1e-10, iters
+        assert rel_err(z, md.cheb(r, iters, ratio)) < 1e-12, iters
+
+
 def test_minres_poisson_against_dense_solve():
     """MINRES on the (-1, 2, -1) matrix of test/linear-solve-test/linop-tester.cpp
     against numpy's dense solve."""
