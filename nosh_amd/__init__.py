@@ -730,6 +730,10 @@ class Batch(_Handle):
         return out
 
     def set_tuning(self, block_threads=256, mode=0, graph_iters=4, poll=16, fold=1):
+        """mode 0: hipGraph iterations; 1: persistent cooperative kernel (grids of at
+        most 64 blocks); 2: hipGraph with two lanes per thread in the gathering kernels
+        (every group at least 2 lanes). All modes give the same results bit for bit;
+        an unsupported mode raises NoshError and leaves the batch in mode 0."""
         _check(_lib.nosh_batch_set_tuning(self._h, block_threads, fold, mode, graph_iters,
                                           poll))
 
