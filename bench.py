@@ -77,7 +77,9 @@ def parse():
     p.add_argument("--spmv-reps", type=int, default=50)
     p.add_argument("--variants", default="plain,mixed")
     p.add_argument("--roofline-only", action="store_true",
-                   help="only the J.phi timing loop and a copy kernel (for profilers)")
+                   help="only the roofline measurements (for profilers)")
+    p.add_argument("--no-roofline-160", dest="roofline_160", action="store_false",
+                   help="skip the 160^3 J.phi timing")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--verbose", action="store_true")
     p.add_argument("--cont-branches", type=int, default=8)
@@ -88,12 +90,16 @@ def parse():
     p.add_argument("--cont-batch", type=int, default=8192, help="surrogate minibatch rows")
     p.add_argument("--cont-train-steps", type=int, default=150,
                    help="Adam steps of the surrogate's first training")
-    p.add_argument("--cont-plain-branches", type=int, default=1,
+    p.add_argument("--cont-plain-branches", type=int, default=8,
                    help="branches run with the plain FP64 continuation for the ratio")
+    p.add_argument("--no-ablation", action="store_true",
+                   help="skip the single-instance ablation rows")
     p.add_argument("--cont-tuning", default="",
                    help="block_threads,fold[,mode] for the batched continuation (skips the "
                         "autotuner; mode 0 by default)")
     p.add_argument("--no-surrogate", action="store_true")
+    p.add_argument("--cont-smooth", type=int, default=4,
+                   help="Jacobi sweeps on the residual of each surrogate guess")
     p.add_argument("--workload", default="cont96",
                    choices=["cont96", "batched", "cont", "disk", "stability"],
                    help="cont96: the headline (continuation, side-by-side solves, roofline, "
@@ -128,9 +134,15 @@ def _profiled_traffic(ncell):
     return None, None
 
 
-def cpu_baseline(coords, cells, A, mu, psi0, phi, counts, cheb, ratio, what):
-    """Time the oracle on this host; extrapolate one solve from the GPU
-    solver's per-solve counts (Newton steps and MINRES iterations)."""
+def cpu_baseline(coords, cells, A, mu, psi0, phi, counts, cheb, ratio, what, model=None):
+    """The oracle on this host's cores: one full FP64 Newton-MINRES solve at
+    the benchmark's size (the first continuation step of branch g = 1: mu =
+    `mu`, from psi = 1, ||F|| <= 1e-10, MINRES rtol 1e-10, Chebyshev(cheb)),
+    timed end to end; ``value`` = 1 / its seconds. The GPU's plain FP64 solve
+    of the same problem is timed beside it. Also reported: medians of 20 of
+    F, K phi and J phi, and the per-solve cost extrapolated from per-operation
+    medians and the plain GPU continuation's own per-solve counts."""
+    import nosh_amd as nosh
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
@@ -166,17 +178,42 @@ def cpu_baseline(coords, cells, A, mu, psi0, phi, counts, cheb, ratio, what):
     t_it = (time.perf_counter() - s) / max(k, 1)
     steps = counts["newton_iters"]
     its = counts["minres_iters"]
-    t_solve = (steps + 1) * t_f + steps * t_rb + its * t_it
+    t_extrap = (steps + 1) * t_f + steps * t_rb + its * t_it
+    # The full solve, CPU and GPU.
+    n = coords.shape[0]
+    start = np.zeros(2 * n)
+    start[0::2] = 1.0
+    s = time.perf_counter()
+    pc, it_c, nf_c, mh_c = om.newton(start, 1.0, mu, tol=1e-10, maxit=40, minres_rtol=1e-10,
+                                     minres_maxit=4000, cheb_iters=cheb, ratio=ratio)
+    t_cpu = time.perf_counter() - s
+    gpu = None
+    if model is not None:
+        kw = dict(tol=1e-10, max_iters=40, minres_rtol=1e-10, minres_max_iters=4000,
+                  cheb_iters=cheb, cheb_ratio=ratio)
+        nosh.newton_solve(model, start, {"g": 1.0, "mu": mu}, **kw)
+        s = time.perf_counter()
+        pg, sg = nosh.newton_solve(model, start, {"g": 1.0, "mu": mu}, **kw)
+        gpu = {"seconds": time.perf_counter() - s, "newton": sg["newton_iters"],
+               "minres": sg["minres_iters_total"],
+               "rel_diff_vs_cpu": float(np.linalg.norm(_phase_align(pg, pc) - pc) /
+                                        np.linalg.norm(pc))}
     sample = (f"oracle on {threads} threads ({_cpu_model()}, nproc {os.cpu_count()}, "
-              f"OMP_NUM_THREADS {os.environ.get('OMP_NUM_THREADS', 'unset')}), 96^3 cube, "
-              f"medians of 20: F {t_f*1e3:.1f} ms, K.phi {t_k*1e3:.1f} ms, J.phi "
-              f"{t_j*1e3:.1f} ms (1 thread: {t_j1*1e3:.1f} ms), rebuild {t_rb*1e3:.1f} ms, MINRES "
-              f"iteration {t_it*1e3:.1f} ms (mean of {k} with Chebyshev {cheb}); one solve = "
-              f"{steps + 1:.2f} F + {steps:.2f} rebuilds + {its:.1f} MINRES iterations, the "
-              f"{what}'s per-solve counts; setup {setup:.1f} s")
-    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": sample, "jphi_ms": t_j * 1e3, "jphi_ms_1thread": t_j1 * 1e3,
-            "f_ms": t_f * 1e3, "keo_ms": t_k * 1e3, "seconds_per_solve": t_solve,
+              f"OMP_NUM_THREADS {os.environ.get('OMP_NUM_THREADS', 'unset')}): one full FP64 "
+              f"Newton-MINRES solve of the 96^3 cube at mu = {mu}, g = 1 from psi = 1 "
+              f"(Chebyshev({cheb}) in place of MueLu AMG), {int(it_c)} Newton steps, "
+              f"{int(np.sum(mh_c))} MINRES iterations, {t_cpu:.1f} s; setup {setup:.1f} s")
+    say("cpu baseline: " + sample + (f"; GPU plain FP64 {gpu['seconds']:.3f} s" if gpu else ""))
+    return {"value": 1.0 / t_cpu if it_c >= 0 else 0.0, "unit": "solves/s", "cores": threads,
+            "kind": "port", "sample": sample, "seconds_per_solve": t_cpu,
+            "cpu_newton": int(it_c), "cpu_minres": int(np.sum(mh_c)),
+            "cpu_final_norm_f": float(nf_c[-1]) if len(nf_c) else None,
+            "gpu_plain_same_solve": gpu,
+            "jphi_ms": t_j * 1e3, "jphi_ms_1thread": t_j1 * 1e3, "f_ms": t_f * 1e3,
+            "keo_ms": t_k * 1e3, "rebuild_ms": t_rb * 1e3, "minres_iter_ms": t_it * 1e3,
+            "extrapolated_continuation_solve_s": t_extrap,
+            "extrapolation_note": f"{steps + 1:.2f} F + {steps:.2f} rebuilds + {its:.1f} MINRES "
+                                  f"iterations, the {what}'s mean per-solve counts",
             "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
 
 
@@ -247,11 +284,64 @@ def build_model(ncell):
     return coords, cells, A, mesh, model, t_mesh
 
 
+def _plain_continuation(model, n, psi0, g, mu0, span, steps, branches, args, **extra):
+    """Single-instance continuation of the given branches (FP64 Newton-MINRES
+    rtol 1e-10 and the tangent predictor unless `extra` changes the Newton
+    options); returns (solves/s, per-branch records)."""
+    import nosh_amd as nosh
+    t_all, recs = 0.0, []
+    for b in branches:
+        kw = dict(tol=1e-10, max_iters=40, minres_rtol=1e-10, minres_max_iters=args.minres_maxit,
+                  cheb_iters=args.cheb, cheb_ratio=args.ratio)
+        kw.update(extra)
+        t1 = time.perf_counter()
+        _, r1 = nosh.continuation(model, psi0[2 * n * b:2 * n * (b + 1)], {"g": g[b]}, "mu",
+                                  mu0[b], mu0[b] + span, steps, predictor=1, **kw)
+        dt = time.perf_counter() - t1
+        t_all += dt
+        ok = len(r1) == steps + 1 and all(r["converged"] for r in r1)
+        recs.append({"branch": b, "seconds": round(dt, 3), "converged": ok,
+                     "newton_iters": sum(r["newton_iters"] for r in r1[1:]),
+                     "minres_iters": sum(r["minres_iters"] for r in r1[1:]),
+                     "halvings": sum(r["halvings"] for r in r1[1:]),
+                     "per_step_seconds": [round(r["seconds"], 4) for r in r1[1:]]})
+        say(f"continuation: single-instance {extra or 'plain FP64'} branch {b}: {dt:.2f}s, "
+            f"newton {recs[-1]['newton_iters']}, converged {ok}")
+    ok = all(r["converged"] for r in recs)
+    return (len(branches) * steps / t_all if ok and t_all > 0 else 0.0), recs
+
+
+def _batched_run(batch, psi0, span, steps, kw, sur):
+    """One batched continuation; returns (seconds, records, surrogate stats, psi)."""
+    t0 = time.perf_counter()
+    psi, rec, sst = batch.continuation(psi0, span, steps, surrogate=sur, **kw)
+    return time.perf_counter() - t0, rec, sst, psi
+
+
+def _run_summary(nb, steps, dt, rec):
+    conv = len(rec) == steps + 1 and all(r["converged"] == nb for r in rec)
+    per = [r["seconds"] for r in rec[1:]]
+    return {"seconds": round(dt, 3), "solves_per_s": (nb * steps / dt) if conv else 0.0,
+            "converged": conv, "steps": steps,
+            "lockstep_newton_iters": sum(r["newton_iters"] for r in rec),
+            "lockstep_minres_iters": sum(r["minres_iters"] for r in rec),
+            "max_norm_f": max(r["max_norm_f"] for r in rec),
+            "largest_step_share": round(max(per) / max(sum(per), 1e-12), 4),
+            "per_step_seconds": [round(r["seconds"], 4) for r in rec],
+            "per_step_newton": [r["newton_iters"] for r in rec]}
+
+
 def continuation_phase(args, model, n):
-    """mu-continuation of several (mu_0, g) branches on one mesh: the batched,
-    mixed-precision, surrogate-seeded driver (timed over args.steps steps
-    after args.warmup untimed ones) against the plain FP64 single-instance
-    tangent continuation of a sample of the branches."""
+    """mu-continuation of 8 (mu_0 = 0, g) branches on one mesh.
+
+    Timed (the headline ``value``): the batched, mixed-precision,
+    surrogate-seeded driver over args.steps steps of mu (after args.warmup
+    untimed steps). Beside it, untimed: the same run at the other step size
+    of the workload (50 steps when the headline has 20, else 20); the plain
+    FP64 single-instance tangent continuation of every branch at the headline
+    step size (the ratio's denominator) and of two branches at the other; an
+    ablation at the headline step size that adds one change at a time (FP64
+    with Eisenstat-Walker forcing; FP32 inner MINRES; batching; surrogate)."""
     import nosh_amd as nosh
     nb, span, steps = args.cont_branches, args.cont_span, args.steps
     g = 0.6 + 0.2 * np.arange(nb)
@@ -275,18 +365,20 @@ def continuation_phase(args, model, n):
     say(f"continuation: autotune {t_tune:.2f}s -> {tuned}")
     sur = None if args.no_surrogate else dict(
         width=256, depth=4, train_after=args.cont_train_after, train_steps=args.cont_train_steps,
-        batch=args.cont_batch, lr=1e-3, newton_budget=6, retrain_steps=args.cont_retrain)
+        batch=args.cont_batch, lr=1e-3, newton_budget=6, retrain_steps=args.cont_retrain,
+        smooth_sweeps=args.cont_smooth)
     if args.warmup > 0:
         t0 = time.perf_counter()
         batch.continuation(psi0, span * args.warmup / steps, args.warmup, **kw)
         say(f"continuation: warmup {args.warmup} steps {time.perf_counter() - t0:.2f}s")
-    t0 = time.perf_counter()
-    psi, rec, sst = batch.continuation(psi0, span, steps, surrogate=sur, **kw)
-    t_comb = time.perf_counter() - t0
-    conv = len(rec) == steps + 1 and all(r["converged"] == nb for r in rec)
-    comb_rate = nb * steps / t_comb
+    # ---- timed region
+    t_comb, rec, sst, psi = _batched_run(batch, psi0, span, steps, kw, sur)
+    # ----
+    head = _run_summary(nb, steps, t_comb, rec)
+    comb_rate = head["solves_per_s"]
     say(f"continuation: combined {t_comb:.2f}s for {nb}x{steps} solves -> {comb_rate:.3f}/s, "
-        f"converged {conv}, surrogate {sst}")
+        f"converged {head['converged']}, largest step {head['largest_step_share']:.1%}, "
+        f"surrogate {sst}")
     # Roofline of the timed region's dominant kernel (the batched Chebyshev
     # step), every branch active, at the final states, timed with HIP events
     # on the library stream.
@@ -314,34 +406,49 @@ def continuation_phase(args, model, n):
         fp64_err.append(float(np.linalg.norm(_phase_align(finals[b], p) - p) /
                               np.linalg.norm(p)))
     say(f"continuation: mixed vs FP64-polished final states, rel. err. max {max(fp64_err):.2e}")
-    t_plain, plain_steps, plain_rec = 0.0, 0, []
-    for b in range(min(args.cont_plain_branches, nb)):
-        t1 = time.perf_counter()
-        p, r1 = nosh.continuation(model, psi0[2 * n * b:2 * n * (b + 1)], {"g": g[b]}, "mu",
-                                  mu0[b], mu0[b] + span, steps, predictor=1, tol=1e-10,
-                                  max_iters=40, minres_rtol=1e-10,
-                                  minres_max_iters=args.minres_maxit, cheb_iters=args.cheb,
-                                  cheb_ratio=args.ratio)
-        dt = time.perf_counter() - t1
-        t_plain += dt
-        plain_steps += steps
-        ok = len(r1) == steps + 1 and all(r["converged"] for r in r1)
-        plain_rec.append({"branch": b, "seconds": round(dt, 3), "converged": ok,
-                          "newton_iters": sum(r["newton_iters"] for r in r1[1:]),
-                          "minres_iters": sum(r["minres_iters"] for r in r1[1:]),
-                          "halvings": sum(r["halvings"] for r in r1[1:]),
-                          "final_gibbs": r1[-1]["gibbs"]})
-        say(f"continuation: plain branch {b}: {dt:.2f}s, {plain_rec[-1]}")
-    plain_rate = plain_steps / t_plain if t_plain > 0 else None
+    # The batched run without the surrogate (the ablation's batching row, and
+    # the surrogate's net gain in this same process).
+    t_ns, rec_ns, _, _ = _batched_run(batch, psi0, span, steps, kw, None)
+    no_sur = _run_summary(nb, steps, t_ns, rec_ns)
+    say(f"continuation: batched without surrogate {t_ns:.2f}s -> {no_sur['solves_per_s']:.3f}/s")
+    # The other step size of the workload.
+    other_steps = 50 if steps != 50 else 20
+    t_o, rec_o, sst_o, _ = _batched_run(batch, psi0, span, other_steps, kw, sur)
+    other = _run_summary(nb, other_steps, t_o, rec_o)
+    other["surrogate"] = sst_o
+    say(f"continuation: combined at {other_steps} steps {t_o:.2f}s -> "
+        f"{other['solves_per_s']:.3f}/s")
+    del batch
+    # Plain FP64 single-instance continuation of every branch.
+    branches = list(range(min(args.cont_plain_branches, nb)))
+    plain_rate, plain_rec = _plain_continuation(model, n, psi0, g, mu0, span, steps, branches,
+                                                args) if branches else (None, [])
+    two = sorted({0, nb - 1})
+    plain_o_rate, plain_o_rec = _plain_continuation(model, n, psi0, g, mu0, span, other_steps,
+                                                    two, args)
+    other["plain_fp64_solves_per_s"] = plain_o_rate
+    other["plain_sample"] = plain_o_rec
+    other["speedup_vs_plain_fp64"] = (other["solves_per_s"] / plain_o_rate
+                                      if plain_o_rate else None)
+    # Ablation at the headline step size (single-instance rows on two branches).
+    abl = {"plain FP64 (tangent predictor, MINRES rtol 1e-10)": plain_rate}
+    if not args.no_ablation:
+        r_ew, _ = _plain_continuation(model, n, psi0, g, mu0, span, steps, two, args, forcing=1,
+                                      forcing_min=1e-6, forcing_max=args.cont_forcing_max)
+        abl["+ Eisenstat-Walker forcing (FP64 inner)"] = r_ew
+        r_32, _ = _plain_continuation(model, n, psi0, g, mu0, span, steps, two, args, forcing=1,
+                                      forcing_min=1e-6, forcing_max=args.cont_forcing_max,
+                                      precision=1)
+        abl["+ FP32 inner MINRES"] = r_32
+    abl["+ batching (8 branches in lockstep, secant predictor, Chebyshev "
+        f"{args.cont_cheb})"] = no_sur["solves_per_s"]
+    abl["+ surrogate (the headline)"] = comb_rate
     seeded = [r for r in rec if r["seeded"]]
-    return {
-        "solves_per_s": comb_rate, "seconds": t_comb, "converged": conv,
-        "branches": nb, "steps": steps, "span": span, "g": [round(v, 3) for v in g],
+    out = dict(head)
+    out.update({
+        "branches": nb, "span": span, "g": [round(v, 3) for v in g],
         "autotuned": {"block_threads": tuned[0], "fold": tuned[2],
                       "mode": MODE_NAMES[tuned[1]], "seconds": round(t_tune, 2)},
-        "lockstep_newton_iters": sum(r["newton_iters"] for r in rec),
-        "lockstep_minres_iters": sum(r["minres_iters"] for r in rec),
-        "max_norm_f": max(r["max_norm_f"] for r in rec),
         "rel_err_vs_fp64_polished": {"branches": sorted({0, nb - 1}), "values": fp64_err},
         "surrogate": sst,
         "seeded_branch_steps": sum(r["seeded"] for r in rec),
@@ -350,16 +457,77 @@ def continuation_phase(args, model, n):
                                          if seeded else None),
         "mean_secant_error_seeded_steps": (float(np.mean([r["sec_error"] for r in seeded]))
                                            if seeded else None),
-        "per_step_seconds": [round(r["seconds"], 4) for r in rec],
-        "per_step_newton": [r["newton_iters"] for r in rec],
+        "without_surrogate": no_sur,
+        "surrogate_net_gain": (comb_rate / no_sur["solves_per_s"]
+                               if no_sur["solves_per_s"] else None),
         "dominant_kernel_roofline": cheb_roof,
         "plain_fp64_solves_per_s": plain_rate, "plain_sample": plain_rec,
         "speedup_vs_plain_fp64": comb_rate / plain_rate if plain_rate else None,
-    }
+        f"at_{other_steps}_steps": other,
+        "ablation_solves_per_s": abl,
+    })
+    return out
 
 
-def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev, A):
-    """FP64 J(psi) phi timed on the library's launch stream (warm and cold)."""
+def _event_time(stream, fn, reps, warm=3):
+    """Average seconds per call of fn() over reps calls, HIP events on stream."""
+    import torch
+    with torch.cuda.stream(stream):
+        for _ in range(warm):
+            fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def _cold_time(stream, fn, reps, flush):
+    """Median seconds of fn() with a 1 GiB read (flush.sum()) before each
+    call: the read replaces the 256 MB Infinity Cache and the L2s with clean
+    lines, so fn streams its operands from HBM (a write-based flush would
+    leave dirty lines whose write-back the timed call would pay for)."""
+    import torch
+    ev = []
+    with torch.cuda.stream(stream):
+        for _ in range(reps):
+            flush.sum()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            ev.append((e0, e1))
+    torch.cuda.synchronize()
+    return float(np.median([a.elapsed_time(b) for a, b in ev])) / 1e3
+
+
+def stream_ceiling():
+    """Attainable HBM bandwidth from the library's own streaming kernels
+    (read, write, copy over 2 GiB buffers; non-temporal and ordinary
+    accesses; 8 workgroups of 256 threads per CU): the best GB/s of each."""
+    import nosh_amd as nosh
+    nbytes = 2 << 30
+    out = {}
+    for kind in ("read", "write", "copy"):
+        best = 0.0
+        for nt in (True, False):
+            t = nosh.bench_stream(kind, nbytes, reps=10, blocks_per_cu=8, nontemporal=nt)
+            moved = (2 if kind == "copy" else 1) * nbytes
+            best = max(best, moved / t / 1e9)
+        out[kind] = round(best, 1)
+    say(f"stream ceiling (own kernels, 2 GiB): read {out['read']:.0f}, write {out['write']:.0f}, "
+        f"copy {out['copy']:.0f} GB/s")
+    return out
+
+
+def jphi_timings(args, mesh, model, params, psi0, phi, stream, dev, flush, fp32=True):
+    """FP64 (and FP32) J(psi) phi on the library's launch stream: warm
+    (back-to-back) and cold (1 GiB read before each launch) average launch
+    times against the canonical bytes 40E + 68n (FP32: 24E + 36n)."""
     import torch
     E, n = mesh.num_edges, mesh.num_vertices
     J = model.create_W_op()
@@ -368,113 +536,117 @@ def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev, A):
     x = torch.empty_like(x_ref)
     y = torch.empty_like(x_ref)
     mesh.to_internal(x_ref.data_ptr(), x.data_ptr(), stream.cuda_stream)
-    with torch.cuda.stream(stream):
-        for _ in range(5):
-            J.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
-        ev0.record(stream)
-        for _ in range(args.spmv_reps):
-            J.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
-        ev1.record(stream)
-    ev1.synchronize()
-    t_j = ev0.elapsed_time(ev1) / 1e3 / args.spmv_reps
-    bytes_canon = 40 * E + 68 * n
-    ach = bytes_canon / t_j / 1e9
-    # Cold-cache variant: reading 1 GiB between launches replaces the 256 MB
-    # Infinity Cache and the L2s with clean lines, so every launch streams its
-    # operands from HBM (a write-based flush would leave dirty lines whose
-    # write-back the next launch would pay for).
+    f64 = lambda: J.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)  # noqa: E731
+    res = {"n": n, "E": E, "bytes64": 40 * E + 68 * n, "bytes32": 24 * E + 36 * n,
+           "warm64": _event_time(stream, f64, args.spmv_reps, warm=5),
+           "cold64": _cold_time(stream, f64, 20, flush)}
+    if fp32:
+        x32 = x.float()
+        y32 = torch.empty_like(x32)
+        f32 = lambda: J.apply_internal_fp32(x32.data_ptr(), y32.data_ptr(),  # noqa: E731
+                                            stream.cuda_stream)
+        f32()  # the FP32 block copy is made once per rebuild, here
+        res["warm32"] = _event_time(stream, f32, args.spmv_reps)
+        res["cold32"] = _cold_time(stream, f32, 20, flush)
+    return res, J, x, y
+
+
+def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev, A):
+    """The roofline of the Jacobian SpMV. Primary figure: the FP64 J(psi) phi
+    at 96^3 with cold caches (its 313 MB working set otherwise partly sits in
+    the 256 MB Infinity Cache), canonical bytes / average launch time against
+    8 TB/s and against the attainable ceiling of the library's streaming
+    kernels. Beside it: the warm (back-to-back) figure, FP32 J phi, K phi,
+    F(psi), and the 160^3 J phi (1.44 GB, no cache effects)."""
+    import torch
+    import nosh_amd as nosh
+    E, n = mesh.num_edges, mesh.num_vertices
     flush = torch.ones(1 << 27, dtype=torch.float64, device=dev)
-    cold = []
-    with torch.cuda.stream(stream):
-        for _ in range(10):
-            fsum = flush.sum()
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            J.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)
-            e1.record(stream)
-            cold.append((e0, e1))
-    torch.cuda.synchronize()
-    t_cold = float(np.median([a.elapsed_time(b) for a, b in cold])) / 1e3
-    del flush, fsum
-    layout = mesh.layout()
-    bytes_layout = 20 * layout["nnz_pad"] + 4 * (layout["nslices"] + 1) + 56 * n
-    # Copy-kernel reference of the same size class (stream bandwidth).
+    t, J, x, y = jphi_timings(args, mesh, model, params, psi0, phi, stream, dev, flush)
+    bytes_canon = t["bytes64"]
+    gbs = lambda b, s: b / s / 1e9  # noqa: E731
+    ceiling = stream_ceiling()
+    # Copy-kernel reference of the same size class (torch's copy_).
     cbuf = torch.empty(bytes_canon // 8, dtype=torch.float64, device=dev)
     cdst = torch.empty_like(cbuf)
-    with torch.cuda.stream(stream):
-        for _ in range(3):
-            cdst.copy_(cbuf)
-        ev0.record(stream)
-        for _ in range(20):
-            cdst.copy_(cbuf)
-        ev1.record(stream)
-    ev1.synchronize()
-    t_copy = ev0.elapsed_time(ev1) / 1e3 / 20
+    t_copy = _event_time(stream, lambda: cdst.copy_(cbuf), 20)
     copy_gbs = 2 * cbuf.numel() * 8 / t_copy / 1e9
     del cbuf, cdst
-    # FP32 J apply (inner solver's operator).
-    x32 = x.float()
-    y32 = torch.empty_like(x32)
-    with torch.cuda.stream(stream):
-        J.apply_internal_fp32(x32.data_ptr(), y32.data_ptr(), stream.cuda_stream)
-        ev0.record(stream)
-        for _ in range(args.spmv_reps):
-            J.apply_internal_fp32(x32.data_ptr(), y32.data_ptr(), stream.cuda_stream)
-        ev1.record(stream)
-    ev1.synchronize()
-    t_j32 = ev0.elapsed_time(ev1) / 1e3 / args.spmv_reps  # includes the block conversion
-    # K.phi and F(psi) (FP64, internal order), against the canonical bytes
-    # 40E + 44n and 40E + 52n.
-    import nosh_amd as nosh
+    layout = mesh.layout()
+    bytes_layout = 20 * layout["nnz_pad"] + 4 * (layout["nslices"] + 1) + 56 * n
+    # K.phi and F(psi) (FP64, internal order), canonical bytes 40E + 44n and
+    # 40E + 52n.
     others = {}
     keo = nosh.Keo(mesh, nosh.ScalarFieldConstant(mesh, 1.0),
                    nosh.VectorFieldExplicitValues(mesh, A, 0.0))
     keo.set_parameters(params)
-
-    def timed(fn):
-        with torch.cuda.stream(stream):
-            for _ in range(3):
-                fn()
-            ev0.record(stream)
-            for _ in range(args.spmv_reps):
-                fn()
-            ev1.record(stream)
-        ev1.synchronize()
-        return ev0.elapsed_time(ev1) / 1e3 / args.spmv_reps
-
-    t_k = timed(lambda: keo.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream))
-    t_f = timed(lambda: model.eval_f_internal(x.data_ptr(), params, y.data_ptr(),
-                                              stream.cuda_stream))
-    for name, t_, b_ in (("keo_apply (K.phi FP64)", t_k, 40 * E + 44 * n),
-                         ("compute_f (F(psi) FP64)", t_f, 40 * E + 52 * n),
-                         ("k_block_apply<float> (J.phi FP32, with the block conversion)", t_j32,
-                          24 * E + 36 * n)):
-        others[name] = {"us_per_launch": round(t_ * 1e6, 2), "bytes_per_launch": b_,
-                        "achieved": round(b_ / t_ / 1e9, 1),
-                        "frac": round(b_ / t_ / 1e9 / HBM_PEAK_GBS, 4)}
-    say("K.phi {:.1f} us ({:.0f} GB/s), F {:.1f} us ({:.0f} GB/s)".format(
-        t_k * 1e6, (40 * E + 44 * n) / t_k / 1e9, t_f * 1e6, (40 * E + 52 * n) / t_f / 1e9))
+    fk = lambda: keo.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)  # noqa
+    ff = lambda: model.eval_f_internal(x.data_ptr(), params, y.data_ptr(),  # noqa: E731
+                                       stream.cuda_stream)
+    for name, fn, b_ in (("keo_apply (K.phi FP64)", fk, 40 * E + 44 * n),
+                         ("compute_f (F(psi) FP64)", ff, 40 * E + 52 * n)):
+        tw, tc = _event_time(stream, fn, args.spmv_reps), _cold_time(stream, fn, 10, flush)
+        others[name] = {"us_per_launch": round(tw * 1e6, 2), "cold_us_per_launch": round(tc * 1e6, 2),
+                        "bytes_per_launch": b_, "achieved": round(gbs(b_, tw), 1),
+                        "cold_achieved": round(gbs(b_, tc), 1),
+                        "frac": round(gbs(b_, tw) / HBM_PEAK_GBS, 4),
+                        "cold_frac": round(gbs(b_, tc) / HBM_PEAK_GBS, 4)}
+    others["k_block_apply<float> (J.phi FP32, blocks converted once per rebuild)"] = {
+        "us_per_launch": round(t["warm32"] * 1e6, 2),
+        "cold_us_per_launch": round(t["cold32"] * 1e6, 2), "bytes_per_launch": t["bytes32"],
+        "achieved": round(gbs(t["bytes32"], t["warm32"]), 1),
+        "cold_achieved": round(gbs(t["bytes32"], t["cold32"]), 1),
+        "frac": round(gbs(t["bytes32"], t["warm32"]) / HBM_PEAK_GBS, 4),
+        "cold_frac": round(gbs(t["bytes32"], t["cold32"]) / HBM_PEAK_GBS, 4)}
+    del J, keo
+    # The 160^3 cube: 1.44 GB per J launch, no cache effects.
+    k160 = None
+    if args.roofline_160:
+        from nosh_amd import meshgen as mg
+        c, cl = mg.kuhn_cube(160)
+        m160 = nosh.Mesh(c, cl)
+        md160 = nosh.Nls(m160, nosh.VectorFieldExplicitValues(m160, mg.magnetic_potential(c), 0.0),
+                         nosh.ScalarFieldConstant(m160, -1.0), 1.0,
+                         nosh.ScalarFieldConstant(m160, 1.0))
+        n1 = c.shape[0]
+        t1, J1, _, _ = jphi_timings(args, m160, md160, params, mg.random_state(n1, 0),
+                                    mg.random_state(n1, 1), stream, dev, flush, fp32=False)
+        k160 = {"n_vertices": n1, "n_edges": t1["E"], "bytes_per_launch": t1["bytes64"],
+                "us_per_launch": round(t1["warm64"] * 1e6, 2),
+                "cold_us_per_launch": round(t1["cold64"] * 1e6, 2),
+                "achieved": round(gbs(t1["bytes64"], t1["cold64"]), 1),
+                "frac": round(gbs(t1["bytes64"], t1["cold64"]) / HBM_PEAK_GBS, 4),
+                "frac_of_attainable_read": round(gbs(t1["bytes64"], t1["cold64"]) /
+                                                 ceiling["read"], 4)}
+        say(f"J.phi 160^3: warm {t1['warm64']*1e6:.1f} us, cold {t1['cold64']*1e6:.1f} us -> "
+            f"{k160['achieved']:.0f} GB/s")
+        del J1, md160, m160
+    del flush
     traffic, traffic_src = _profiled_traffic(args.ncell)
-    say(f"J.phi {t_j*1e6:.1f} us -> {ach:.0f} GB/s ({ach/HBM_PEAK_GBS:.1%}); copy "
-        f"{copy_gbs:.0f} GB/s; fp32 J.phi {t_j32*1e6:.1f} us")
-    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+    ach_cold = gbs(bytes_canon, t["cold64"])
+    ach_warm = gbs(bytes_canon, t["warm64"])
+    say(f"J.phi 96^3 FP64: cold {t['cold64']*1e6:.1f} us -> {ach_cold:.0f} GB/s "
+        f"({ach_cold/HBM_PEAK_GBS:.1%}); warm {t['warm64']*1e6:.1f} us -> {ach_warm:.0f} GB/s; "
+        f"FP32 cold {t['cold32']*1e6:.1f} us, warm {t['warm32']*1e6:.1f} us; torch copy "
+        f"{copy_gbs:.0f} GB/s")
+    return {"bound": "hbm", "achieved": round(ach_cold, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach_cold / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": traffic_src,
-            "kernel": "k_block_apply<double> (J.phi FP64)",
-            "us_per_launch": round(t_j * 1e6, 2), "bytes_per_launch": bytes_canon,
-            "cold_us_per_launch": round(t_cold * 1e6, 2),
-            "cold_achieved": round(bytes_canon / t_cold / 1e9, 1),
-            "cold_frac": round(bytes_canon / t_cold / 1e9 / HBM_PEAK_GBS, 4),
-            "copy_kernel_GBs": round(copy_gbs, 1),
+            "kernel": "k_block_apply<double> (J.phi FP64, 96^3, cold caches)",
+            "us_per_launch": round(t["cold64"] * 1e6, 2), "bytes_per_launch": bytes_canon,
+            "attainable": {"unit": "GB/s", "own_stream_kernels": ceiling,
+                           "torch_copy": round(copy_gbs, 1)},
+            "frac_of_attainable_read": round(ach_cold / ceiling["read"], 4),
+            "warm": {"us_per_launch": round(t["warm64"] * 1e6, 2), "achieved": round(ach_warm, 1),
+                     "frac": round(ach_warm / HBM_PEAK_GBS, 4),
+                     "note": "back-to-back launches: part of the 313 MB working set is served "
+                             "by the 256 MB Infinity Cache"},
+            "kuhn160": k160,
             "layout_bytes_per_launch": bytes_layout,
             "layout_note": "sliced ELLPACK (64-row slices) of the off-diagonal entries: "
                            "16 B value + 4 B column per padded slot, 4 B per slice, "
                            "24 B 2x2 block + 16 B x + 16 B y per vertex",
-            "nnz_pad": layout["nnz_pad"], "jphi_fp32_us": round(t_j32 * 1e6, 2),
-            "other_kernels": others}
+            "nnz_pad": layout["nnz_pad"], "other_kernels": others}
 
 
 def surrogate_mfma(rows, stream, dev, nin=9, width=256, depth=4, reps=5):
@@ -506,46 +678,66 @@ def surrogate_mfma(rows, stream, dev, nin=9, width=256, depth=4, reps=5):
 
 
 def single_solves(args, model, params, psi0, stream, dev):
-    """The random-start solve, plain FP64 and FP32-inner, side by side."""
+    """The 96^3 solve side by side, plain FP64 and FP32-inner / FP64-outer
+    Newton-MINRES, from two starts: the seeded random state (|psi| ~
+    U(0.5,1), arg ~ U(0, 2 pi); on this fine mesh the random phases cancel
+    and both variants converge to the trivial state psi = 0, which the
+    record names), and a superconducting start (psi = 1 plus 10 % seeded
+    noise), which both take to the same nontrivial state. Reported per
+    variant: ||F||, ||psi|| and the phase-aligned ||psi - psi_FP64||
+    (absolute, and relative when psi_FP64 is nontrivial)."""
     import torch
     import nosh_amd as nosh
-    psi_dev0 = torch.from_numpy(psi0).to(dev)
-    psi = torch.empty_like(psi_dev0)
-    results, sols = {}, {}
-    for var in [v for v in args.variants.split(",") if v]:
-        kw = dict(tol=1e-10, max_iters=40, minres_rtol=1e-10,
-                  minres_max_iters=args.minres_maxit, cheb_iters=args.cheb,
-                  cheb_ratio=args.ratio)
-        if var == "mixed":
-            kw.update(precision=1, forcing=1, forcing_min=1e-6, forcing_max=0.1)
-        opts = nosh.default_newton_options(**kw)
-        psi.copy_(psi_dev0)
-        torch.cuda.synchronize()
-        nosh.newton_solve_device(model, psi.data_ptr(), params, opts, stream.cuda_stream)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        all_stats = []
-        for _ in range(args.solve_reps):
-            with torch.cuda.stream(stream):
-                psi.copy_(psi_dev0)
-            all_stats.append(nosh.newton_solve_device(model, psi.data_ptr(), params, opts,
-                                                      stream.cuda_stream))
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        ok = all(s["converged"] and s["final_norm_f"] <= 1e-10 for s in all_stats)
-        sols[var] = psi.cpu().numpy()
-        results[var] = {"solves_per_s": args.solve_reps / dt,
-                        "s_per_solve": dt / args.solve_reps, "converged": ok,
-                        "final_norm_f": all_stats[-1]["final_norm_f"],
-                        "newton_iters": all_stats[-1]["newton_iters"],
-                        "minres_iters": all_stats[-1]["minres_iters_total"]}
-        say(var, results[var])
-    if "plain" in sols:
-        ref = sols["plain"]
-        for var, s in sols.items():
-            d = np.linalg.norm(_phase_align(s, ref) - ref)
-            results[var]["rel_diff_vs_fp64"] = float(d / max(np.linalg.norm(ref), 1e-300))
-    return results
+    from nosh_amd import meshgen as mg
+    n = psi0.size // 2
+    s1 = 0.1 * mg.random_state(n, 5)
+    s1[0::2] += 1.0
+    out = {}
+    for start, p0 in (("random", psi0), ("superconducting", s1)):
+        psi_dev0 = torch.from_numpy(p0).to(dev)
+        psi = torch.empty_like(psi_dev0)
+        results, sols = {}, {}
+        for var in [v for v in args.variants.split(",") if v]:
+            kw = dict(tol=1e-10, max_iters=40, minres_rtol=1e-10,
+                      minres_max_iters=args.minres_maxit, cheb_iters=args.cheb,
+                      cheb_ratio=args.ratio)
+            if var == "mixed":
+                kw.update(precision=1, forcing=1, forcing_min=1e-6, forcing_max=0.1)
+            opts = nosh.default_newton_options(**kw)
+            psi.copy_(psi_dev0)
+            torch.cuda.synchronize()
+            nosh.newton_solve_device(model, psi.data_ptr(), params, opts, stream.cuda_stream)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            all_stats = []
+            for _ in range(args.solve_reps):
+                with torch.cuda.stream(stream):
+                    psi.copy_(psi_dev0)
+                all_stats.append(nosh.newton_solve_device(model, psi.data_ptr(), params, opts,
+                                                          stream.cuda_stream))
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            ok = all(s["converged"] and s["final_norm_f"] <= 1e-10 for s in all_stats)
+            sols[var] = psi.cpu().numpy()
+            results[var] = {"solves_per_s": args.solve_reps / dt,
+                            "s_per_solve": dt / args.solve_reps, "converged": ok,
+                            "norm_f": all_stats[-1]["final_norm_f"],
+                            "norm_psi": float(np.linalg.norm(sols[var])),
+                            "newton_iters": all_stats[-1]["newton_iters"],
+                            "minres_iters": all_stats[-1]["minres_iters_total"]}
+        if "plain" in sols:
+            ref = sols["plain"]
+            nref = float(np.linalg.norm(ref))
+            trivial = nref < 1e-6 * np.sqrt(n)
+            for var, x in sols.items():
+                d = float(np.linalg.norm(_phase_align(x, ref) - ref))
+                results[var]["norm_diff_vs_fp64"] = d
+                results[var]["rel_diff_vs_fp64"] = None if trivial else d / nref
+            results["state"] = ("trivial (psi = 0: both variants converge to the normal state)"
+                                if trivial else "nontrivial")
+        say(start, results)
+        out[start] = results
+    return out
 
 
 def headline(args):
@@ -570,10 +762,10 @@ def headline(args):
     cpu = None
     if not args.no_cpu and cont["plain_sample"]:
         ps = cont["plain_sample"][0]
-        cpu = cpu_baseline(coords, cells, A, 0.5 * args.cont_span, psi0, phi,
+        cpu = cpu_baseline(coords, cells, A, args.cont_span / args.steps, psi0, phi,
                            {"newton_iters": ps["newton_iters"] / args.steps,
                             "minres_iters": ps["minres_iters"] / args.steps},
-                           args.cheb, args.ratio, "plain FP64 continuation")
+                           args.cheb, args.ratio, "plain FP64 continuation", model=model)
     c0 = None if args.no_cpu else config0_solves(args)
     if c0:
         say(f"config 0 solve: GPU {c0['gpu_seconds']:.3f}s, CPU {c0['cpu_seconds']:.3f}s, "
@@ -592,7 +784,7 @@ def headline(args):
         "vs_baseline": None,
         "dtype": "f64/f32 (FP32 inner MINRES, FP64 Newton); bf16/f32 MFMA surrogate",
         "data": "synthetic (structured Kuhn mesh; branch start states 1/sqrt(g); seeded "
-                "SplitMix64 psi_0 for the random-start solves)",
+                "SplitMix64 psi_0 for the side-by-side solves)",
         "config": {"workload": f"Kuhn-tet cube {args.ncell}^3: mu-continuation 0 -> "
                                f"{args.cont_span} in {args.steps} steps of "
                                f"{args.cont_branches} (mu_0 = 0, g) branches, batched",
@@ -606,7 +798,7 @@ def headline(args):
         "speedup_vs_plain_fp64": (round(cont["speedup_vs_plain_fp64"], 3)
                                   if cont["speedup_vs_plain_fp64"] else None),
         "continuation": cont,
-        "random_start_solves": solves,
+        "side_by_side_solves": solves,
         "roofline": roofline,
         "roofline_timed_region": cont["dominant_kernel_roofline"],
         "surrogate_mfma": mfma,
@@ -767,11 +959,11 @@ def batched_workload(args):
     psi0 = np.concatenate([mg.random_state(meshes[owner[i]][0].shape[0], 1000 + i)
                            for i in order])
     batch = nosh.Batch([(models[k], counts[k]) for k in range(4)], mu, g)
-    kw = dict(tol=1e-10, max_iters=80, minres_rtol=1e-10, minres_max_iters=4000,
+    # Newton from random phases is chaotic on these meshes (5 to about 120
+    # steps per instance); the limit leaves room for the slowest.
+    kw = dict(tol=1e-10, max_iters=200, minres_rtol=1e-10, minres_max_iters=4000,
               cheb_iters=args.cheb, cheb_ratio=args.ratio, forcing=1, forcing_min=1e-6,
               forcing_max=0.1)
-    def say(*a):
-        print(*a, file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
     tuned = batch.autotune(psi0, use_cache=False, **kw)
@@ -805,14 +997,15 @@ def batched_workload(args):
         p0 = batch.split(psi0)[j]
         t1 = time.perf_counter()
         p, s = nosh.newton_solve(models[k], p0, {"g": g[j], "mu": mu[j]}, tol=1e-10,
-                                 max_iters=80, minres_rtol=1e-10, minres_max_iters=4000,
+                                 max_iters=200, minres_rtol=1e-10, minres_max_iters=4000,
                                  cheb_iters=args.cheb, cheb_ratio=args.ratio)
         t_plain += time.perf_counter() - t1
         worst = max(worst, s["final_norm_f"])
         say(f"plain instance {j} (mesh {sizes[k]}^3): {time.perf_counter() - t1:.2f}s, "
             f"newton {s['newton_iters']}, minres {s['minres_iters_total']}")
     plain_rate = len(sample) / t_plain
-    rate = ninst / dt
+    # Only converged instances count as solves.
+    rate = conv / dt
     out = {"metric": "batched Newton-MINRES solves/sec at ||F||<=1e-10 (128 uneven instances)",
            "value": round(rate, 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 2), "higher_is_better": True,

@@ -356,6 +356,16 @@ int nosh_mlp_destroy(nosh_mlp_t mlp);
  * sum_k c_k Re(conj(phi_k) psi_k) / sum_k c_k. */
 int nosh_nls_gibbs_energy(nosh_nls_t nls, const double* psi, double* energy);
 int nosh_nls_inner_product(nosh_nls_t nls, const double* phi, const double* psi, double* value);
+/* ------------------------------------------------ measurement helper -----
+ * No reference counterpart. Average seconds per launch of the library's own
+ * streaming kernel `kind` (0 read, 1 write, 2 copy) over buffers of `bytes`
+ * bytes (16-byte vectors, 256-thread workgroups, blocks_per_cu per CU, 0 =
+ * 8; nt != 0: non-temporal accesses), timed with events over `reps`
+ * launches: the attainable HBM bandwidth the operator kernels are measured
+ * against (bytes, or 2 bytes for copy, per launch). */
+int nosh_bench_stream(int kind, int64_t bytes, int reps, int blocks_per_cu, int nt,
+                      double* seconds_per_launch);
+
 
 /* ------------------------------------------------------ continuation -----
  * Natural-parameter continuation, the in-repo stand-in for the LOCA stepper
@@ -422,6 +432,9 @@ typedef struct {
   int retrain_steps;    /* 0; > 0: after the first training, every accepted
                            step replaces the oldest of the train_after stored
                            pairs and runs this many more Adam steps */
+  int smooth_sweeps;    /* batched driver: damped Jacobi sweeps on the residual
+                           of every surrogate guess before it is compared with
+                           the secant guess (default 0) */
 } nosh_surrogate_options;
 
 typedef struct {
@@ -472,6 +485,8 @@ typedef struct {
   double sec_error;     /* mean over branches of ||secant guess - psi|| */
   int halvings;         /* step halvings needed by the branches no guess reached (FP64
                            fallback or batched sub-steps) */
+  int retries;          /* branch retries by pseudo-transient continuation */
+  int serial_fallbacks; /* branches handed to the single-instance FP64 fallback */
 } nosh_batch_continuation_step;
 
 int nosh_batch_continuation_run(nosh_batch_t batch, double* psi, double span, int steps,

@@ -596,6 +596,7 @@ def _load_more():
     sig = {
         "nosh_nls_gibbs_energy": (i32, [vp, vp, C.POINTER(d)]),
         "nosh_nls_inner_product": (i32, [vp, vp, vp, C.POINTER(d)]),
+        "nosh_bench_stream": (i32, [i32, C.c_int64, i32, i32, i32, C.POINTER(d)]),
         "nosh_continuation_default_options": (i32, [C.POINTER(ContinuationOptions)]),
         "nosh_continuation_run": (i32, [vp, vp, i32, cpp, vp, C.POINTER(ContinuationOptions),
                                         C.POINTER(ContinuationStep), C.POINTER(i32)]),
@@ -615,6 +616,18 @@ def gibbs_energy(model: Nls, psi) -> float:
     v = C.c_double()
     _check(lib.nosh_nls_gibbs_energy(model._h, _ptr(x), C.byref(v)))
     return v.value
+
+
+def bench_stream(kind: str, nbytes: int, reps: int = 20, blocks_per_cu: int = 8,
+                 nontemporal: bool = True) -> float:
+    """Seconds per launch of the library's streaming kernel ("read", "write"
+    or "copy") over nbytes-byte buffers (copy moves 2 nbytes per launch)."""
+    lib = _load_more()
+    k = {"read": 0, "write": 1, "copy": 2}[kind]
+    t = C.c_double()
+    _check(lib.nosh_bench_stream(k, int(nbytes), reps, blocks_per_cu, int(nontemporal),
+                                 C.byref(t)))
+    return t.value
 
 
 def inner_product(model: Nls, phi, psi) -> float:
@@ -730,10 +743,11 @@ class Batch(_Handle):
         return out
 
     def set_tuning(self, block_threads=256, mode=0, graph_iters=4, poll=16, fold=1):
-        """mode 0: hipGraph iterations; 1: persistent cooperative kernel (grids of at
-        most 64 blocks); 2: hipGraph with two lanes per thread in the gathering kernels
-        (every group at least 2 lanes). All modes give the same results bit for bit;
-        an unsupported mode raises NoshError and leaves the batch in mode 0."""
+        """mode 0: hipGraph iterations; 1: persistent cooperative kernel (one or two
+        workgroups per CU striding over the row tiles, grid barriers with a bounded
+        wait); 2: hipGraph with two lanes per thread in the gathering kernels (every
+        group at least 2 lanes). All modes give the same results bit for bit; an
+        unsupported mode raises NoshError and leaves the batch in mode 0."""
         _check(_lib.nosh_batch_set_tuning(self._h, block_threads, fold, mode, graph_iters,
                                           poll))
 
@@ -804,7 +818,8 @@ class BatchContinuationStep(C.Structure):
     _fields_ = [("param", C.c_double), ("newton_iters", C.c_int), ("minres_iters", C.c_int),
                 ("converged", C.c_int), ("seeded", C.c_int), ("fallbacks", C.c_int),
                 ("max_norm_f", C.c_double), ("seconds", C.c_double), ("pred_error", C.c_double),
-                ("sec_error", C.c_double), ("halvings", C.c_int)]
+                ("sec_error", C.c_double), ("halvings", C.c_int), ("retries", C.c_int),
+                ("serial_fallbacks", C.c_int)]
 
 
 def _load_mlp():
@@ -894,7 +909,7 @@ class SurrogateOptions(C.Structure):
     _fields_ = [("width", C.c_int), ("depth", C.c_int), ("train_after", C.c_int),
                 ("train_steps", C.c_int), ("batch", C.c_int), ("lr", C.c_float),
                 ("newton_budget", C.c_int), ("energy_tol", C.c_double), ("seed", C.c_uint64),
-                ("retrain_steps", C.c_int)]
+                ("retrain_steps", C.c_int), ("smooth_sweeps", C.c_int)]
 
 
 class SurrogateStats(C.Structure):

@@ -121,7 +121,8 @@ struct Arc {
       nls->pblk.alloc(3 * n);
     }
     if (nls->bound.n < 1) nls->bound.alloc(1);
-    launch_dfdp(m, dk->kv, nls->ct.p, x, param == "g", nls->V->sf.dvdp_constant(param), fl.p, s);
+    launch_dfdp(m, dk->kv, nls->ct.p, x, param == "g", nls->V->sf.dvdp_constant(param),
+                nls->V->sf.dvdp_values(param, s), fl.p, s);
     launch_rebuild_blocks(m, nls->keo->kv, nls->ct.p, V, g, x, nls->jblk.p, nls->pblk.p,
                           nls->bound.p, s);
     if (no.precision == 1) {

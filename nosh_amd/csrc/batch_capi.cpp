@@ -117,7 +117,8 @@ void batch_residuals(nosh_batch_s* h, std::vector<double>& norms, hipStream_t s)
 }
 
 int newton_batch(nosh_batch_s* h, const nosh_newton_options& o, nosh_batch_instance_stats* st,
-                 hipStream_t s, const int* skip, const int* max_iters, int stop_below) {
+                 hipStream_t s, const int* skip, const int* max_iters, int stop_below,
+                 const double* ptc0) {
   Batch& B = h->b;
   const int ni = B.args.ninst;
   B.args.iters = o.cheb_iters;
@@ -130,6 +131,11 @@ int newton_batch(nosh_batch_s* h, const nosh_newton_options& o, nosh_batch_insta
   if (skip)
     for (int i = 0; i < ni; ++i) done[i] = skip[i] != 0;
   std::vector<double> final_nf(ni, 0.0);
+  // Pseudo-transient shifts: sigma_{k+1} = sigma_k ||F_k|| / ||F_{k-1}||
+  // (switched evolution relaxation), 0 for plain Newton.
+  std::vector<double> sig(ni, 0.0);
+  if (ptc0)
+    for (int i = 0; i < ni; ++i) sig[i] = ptc0[i];
   int lockstep = 0;
   for (int it = 0;; ++it) {
     B.compute_norms(s, nf);
@@ -161,6 +167,7 @@ int newton_batch(nosh_batch_s* h, const nosh_newton_options& o, nosh_batch_insta
       }
       rt[i] = eta;
       eta_prev[i] = eta;
+      if (ptc0 && nf_prev[i] > 0.0) sig[i] *= nf[i] / nf_prev[i];
       nf_prev[i] = nf[i];
       B.active_h[i] = 1;
       newton[i] = it + 1;
@@ -178,10 +185,20 @@ int newton_batch(nosh_batch_s* h, const nosh_newton_options& o, nosh_batch_insta
     NOSH_HIP(hipMemcpyAsync(B.active.p, B.active_h.data(), sizeof(int) * ni,
                             hipMemcpyHostToDevice, s));
     NOSH_HIP(hipMemcpyAsync(B.rtol.p, rt.data(), sizeof(double) * ni, hipMemcpyHostToDevice, s));
+    if (ptc0) {
+      int i0 = 0;
+      while (i0 + 1 < ni && !B.active_h[i0]) ++i0;
+      if (std::getenv("NOSH_PROGRESS"))
+        std::fprintf(stderr, "[batch newton] step %d instance %d ||F|| %.3e sigma %.3e\n", it,
+                     i0, nf[i0], sig[i0]);
+      NOSH_HIP(hipMemcpyAsync(B.shift.p, sig.data(), sizeof(double) * ni, hipMemcpyHostToDevice,
+                              s));
+    }
     lockstep += B.minres(s, o.minres_max_iters);
     for (int i = 0; i < ni; ++i)
       if (B.active_h[i]) mres[i] += B.Sh[i].it;
   }
+  if (ptc0) B.shift.zero(s);
   if (st)
     for (int i = 0; i < ni; ++i) {
       if (skip && skip[i]) continue;
@@ -259,8 +276,13 @@ int nosh_batch_create(int ngroups, const nosh_nls_t* models, const int* counts, 
       G.mesh = md->mesh->m.get();
       G.B = counts[k];
       MeshCache& m = *G.mesh;
-      // alpha_e with the model's thickness (keo::build_alpha_cache_).
-      std::map<std::string, double> p0{{"mu", mu[inst]}, {"g", g[inst]}};
+      // The model's nominal parameters (a parameterised potential keeps its
+      // own value) with this group's first mu and g; alpha_e with the
+      // model's thickness (keo::build_alpha_cache_).
+      std::map<std::string, double> p0;
+      for (size_t q = 0; q < md->p_names.size(); ++q) p0[md->p_names[q]] = md->p_nominal[q];
+      p0["mu"] = mu[inst];
+      p0["g"] = g[inst];
       md->keo->set_parameters(p0, s);
       G.s32.alloc(m.nnz_pad);
       G.a64.alloc(m.nnz_pad);
@@ -283,6 +305,7 @@ int nosh_batch_create(int ngroups, const nosh_nls_t* models, const int* counts, 
       d.scol = m.scol.p;
       d.s32 = G.s32.p;
       d.kval = nullptr;
+      d.kval64 = nullptr;
       d.a64 = G.a64.p;
       d.t64 = G.t64.p;
       d.diag = G.diag.p;
@@ -314,6 +337,9 @@ int nosh_batch_create(int ngroups, const nosh_nls_t* models, const int* counts, 
     B.args.g = B.g.p;
     B.args.bound = B.bound.p;
     B.args.rtol = B.rtol.p;
+    B.shift.alloc(inst);
+    B.shift.zero(s);
+    B.args.shift = B.shift.p;
     B.args.active = B.active.p;
     B.relayout(256, 1);
     NOSH_HIP(hipStreamSynchronize(s));
@@ -341,11 +367,6 @@ int nosh_batch_set_tuning(nosh_batch_t batch, int block_threads, int fold, int m
     batch->b.drop_graph();
     batch->b.mode = 0;
     batch->b.lanes = 1;
-    NOSH_CHECK(mode != 1 ||
-                   batch->b.args.total_blocks / batch->b.fold <= nosh::kPersistentMaxBlocks,
-               kUnsupported,
-               "persistent mode needs a grid of at most 64 workgroups at this block size; "
-               "use mode 0 (hipGraph)");
     int minbp = 64;
     for (auto& G : batch->b.ghost) minbp = std::min(minbp, G.dev.bp);
     NOSH_CHECK(mode != 2 || minbp >= nosh::kVecLanes, kUnsupported,

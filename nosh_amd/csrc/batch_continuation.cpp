@@ -15,9 +15,11 @@
 // branch whose Newton from the surrogate guess does not converge within the
 // budget, or whose Gibbs energy leaves its linear trend, is solved again
 // from psi_sec; at a branch transition, where Newton from the secant guess
-// may wander, psi_k is tried too (short tries of both); a branch neither
-// brings to convergence is handed, alone, to the single-instance FP64
-// tangent continuation over the step, and last to batched sub-steps.
+// may wander, the branches a short try leaves are retried in lockstep by
+// pseudo-transient continuation (shifted Newton steps, J + sigma c t), from
+// the secant guess and then from psi_k; a branch none of these brings to
+// convergence is handed, alone, to the single-instance FP64 solver, and
+// last to batched sub-steps.
 //
 // This is the caller the reference delegates to LOCA (natural stepping,
 // executables/nosh-cont/nosh-cont.cpp:226-290,344 with examples/conf.xml:35-190);
@@ -64,11 +66,22 @@ constexpr int kShortNewton = 20;
 // solver's best).
 constexpr int kFallbackCheb = 6;
 constexpr int kMaxHalvings = 3;   // step splits (2, 4, 8 sub-steps) for failed branches
+// Pseudo-transient retries: initial shift sigma_0 (in units of the control
+// volume times thickness, the mass matrix of F, so independent of the mesh
+// size), forcing cap and Chebyshev degree (96^3 transition mu 0.70 -> 0.75:
+// sigma_0 = 0.1 in 10 steps, 1 in 22, 10 did not converge in 40; forcing caps
+// 1e-3 .. 1e-5 and degrees 5 / 6 measured within 10 % of each other without
+// the shift).
+constexpr double kPtcShift0 = 0.1;
+constexpr double kPtcForcing = 1e-4;
+constexpr int kPtcCheb = 6;
 // A surrogate guess is used only when its residual is at most this fraction
 // of the secant guess's: inexact Newton (Eisenstat-Walker forcing) takes the
 // same number of steps, and often more MINRES iterations, from a guess that
 // is only slightly better.
 constexpr double kSurrogateMargin = 0.5;
+// Damping of the Jacobi sweeps that smooth the surrogate's guess.
+constexpr double kSmoothOmega = 0.6;
 constexpr int kRed = 256;       // threads of the per-lane reduction blocks
 constexpr int kRedBlocks = 512; // blocks of the per-lane reduction (grid-stride)
 
@@ -131,6 +144,31 @@ __global__ void k_bc_apply(int64_t n, int bp, int B, const double2* __restrict__
     s.y += scale * double(Y[2 * k + 1]);
   }
   out[i] = s;
+}
+
+// One damped Jacobi sweep on the residual, for the real lanes:
+//   out = psi - omega F(psi) / d,   d = sum alpha + c t (V + 2 g |psi|^2)
+// (the diagonal of J's 2 x 2 block, averaged), psi and F(psi) interleaved
+// [n][bp]. It removes the rough part of a guess's error, which K amplifies
+// in the residual, and leaves the smooth part alone.
+__global__ void k_bc_smooth(int64_t n, int bp, int B, int inst0, const double* __restrict__ diag,
+                            const double* __restrict__ ct, const double* __restrict__ V,
+                            const double* __restrict__ g, double omega,
+                            const double2* __restrict__ psi, const double2* __restrict__ f,
+                            double2* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n * bp) return;
+  const int64_t row = i / bp;
+  const int b = int(i - row * bp);
+  const double2 p = psi[i];
+  if (b >= B) {
+    out[i] = p;
+    return;
+  }
+  const double2 r = f[i];
+  const double d = diag[row] + ct[row] * (V[row] + 2.0 * g[inst0 + b] * (p.x * p.x + p.y * p.y));
+  const double w = omega / d;
+  out[i] = make_double2(p.x - w * r.x, p.y - w * r.y);
 }
 
 // dst[row][b] = src[row][b] for the lanes with take[b] != 0.
@@ -240,7 +278,7 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
   int rc = bcguard([&] {
     NOSH_CHECK(bt && psi && record && steps >= 1, kInvalid, "invalid batch continuation call");
     NOSH_CHECK(!so || (so->train_after >= 1 && so->batch >= 64 && so->newton_budget >= 1 &&
-                       so->retrain_steps >= 0),
+                       so->retrain_steps >= 0 && so->smooth_sweeps >= 0),
                kInvalid, "bad surrogate options");
     nosh_newton_options o;
     if (opts)
@@ -364,6 +402,19 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
         GroupHost& G = B.ghost[k];
         k_bc_apply<<<grid_for(G.mesh->n * G.dev.bp, 256), 256, 0, s>>>(
             G.mesh->n, G.dev.bp, G.B, gc[k].sec.p, Y.p + gc[k].xoff * 2, scale, gc[k].guess.p);
+      }
+      // Jacobi sweeps on the residual of the guess (each: F at the guess,
+      // then the damped update).
+      std::vector<double> nf_tmp;
+      for (int sw = 0; sw < so->smooth_sweeps; ++sw) {
+        copy_state(&GroupCont::guess, false);
+        B.compute_norms(s, nf_tmp);
+        for (size_t k = 0; k < ng; ++k) {
+          GroupHost& G = B.ghost[k];
+          k_bc_smooth<<<grid_for(G.mesh->n * G.dev.bp, 256), 256, 0, s>>>(
+              G.mesh->n, G.dev.bp, G.B, G.dev.inst0, G.diag.p, G.ct.p, G.V.p, B.g.p,
+              kSmoothOmega, G.psi.p, G.f.p, gc[k].guess.p);
+        }
       }
     };
     // Adam steps on random vertex minibatches of the stored pairs; the first
@@ -505,34 +556,61 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
           sst.fallbacks += r.fallbacks;
         }
       }
-      // Secant guess for every branch not yet accepted, then psi_k.
-      // Branches not yet accepted: Newton from the secant guess (unless a
-      // branch already started from it), then from psi_k, first with at most
-      // kShortNewton steps each, then with the full limit. Near a branch
-      // transition Newton wanders from one of the two guesses and converges
-      // from the other; the short tries find out which at a fraction of the
-      // cost of a full failed run.
-      for (int attempt = 0; attempt < 2; ++attempt) {
+      // Branches not yet accepted, in lockstep: Newton from the secant guess
+      // (unless a branch already started from it) and then from psi_k, with
+      // at most kShortNewton steps each; then, for the branches both leave,
+      // pseudo-transient continuation from the secant guess and then from
+      // psi_k. Near a branch transition plain Newton wanders (the old branch
+      // has ended and J is near singular along the mode that carries the
+      // state to the new one), and often converges from one of the two
+      // guesses but not the other. The shifted Newton steps
+      // (J + sigma c t) d = -F, sigma_0 = kPtcShift0 and sigma following ||F||
+      // (switched evolution relaxation), damp that mode while the shift lasts
+      // and reach the new state in about 10 steps where plain Newton took
+      // 20-40 (96^3, 8 branches, mu 0.70 -> 0.75: 0.62 s instead of 4.1 s for
+      // full-length FP32 retries without the shift and 15.7 s for the serial
+      // FP64 fallback).
+      for (int attempt = 0; attempt < 4; ++attempt) {
         const bool from_sec = (attempt % 2) == 0;
-        const int cap = std::min(o.max_iters, kShortNewton);
+        const bool ptc = attempt >= 2;
         int todo = 0;
         for (int i = 0; i < ni; ++i) {
           skip[i] = ok[i] || (attempt == 0 && tried_sec[i]);
           todo += !skip[i];
-          maxit[i] = cap;
+          maxit[i] = ptc ? o.max_iters : std::min(o.max_iters, kShortNewton);
         }
         if (todo == 0) continue;
+        const auto ta = std::chrono::steady_clock::now();
         std::vector<int> take(ni);
         for (int i = 0; i < ni; ++i) take[i] = !skip[i];
         take_into_psi(from_sec ? &GroupCont::sec : &GroupCont::cur, take);
-        const int it = newton_batch(bt, o, st.data(), s, skip.data(), maxit.data());
-        int mx = 0;
+        nosh_newton_options oa = o;
+        std::vector<double> sig0;
+        if (ptc) {
+          oa.forcing_max = std::min(o.forcing_max, kPtcForcing);
+          oa.cheb_iters = std::max(o.cheb_iters, kPtcCheb);
+          sig0.assign(ni, kPtcShift0);
+        }
+        const int it = newton_batch(bt, oa, st.data(), s, skip.data(), maxit.data(), 0,
+                                    ptc ? sig0.data() : nullptr);
+        int mx = 0, nc = 0;
         for (int i = 0; i < ni; ++i)
-          if (!skip[i]) mx = std::max(mx, st[i].newton_iters);
+          if (!skip[i]) {
+            mx = std::max(mx, st[i].newton_iters);
+            ok[i] = st[i].converged;
+            nc += ok[i];
+          }
         r.newton_iters += mx;
         r.minres_iters += it;
-        for (int i = 0; i < ni; ++i)
-          if (!skip[i]) ok[i] = st[i].converged;
+        if (ptc) r.retries += todo;
+        if (progress && (ptc || nc < todo))
+          std::fprintf(stderr,
+                       "[batch continuation] step %d %s from %s: %d of %d converged, newton %d "
+                       "minres %d, %.3fs\n",
+                       k, ptc ? "pseudo-transient retry" : "short try",
+                       from_sec ? "the secant guess" : "psi_k", nc, todo, mx, it,
+                       std::chrono::duration<double>(std::chrono::steady_clock::now() - ta)
+                           .count());
       }
       // Branches the short tries did not bring to convergence, one at a time
       // with the single-instance FP64 solver: Newton-MINRES (inner solves to
@@ -550,6 +628,7 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
         o64.cheb_iters = kFallbackCheb;
         for (int i = 0; i < ni; ++i) {
           if (ok[i]) continue;
+          ++r.serial_fallbacks;
           const auto tf = std::chrono::steady_clock::now();
           size_t g = 0;
           while (g + 1 < ng && i >= B.ghost[g + 1].dev.inst0) ++g;
@@ -557,7 +636,11 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
           const int b = i - G.dev.inst0;
           const int64_t n = G.mesh->n;
           if (fb.n < size_t(2 * n)) fb.alloc(2 * n);
-          std::map<std::string, double> p1{{"g", bt->g_h[i]}, {"mu", mu[i]}};
+          std::map<std::string, double> p1;
+          nosh_nls_s* md = bt->models[g];
+          for (size_t q = 0; q < md->p_names.size(); ++q) p1[md->p_names[q]] = md->p_nominal[q];
+          p1["g"] = bt->g_h[i];
+          p1["mu"] = mu[i];
           int how = 0, fnewton = 0, fminres = 0, fhalv = 0;
           for (int tr = 0; tr < 2 && !ok[i]; ++tr) {
             k_bc_lane_get<<<grid_for(n, 256), 256, 0, s>>>(
@@ -592,10 +675,15 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
             co.newton = o64;
             nosh_continuation_step crec[2];
             int cdone = 0;
-            const char* cn[] = {"g"};
-            const double cv[] = {bt->g_h[i]};
-            const int crc = nosh_continuation_run(bt->models[g], fb_h.data(), 1, cn, cv, &co,
-                                                  crec, &cdone);
+            std::vector<const char*> cn;
+            std::vector<double> cv;
+            for (auto& kv : p1)
+              if (kv.first != "mu") {
+                cn.push_back(kv.first.c_str());
+                cv.push_back(kv.second);
+              }
+            const int crc = nosh_continuation_run(bt->models[g], fb_h.data(), int(cn.size()),
+                                                  cn.data(), cv.data(), &co, crec, &cdone);
             for (int q = 0; q < std::min(cdone + 1, 2); ++q) {
               fnewton += crec[q].newton_iters;
               fminres += crec[q].minres_iters;

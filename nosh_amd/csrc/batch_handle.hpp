@@ -32,7 +32,7 @@ void gather_all(nosh_batch_s* h, double* dst_dev, hipStream_t s);
 // their current iterate in the state).
 int newton_batch(nosh_batch_s* h, const nosh_newton_options& o, nosh_batch_instance_stats* st,
                  hipStream_t s, const int* skip = nullptr, const int* max_iters = nullptr,
-                 int stop_below = 0);
+                 int stop_below = 0, const double* ptc0 = nullptr);
 
 // ||F(psi)|| of every instance at the batch's current state and mu, g.
 void batch_residuals(nosh_batch_s* h, std::vector<double>& norms, hipStream_t s);

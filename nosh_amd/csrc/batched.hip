@@ -16,25 +16,25 @@
 // Krylov vectors, blocks and inner SpMVs are FP32 (the FP32-inner / FP64-outer
 // scheme). MINRES runs in lockstep over the batch with per-instance scalars
 // and done flags; one iteration is either a captured hipGraph (replayed) or
-// a persistent cooperative kernel with grid-wide barriers between phases.
+// a persistent cooperative kernel with grid-wide barriers between phases
+// (a few workgroups per CU, each striding over the row tiles).
 //
 // Reference operators: keo::refill_ (src/parameter_matrix_keo.cpp:74-184),
 // nls::compute_f_ (src/model_evaluator_nls.cpp:527-628), jacobian_operator
 // (src/jacobian_operator.cpp:143-200), keo_regularized::rebuild
 // (src/keo_regularized.cpp:181-264).
 
-#include <hip/hip_cooperative_groups.h>
+#include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 
 #include "batched.hpp"
 #include "kernels.hpp"
 #include "minres_device.hpp"
-
-namespace cg = cooperative_groups;
 
 namespace nosh {
 namespace {
@@ -399,13 +399,52 @@ __device__ __forceinline__ typename Vt<T>::v2 brow(const BGroup& G, int64_t row,
 // Every phase is a device function over one virtual block, used both by the
 // per-phase kernels (graph mode) and by the persistent kernel.
 
+// The FP64 row product from per-slot values precomputed at the group's mu
+// (kval64): the same products in the same order as brow<double>, without a
+// sine and cosine per slot and lane.
+__device__ __forceinline__ double2 brow_val64(const BGroup& G, int64_t row, int b,
+                                              const double2* __restrict__ x) {
+  const int32_t s = uniform(int32_t(row >> 6));
+  const int32_t beg = ld_uniform(G.slice_ptr + s), end = ld_uniform(G.slice_ptr + s + 1);
+  const int32_t w = (end - beg) / kSliceH;
+  double ar = 0.0, ai = 0.0;
+  int32_t j = beg + int32_t(row & 63);
+  const int bp = G.bp;
+  int32_t k = 0;
+  for (; k + kSlotBatch <= w; k += kSlotBatch, j += kSlotBatch * kSliceH) {
+    int32_t c[kSlotBatch];
+    double2 v[kSlotBatch], xv[kSlotBatch];
+#pragma unroll
+    for (int q = 0; q < kSlotBatch; ++q) {
+      c[q] = G.scol[j + q * kSliceH];
+      v[q] = G.kval64[j + q * kSliceH];
+    }
+#pragma unroll
+    for (int q = 0; q < kSlotBatch; ++q) xv[q] = x[int64_t(c[q]) * bp + b];
+#pragma unroll
+    for (int q = 0; q < kSlotBatch; ++q) cmac(ar, ai, v[q].x, v[q].y, xv[q]);
+  }
+  for (; k < w; ++k, j += kSliceH) {
+    const double2 v = G.kval64[j];
+    cmac(ar, ai, v.x, v.y, x[int64_t(G.scol[j]) * bp + b]);
+  }
+  return make_double2(ar, ai);
+}
+
+// F and |F|^2 of one virtual block; VAL64: every group has kval64 (a kernel
+// of its own, so the sine/cosine path does not set its register count).
+template <bool VAL64>
 __device__ double ph_compute_f(const BatchArgs& a, int64_t vb) {
   const Lane L = lane_ctx(a, vb);
   double s2 = 0.0;
   if (L.valid) {
     const BGroup& G = L.G;
     const double mu = a.mu[L.inst], g = a.g[L.inst];
-    const double2 acc = brow<double>(G, L.row, L.b, mu, G.psi);
+    double2 acc;
+    if constexpr (VAL64)
+      acc = brow_val64(G, L.row, L.b, G.psi);
+    else
+      acc = brow<double>(G, L.row, L.b, mu, G.psi);
     const double2 xi = G.psi[L.idx];
     const double d = G.diag[L.row] + G.ct[L.row] * (G.V[L.row] + g * (xi.x * xi.x + xi.y * xi.y));
     const double2 r = make_double2(acc.x + d * xi.x, acc.y + d * xi.y);
@@ -427,17 +466,19 @@ __device__ void ph_rebuild(const BatchArgs& a, int64_t vb) {
   const double2 p = G.psi[L.idx];
   const double c = G.ct[L.row], d = G.diag[L.row];
   const double r2 = p.x * p.x + p.y * p.y;
-  const double al = c * (G.V[L.row] + g * 2.0 * r2);
+  // Pseudo-transient shift sigma c t on both diagonals (0: Newton).
+  const double sh = c * a.shift[L.inst];
+  const double al = c * (G.V[L.row] + g * 2.0 * r2) + sh;
   const double rx2 = g * c * (p.x * p.x - p.y * p.y);
   const double d1b = g * c * (2.0 * p.x * p.y);
   G.jblk[L.idx] = float(d + al + rx2);
   G.jblk[nb3 + L.idx] = float(d + al - rx2);
   G.jblk[2 * nb3 + L.idx] = float(d1b);
-  double a0 = d, a1 = d, bb = 0.0;
+  double a0 = d + sh, a1 = d + sh, bb = 0.0;
   if (g > 0.0) {
     const double pa = g * c * 2.0 * r2;
-    a0 = d + pa + rx2;
-    a1 = d + pa - rx2;
+    a0 = d + pa + rx2 + sh;
+    a1 = d + pa - rx2 + sh;
     bb = d1b;
   }
   G.pblk[L.idx] = float(a0);
@@ -1032,10 +1073,11 @@ __device__ void ph_finalize(const BatchArgs& a, int inst, int what, double* out)
 // ---------------------------------------------------------------- kernels
 // Launched block rb covers the virtual blocks rb*fold .. rb*fold + fold - 1,
 // all of one group (every group's virtual block count is a multiple of fold).
+template <bool VAL64>
 __device__ __forceinline__ void blk_compute_f(const BatchArgs& a, int64_t rb) {
   const int64_t v0 = rb * a.fold;
   double s = 0.0;
-  for (int r = 0; r < a.fold; ++r) s += ph_compute_f(a, tile_vb(a, rb, r));
+  for (int r = 0; r < a.fold; ++r) s += ph_compute_f<VAL64>(a, tile_vb(a, rb, r));
   lane_partials(a, v0, s);
 }
 __device__ __forceinline__ void blk_cheb_first(const BatchArgs& a, int64_t rb, int mode) {
@@ -1055,7 +1097,8 @@ __device__ __forceinline__ void blk_minres_a(const BatchArgs& a, int64_t rb) {
 
 // Launched blocks are remapped so that each XCD works on one contiguous
 // range of row tiles and its L2 holds the x entries those rows gather.
-__global__ void kb_compute_f(BatchArgs a) { blk_compute_f(a, xcd_block(true)); }
+__global__ void kb_compute_f(BatchArgs a) { blk_compute_f<false>(a, xcd_block(true)); }
+__global__ void kb_compute_f64v(BatchArgs a) { blk_compute_f<true>(a, xcd_block(true)); }
 __global__ void kb_rebuild(BatchArgs a) {
   const int64_t rb = xcd_block(true);
   for (int r = 0; r < a.fold; ++r) ph_rebuild(a, tile_vb(a, rb, r));
@@ -1124,32 +1167,94 @@ __global__ void kb_fill_kval(const SlotMeta* __restrict__ s32, int64_t nnz, floa
   kval[j] = v;
 }
 
-// Persistent MINRES: all iterations of the batch in one cooperative launch;
-// grid-wide barriers separate the phases; the loop ends when every instance
-// is done or after maxit iterations, a decision every block takes from the
-// same post-barrier state.
+// Grid-wide barrier of the persistent kernel with a bounded wait. Thread 0
+// of each workgroup publishes the workgroup's writes (release fence at
+// device scope), counts itself in bar[0], and either releases the others
+// (the last arrival resets the count and advances the generation bar[1]) or
+// waits for the generation to change (acquire), at most a.bar_ticks of the
+// constant-rate wall clock. A wait that runs out sets a.bar_err; every
+// waiter also leaves as soon as it sees a.bar_err, so the kernel always
+// ends. Returns true when the barrier failed (then the caller leaves).
+__device__ bool grid_barrier(const BatchArgs& a) {
+  __shared__ int failed;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned* count = a.bar;
+    unsigned* gen = a.bar + 1;
+    const unsigned g0 = __hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    __atomic_thread_fence(__ATOMIC_RELEASE);  // device scope: this workgroup's writes
+    const unsigned prev =
+        __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    int f = 0;
+    if (prev + 1 == gridDim.x) {
+      __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(gen, g0 + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      const long long t0 = wall_clock64();
+      while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+        if (__hip_atomic_load(a.bar_err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          f = 1;
+          break;
+        }
+        if (wall_clock64() - t0 > a.bar_ticks) {
+          __hip_atomic_store(a.bar_err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          f = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(8);
+      }
+    }
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);  // device scope: the others' writes
+    failed = f;
+  }
+  __syncthreads();
+  return failed != 0;
+}
+
+// FP64 slot values at one mu for a whole group (the FP64 arithmetic of the
+// per-lane rebuild, so either path gives the same bits).
+__global__ void kb_fill_kval64(const double* __restrict__ a64, const double* __restrict__ t64,
+                               int64_t nnz, double mu, double2* __restrict__ kval64) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= nnz) return;
+  double vr, vi;
+  slot_value(a64[j], t64[j], mu, vr, vi);
+  kval64[j] = make_double2(vr, vi);
+}
+
+// Persistent MINRES: all iterations of the batch in one cooperative launch
+// of a few workgroups per CU, each striding over the row tiles; grid-wide
+// barriers separate the phases; the loop ends when every instance is done or
+// after maxit iterations, a decision every block takes from the same
+// post-barrier state.
 __global__ void kb_minres_persistent(BatchArgs a, int maxit) {
-  cg::grid_group grid = cg::this_grid();
   const int64_t nrb = a.total_blocks / a.fold;
   for (int it = 0; it < maxit; ++it) {
     for (int64_t rb = blockIdx.x; rb < nrb; rb += gridDim.x) blk_minres_a(a, rb);
-    grid.sync();
+    if (grid_barrier(a)) return;
     for (int inst = blockIdx.x; inst < a.ninst; inst += gridDim.x)
       ph_finalize(a, inst, 2, nullptr);
-    grid.sync();
+    if (grid_barrier(a)) return;
     for (int64_t rb = blockIdx.x; rb < nrb; rb += gridDim.x) blk_cheb_first(a, rb, 2);
-    grid.sync();
+    if (grid_barrier(a)) return;
     for (int j = 1; j < a.iters; ++j) {
       for (int64_t rb = blockIdx.x; rb < nrb; rb += gridDim.x) blk_cheb_step(a, rb, 2, j);
-      grid.sync();
+      if (grid_barrier(a)) return;
     }
     for (int inst = blockIdx.x; inst < a.ninst; inst += gridDim.x)
       ph_finalize(a, inst, 3, nullptr);
-    grid.sync();
+    if (grid_barrier(a)) return;
     int all_done = 1;
     for (int i = 0; i < a.ninst; ++i) all_done &= a.S[i].done;
     if (all_done) break;
   }
+}
+
+// Barrier state and time-out flag cleared before a persistent launch (a
+// kernel rather than a memset, so the clear may sit in a captured graph).
+__global__ void kb_bar_clear(unsigned* bar, int* err) {
+  if (threadIdx.x < 2) bar[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) *err = 0;
 }
 
 }  // namespace
@@ -1196,14 +1301,15 @@ void Batch::drop_graph() {
 }
 
 int Batch::persistent_grid() {
-  int per_cu = 0;
-  NOSH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void*>(kb_minres_persistent), block_threads, 0));
   int dev = 0, cus = 0;
   NOSH_HIP(hipGetDevice(&dev));
   NOSH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const int64_t cap = int64_t(std::max(per_cu, 1)) * cus;
-  return int(std::min<int64_t>(cap, args.total_blocks / args.fold));
+  hipFuncAttributes fa{};
+  NOSH_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kb_minres_persistent)));
+  NOSH_CHECK(fa.maxThreadsPerBlock >= block_threads, kUnsupported,
+             "persistent kernel: block size above the kernel's limit");
+  const int per_cu = block_threads <= 256 ? kPersistentPerCu : 1;
+  return int(std::min<int64_t>(int64_t(per_cu) * cus, args.total_blocks / args.fold));
 }
 
 void Batch::set_mu(const double* m, hipStream_t s) {
@@ -1246,18 +1352,48 @@ void Batch::prepare_kval(hipStream_t s) {
       changed = true;
     }
   }
-  if (changed) {
-    records_host.clear();
-    for (auto& gh : ghost) records_host.push_back(gh.dev);
-    NOSH_HIP(hipMemcpyAsync(groups.p, records_host.data(), sizeof(BGroup) * records_host.size(),
-                            hipMemcpyHostToDevice, s));
-    NOSH_HIP(hipStreamSynchronize(s));
+  if (changed) upload_records(s);
+}
+
+void Batch::upload_records(hipStream_t s) {
+  records_host.clear();
+  for (auto& gh : ghost) records_host.push_back(gh.dev);
+  NOSH_HIP(hipMemcpyAsync(groups.p, records_host.data(), sizeof(BGroup) * records_host.size(),
+                          hipMemcpyHostToDevice, s));
+  NOSH_HIP(hipStreamSynchronize(s));
+}
+
+void Batch::prepare_kval64(hipStream_t s, bool* all) {
+  bool changed = false;
+  if (all) *all = true;
+  for (auto& gh : ghost) {
+    BGroup& G = gh.dev;
+    // F is evaluated for every instance of the group, so all must share mu.
+    bool shared = true;
+    const double m0 = mu_host[G.inst0];
+    for (int b = 1; b < G.B; ++b) shared = shared && mu_host[G.inst0 + b] == m0;
+    const double2* want = nullptr;
+    if (shared) {
+      if (!gh.kval64.p) gh.kval64.alloc(gh.mesh->nnz_pad);
+      if (!gh.kval64_set || gh.kval64_mu != m0) {
+        kb_fill_kval64<<<grid_for(gh.mesh->nnz_pad, 256), 256, 0, s>>>(
+            G.a64, G.t64, gh.mesh->nnz_pad, m0, gh.kval64.p);
+        gh.kval64_mu = m0;
+        gh.kval64_set = true;
+      }
+      want = gh.kval64.p;
+    } else if (all) {
+      *all = false;
+    }
+    if (G.kval64 != want) {
+      G.kval64 = want;
+      changed = true;
+    }
   }
+  if (changed) upload_records(s);
 }
 
 int Batch::minres(hipStream_t s, int maxit) {
-  NOSH_CHECK(mode == 0 || args.total_blocks / args.fold <= kPersistentMaxBlocks, kUnsupported,
-             "persistent grid too large");
   const unsigned nb = unsigned(args.total_blocks / args.fold), bt = unsigned(block_threads);
   prepare_kval(s);
   kb_reset<<<grid_for(args.ninst, 128), 128, 0, s>>>(args, maxit);
@@ -1269,9 +1405,29 @@ int Batch::minres(hipStream_t s, int maxit) {
   kb_finalize<<<args.ninst, 256, 0, s>>>(args, 1, nullptr);
   int issued = 0;
   if (mode == 1) {
+    if (!bar.p) {
+      bar.alloc(2);
+      bar_err.alloc(1);
+      args.bar = bar.p;
+      args.bar_err = bar_err.p;
+    }
+    // The wait limit (seconds) can be lowered through NOSH_BARRIER_TIMEOUT_S,
+    // which the failure-path test uses to force a time-out.
+    double lim = kBarrierTimeoutS;
+    if (const char* e = std::getenv("NOSH_BARRIER_TIMEOUT_S")) lim = std::atof(e);
+    int dev = 0, khz = 0;
+    NOSH_HIP(hipGetDevice(&dev));
+    NOSH_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    args.bar_ticks = static_cast<long long>(std::max(lim, 0.0) * 1e3 * std::max(khz, 1));
+    kb_bar_clear<<<1, 64, 0, s>>>(bar.p, bar_err.p);
     void* kargs[] = {&args, &maxit};
     NOSH_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kb_minres_persistent),
                                         dim3(persistent_grid()), dim3(bt), kargs, 0, s));
+    int err = 0;
+    NOSH_HIP(hipMemcpyAsync(&err, bar_err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    NOSH_HIP(hipStreamSynchronize(s));
+    NOSH_CHECK(err == 0, kUnsupported,
+               "persistent MINRES: a grid barrier timed out (workgroups not co-resident)");
     issued = maxit;
   } else {
     build_graph(s);
@@ -1338,7 +1494,12 @@ double Batch::time_cheb_step(hipStream_t s, int reps) {
 
 void Batch::compute_norms(hipStream_t s, std::vector<double>& nrm) {
   const unsigned nb = unsigned(args.total_blocks / args.fold), bt = unsigned(block_threads);
-  kb_compute_f<<<nb, bt, 0, s>>>(args);
+  bool all64 = true;
+  prepare_kval64(s, &all64);
+  if (all64)
+    kb_compute_f64v<<<nb, bt, 0, s>>>(args);
+  else
+    kb_compute_f<<<nb, bt, 0, s>>>(args);
   kb_finalize<<<args.ninst, 256, 0, s>>>(args, 0, norms.p);
   nrm.resize(args.ninst);
   NOSH_HIP(hipMemcpyAsync(nrm.data(), norms.p, sizeof(double) * args.ninst,

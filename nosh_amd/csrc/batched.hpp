@@ -39,6 +39,9 @@ struct BGroup {
   const SlotMeta* s32;  // per padded slot, FP32 path
   const SlotVal* kval;  // per padded slot at the mu every active instance of the
                         // group shares, or null (each lane rebuilds K at its mu)
+  const double2* kval64;  // per padded slot, FP64 value -alpha e^{-i mu a~} at the mu
+                          // every instance of the group shares, or null (F then
+                          // rebuilds it per lane)
   const double* a64;
   const double* t64;
   const double* diag;    // per row: sum alpha
@@ -67,10 +70,14 @@ struct BatchArgs {
   double* bound;        // per instance: max ratio for the Chebyshev interval
   const double* rtol;   // per instance MINRES tolerance
   const int* active;    // per instance: 1 while Newton runs
+  const double* shift;  // per instance: pseudo-transient shift sigma (J + sigma c t I)
   double upper, ratio;
   int iters;            // Chebyshev iterations
   int z0;               // parity of the first Chebyshev output buffer
   double* partials;     // [instance][maxblocks]
+  unsigned* bar;        // persistent mode: grid barrier {arrivals, generation}
+  int* bar_err;         // persistent mode: set when a barrier wait times out
+  long long bar_ticks;  // persistent mode: barrier wait limit in wall-clock ticks
 };
 
 struct GroupHost {
@@ -80,6 +87,9 @@ struct GroupHost {
   DBuf<SlotVal> kval;     // allocated on first use
   double kval_mu = 0.0;   // the mu kval holds (valid when kval_set)
   bool kval_set = false;
+  DBuf<double2> kval64;   // FP64 slot values for F, allocated on first use
+  double kval64_mu = 0.0;
+  bool kval64_set = false;
   DBuf<double> a64, t64, diag, rowabs, ct, V;
   DBuf<double2> psi, f, x;
   DBuf<float> jblk, pblk;
@@ -87,13 +97,17 @@ struct GroupHost {
   BGroup dev{};
 };
 
-// Largest persistent (cooperative) grid accepted. The persistent kernel is
-// correct whenever its whole grid is co-resident; on gfx950 that has been
-// established for grids of up to 64 workgroups, while a 3584-workgroup
-// launch (128 threads, 14 per CU by the occupancy query) never finished, for
-// a reason not yet identified. Larger grids are refused; the per-phase
-// (hipGraph) mode serves them.
-constexpr int64_t kPersistentMaxBlocks = 64;
+// Workgroups per CU of the persistent (cooperative) kernel's grid: one, or
+// two for blocks of at most 256 threads. The kernel strides over all row
+// tiles, so any grid size is correct as long as every workgroup is resident
+// at once; a grid sized to the occupancy limit (occupancy x CUs) leaves no
+// margin, since the dispatcher hands workgroups to the XCDs round-robin and
+// every CU then has to hold exactly its share.
+constexpr int kPersistentPerCu = 2;
+// Wait limit of one grid barrier of the persistent kernel (seconds). A wait
+// that exceeds it (workgroups not co-resident) makes every workgroup leave
+// the kernel and the solve report an error instead of hanging.
+constexpr double kBarrierTimeoutS = 2.0;
 
 // Lanes per thread of the gathering kernels in tuning mode 2.
 constexpr int kVecLanes = 2;
@@ -104,8 +118,10 @@ struct Batch {
   BatchArgs args{};
   DBuf<MinresScalars> S;
   std::vector<MinresScalars> Sh;
-  DBuf<double> mu, g, bound, rtol, norms, partials;
+  DBuf<double> mu, g, bound, rtol, norms, partials, shift;
   DBuf<int> active;
+  DBuf<unsigned> bar;  // persistent mode: grid barrier state
+  DBuf<int> bar_err;   // persistent mode: barrier time-out flag
   std::vector<int> active_h;
   int block_threads = 256;
   int fold = 1;         // virtual blocks per launched block (fewer reduction partials)
@@ -126,6 +142,11 @@ struct Batch {
   // that mu (once per value) and points the group record at it; other groups
   // rebuild K per lane.
   void prepare_kval(hipStream_t s);
+  // For every group whose instances all share one mu, fills kval64 at that
+  // mu (once per value) for the FP64 residual; other groups rebuild K per
+  // lane there.
+  void prepare_kval64(hipStream_t s, bool* all = nullptr);
+  void upload_records(hipStream_t s);
   void launch_phases_iteration(hipStream_t s);
   void launch_cheb_step(hipStream_t s, int mode, int j);
   void launch_minres_a(hipStream_t s);

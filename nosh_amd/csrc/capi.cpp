@@ -37,7 +37,7 @@ hipStream_t default_stream() {
 
 std::map<std::string, double> ScalarField::scalar_parameters() const {
   std::map<std::string, double> m;
-  if (constant && !param.empty()) m[param] = param_init;
+  if (!param.empty()) m[param] = param_init;
   return m;
 }
 
@@ -57,14 +57,29 @@ const double* ScalarField::get_v(const std::map<std::string, double>& p, hipStre
       dev_value = v;
       dev_valid = true;
     }
-  } else if (!dev_valid) {
-    DBuf<double> tmp(mesh->n);
-    tmp.upload(values.data(), mesh->n, s);
-    launch_gather_ref(mesh->n, mesh->perm.p, tmp.p, dev.p, s);
-    NOSH_HIP(hipStreamSynchronize(s));
-    dev_valid = true;
+  } else {
+    // vals.scale(params.at("beta")) (src/scalar_field_explicit_values.cpp:36-43).
+    const double beta = need(p, "beta");
+    dvdp_values("beta", s);
+    if (!dev_valid || beta != dev_value) {
+      launch_scale(base.p, beta, dev.p, mesh->n, s);
+      dev_value = beta;
+      dev_valid = true;
+    }
   }
   return dev.p;
+}
+
+const double* ScalarField::dvdp_values(const std::string& name, hipStream_t s) {
+  if (constant || name != "beta") return nullptr;
+  if (base.n != size_t(mesh->n)) {
+    base.alloc(mesh->n);
+    DBuf<double> tmp(mesh->n);
+    tmp.upload(values.data(), mesh->n, s);
+    launch_gather_ref(mesh->n, mesh->perm.p, tmp.p, base.p, s);
+    NOSH_HIP(hipStreamSynchronize(s));
+  }
+  return base.p;
 }
 
 }  // namespace nosh
@@ -160,15 +175,15 @@ void download_internal(MeshCache& m, Staging& st, const double* src, double* y_h
 }
 
 // thickness_->get_v(dummy) in reference order (for the alpha cache).
+// The reference asks with an empty parameter map, so a constant field gives
+// c and an explicit-values field throws (params.at("beta"),
+// src/parameter_matrix_keo.cpp:202-203).
 const double* thickness_ref(nosh_sfield_s* t, hipStream_t s) {
   MeshCache& m = *t->sf.mesh;
+  if (!t->sf.constant) need({}, "beta");
   if (t->ref.n != size_t(m.n)) {
     t->ref.alloc(m.n);
-    if (t->sf.constant) {
-      launch_fill(t->ref.p, m.n, t->sf.c, s);
-    } else {
-      t->ref.upload(t->sf.values.data(), m.n, s);
-    }
+    launch_fill(t->ref.p, m.n, t->sf.c, s);
   }
   return t->ref.p;
 }
@@ -358,6 +373,8 @@ int nosh_sfield_explicit_values_create(nosh_mesh_t mesh, const double* values,
     auto h = std::make_unique<nosh_sfield_s>();
     h->sf.mesh = mesh->m.get();
     h->sf.constant = false;
+    h->sf.param = "beta";
+    h->sf.param_init = 1.0;
     h->sf.values.assign(values, values + mesh->m->n);
     *out = h.release();
   });
@@ -414,7 +431,7 @@ static void keo_op(nosh_keo_s* keo, const double* x, double* y, hipStream_t s) {
       zero.alloc(m.n);
       zero.zero(s);
     }
-    launch_dfdp(m, keo->kv, zero.p, x, false, 0.0, y, s);
+    launch_dfdp(m, keo->kv, zero.p, x, false, 0.0, nullptr, y, s);
   } else {
     launch_keo_apply(m, keo->kv, x, y, s);
   }
@@ -478,6 +495,7 @@ static void jac_rebuild_dev(nosh_jac_s* jac, const std::map<std::string, double>
   launch_rebuild_blocks(m, jac->keo->kv, jac->ct.p, V, g, psi_int, jac->blk.p, nullptr, nullptr,
                         s);
   jac->built = true;
+  jac->blk32_valid = false;
 }
 
 int nosh_jac_rebuild(nosh_jac_t jac, int np, const char* const* names, const double* values,
@@ -533,8 +551,12 @@ int nosh_jac_apply_internal_fp32(nosh_jac_t jac, const float* x, float* y, void*
     MeshCache& m = *jac->mesh->m;
     hipStream_t s = stream_of(stream);
     if (!jac->keo->kv.valid32) launch_keo_to_fp32(m, jac->keo->kv, s);
-    if (jac->blk32.n != size_t(3 * m.n)) jac->blk32.alloc(3 * m.n);
-    convert_to<float>(jac->blk.p, jac->blk32.p, 3 * m.n, s);
+    // The FP32 copy of the blocks is made once per rebuild.
+    if (!jac->blk32_valid || jac->blk32.n != size_t(3 * m.n)) {
+      if (jac->blk32.n != size_t(3 * m.n)) jac->blk32.alloc(3 * m.n);
+      convert_to<float>(jac->blk.p, jac->blk32.p, 3 * m.n, s);
+      jac->blk32_valid = true;
+    }
     launch_block_apply32(m, jac->keo->kv, jac->blk32.p, x, y, s);
   });
 }
@@ -803,7 +825,8 @@ int nosh_nls_eval_dfdp(nosh_nls_t nls, const double* x, int np, const char* cons
     nls->st.ensure(2 * m.n);
     upload_internal(m, nls->st, x, nls->st.b.p, s);
     launch_dfdp(m, nls->dkeo->kv, nls->ct.p, nls->st.b.p, name == "g",
-                nls->V->sf.dvdp_constant(name), nls->st.c.p, s);
+                nls->V->sf.dvdp_constant(name), nls->V->sf.dvdp_values(name, s), nls->st.c.p,
+                s);
     download_internal(m, nls->st, nls->st.c.p, dfdp, s);
   });
 }

@@ -74,7 +74,7 @@ int tangent(nosh_nls_s* nls, ContWork& w, const std::string& param, double* psi_
   const double* V = nls->V->sf.get_v(p, s);
   // dF/dp: dK/dp psi + (p == "g" ? ct |psi|^2 psi : ct dV/dp psi).
   launch_dfdp(m, w.dk->kv, nls->ct.p, psi_int, param == "g", nls->V->sf.dvdp_constant(param),
-              w.rhs.p, s);
+              nls->V->sf.dvdp_values(param, s), w.rhs.p, s);
   if (nls->jblk.n != size_t(3 * n)) {
     nls->jblk.alloc(3 * n);
     nls->pblk.alloc(3 * n);
@@ -389,6 +389,7 @@ int nosh_surrogate_default_options(nosh_surrogate_options* o) {
     o->energy_tol = 1e-8;
     o->seed = 3;
     o->retrain_steps = 0;
+    o->smooth_sweeps = 0;
   });
 }
 

@@ -66,6 +66,7 @@ struct nosh_jac_s {
   DBuf<double> ct, blk, psi;
   DBuf<float> blk32;
   bool built = false;
+  bool blk32_valid = false;  // blk32 holds the FP32 copy of the current blk
   Staging st;
 };
 

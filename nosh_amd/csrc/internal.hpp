@@ -45,21 +45,30 @@ std::unique_ptr<MeshCache> build_mesh_cache(const double* coords, int64_t n, con
                                             int64_t T, int nv, bool reorder, hipStream_t s);
 
 // Scalar field on vertices (constant with optional parameter, or explicit).
+// scalar_field::constant (src/scalar_field_constant.cpp:43-75): V = c, or
+// c + params[param] when the field has a parameter and the map holds it.
+// scalar_field::explicit_values (src/scalar_field_explicit_values.cpp:33-57):
+// V = params.at("beta") * values (std::out_of_range without "beta"), with
+// the scalar parameter {"beta": 1}.
 struct ScalarField {
   MeshCache* mesh = nullptr;
   bool constant = true;
   double c = 0.0;
-  std::string param;  // "" = none
+  std::string param;  // "" = none; "beta" for explicit values
   double param_init = 0.0;
   std::vector<double> values;  // explicit values, reference order (host copy)
+  DBuf<double> base;           // explicit values, internal order (unscaled)
   DBuf<double> dev;            // internal order, current parameters
-  double dev_value = 0.0;      // constant value currently in dev
+  double dev_value = 0.0;      // constant (or beta) currently applied in dev
   bool dev_valid = false;
   std::map<std::string, double> scalar_parameters() const;
   // Value at the given parameters, written to dev (internal order).
   const double* get_v(const std::map<std::string, double>& p, hipStream_t s);
-  // dV/dp for parameter name (1 if it is this field's parameter, else 0).
+  // dV/dp (get_dvdp): a constant (1 if `name` is a constant field's
+  // parameter, else 0) and, for explicit values and "beta", the per-vertex
+  // values (internal order; null otherwise).
   double dvdp_constant(const std::string& name) const { return (constant && name == param) ? 1.0 : 0.0; }
+  const double* dvdp_values(const std::string& name, hipStream_t s);
 };
 
 // explicit_values magnetic potential: per-edge projection a_e / mu.
@@ -114,8 +123,9 @@ void launch_gather_ref(int64_t n, const int32_t* perm, const double* v_ref, doub
 void launch_keo_apply(const MeshCache& m, const KeoValues& kv, const double* x, double* y,
                       hipStream_t s);
 // y = dK x + (is_g ? ct |x|^2 x : ct dvdp x)   (dF/dp column).
+// y = dK/dp x + ct (is_g ? |x|^2 : dvdp + dvdp_vec[k]) x; dvdp_vec may be null.
 void launch_dfdp(const MeshCache& m, const KeoValues& kv, const double* ct, const double* x,
-                 bool is_g, double dvdp, double* y, hipStream_t s);
+                 bool is_g, double dvdp, const double* dvdp_vec, double* y, hipStream_t s);
 // f = K x + ct (V + g |x|^2) x; optional ||f||^2 into *nrm2 (device scalar).
 void launch_compute_f(const MeshCache& m, const KeoValues& kv, const double* ct,
                       const double* V, double g, const double* x, double* f, double* nrm2,
@@ -132,6 +142,7 @@ void launch_block_apply(const MeshCache& m, const KeoValues& kv, const double* b
 // Elementwise helpers.
 void launch_fill(double* a, int64_t n, double v, hipStream_t s);
 void launch_mul(const double* a, const double* b, double* c, int64_t n, hipStream_t s);
+void launch_scale(const double* a, double v, double* c, int64_t n, hipStream_t s);
 // y = K x (internal order), FP32 values and vectors.
 void launch_block_apply32(const MeshCache& m, const KeoValues& kv, const float* blk,
                           const float* x, float* y, hipStream_t s);

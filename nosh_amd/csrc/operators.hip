@@ -160,12 +160,13 @@ __global__ void k_finalize_store(const double* __restrict__ partials, int nb,
 __global__ __launch_bounds__(kBlock) void k_dfdp(
     int64_t n, const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ scol,
     const double2* __restrict__ dval, const double* __restrict__ ct, int is_g, double dvdp,
-    const double2* __restrict__ x, double2* __restrict__ y) {
+    const double* __restrict__ dvdp_vec, const double2* __restrict__ x, double2* __restrict__ y) {
   const int64_t row = xcd_block(true) * kBlock + threadIdx.x;
   if (row >= n) return;
   const double2 acc = sell_row<double>(row, slice_ptr, scol, dval, x);
   const double2 xi = x[row];
-  const double a = is_g ? ct[row] * (xi.x * xi.x + xi.y * xi.y) : ct[row] * dvdp;
+  const double dv = dvdp_vec ? dvdp + dvdp_vec[row] : dvdp;
+  const double a = is_g ? ct[row] * (xi.x * xi.x + xi.y * xi.y) : ct[row] * dv;
   y[row] = make_double2(acc.x + a * xi.x, acc.y + a * xi.y);
 }
 
@@ -298,9 +299,9 @@ void launch_keo_apply(const MeshCache& m, const KeoValues& kv, const double* x, 
 }
 
 void launch_dfdp(const MeshCache& m, const KeoValues& kv, const double* ct, const double* x,
-                 bool is_g, double dvdp, double* y, hipStream_t s) {
+                 bool is_g, double dvdp, const double* dvdp_vec, double* y, hipStream_t s) {
   k_dfdp<<<grid_for(m.n, kBlock), kBlock, 0, s>>>(
-      m.n, m.slice_ptr.p, m.scol.p, kv.dval.p, ct, is_g ? 1 : 0, dvdp,
+      m.n, m.slice_ptr.p, m.scol.p, kv.dval.p, ct, is_g ? 1 : 0, dvdp, dvdp_vec,
       reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y));
 }
 
@@ -353,7 +354,16 @@ __global__ void k_mul(const double* __restrict__ a, const double* __restrict__ b
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) c[i] = a[i] * b[i];
 }
+__global__ void k_scale(const double* __restrict__ a, double v, double* __restrict__ c,
+                        int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = v * a[i];
+}
 }  // namespace
+
+void launch_scale(const double* a, double v, double* c, int64_t n, hipStream_t s) {
+  k_scale<<<grid_for(n, kBlock), kBlock, 0, s>>>(a, v, c, n);
+}
 
 void launch_fill(double* a, int64_t n, double v, hipStream_t s) {
   k_fill<<<grid_for(n, kBlock), kBlock, 0, s>>>(a, n, v);

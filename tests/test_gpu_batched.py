@@ -125,19 +125,46 @@ def test_batched_autotune(setup, tmp_path, monkeypatch):
     _check_with_oracle(meshes, counts, mu, g, batch.split(psi))
 
 
-def test_persistent_mode_refuses_large_grids():
-    """Persistent (cooperative) mode accepts only grids of at most 64
-    workgroups; a larger batch is refused with an error and stays in graph
-    mode, which then solves it."""
+def test_persistent_mode_large_grid_matches_graph():
+    """Persistent mode on a batch whose row tiles outnumber its grid (one or
+    two workgroups per CU, each striding over the tiles): bitwise the graph
+    mode's result. With the barrier wait limit forced to 0 from the host, the
+    barriers time out, every workgroup leaves the kernel and the solve
+    reports an error instead of hanging; the batch then solves normally."""
+    import os
     import nosh_amd as nosh
-    c, t = mg.kuhn_cube(12)
+    c, t = mg.kuhn_cube(32)
     m = _model(c, t)
-    batch = nosh.Batch([(m, 8)], np.linspace(0.1, 0.5, 8), np.ones(8))
-    with pytest.raises(nosh.NoshError):
-        batch.set_tuning(128, 1, 4, 16)
-    psi0 = np.tile(np.stack([np.ones(c.shape[0]), np.zeros(c.shape[0])], 1).ravel(), 8)
-    psi, st = batch.solve(psi0, tol=1e-10, max_iters=30, minres_rtol=1e-6, forcing=0)
+    n = c.shape[0]
+    mu = np.linspace(0.02, 0.16, 8)
+    batch = nosh.Batch([(m, 8)], mu, np.ones(8))
+    psi0 = []
+    for b in range(8):
+        p = 0.1 * mg.random_state(n, 300 + b)
+        p[0::2] += 1.0
+        psi0.append(p)
+    psi0 = np.concatenate(psi0)
+    opts = dict(tol=1e-10, max_iters=60, minres_rtol=1e-6, minres_max_iters=2000, cheb_iters=4,
+                cheb_ratio=100.0, forcing=0)
+    out = {}
+    for mode in (0, 1):
+        print(f"[persistent large grid: mode {mode}, 256 threads, 35937 rows x 8 lanes]",
+              flush=True)
+        batch.set_tuning(256, mode, 4, 16, fold=1)
+        psi, st = batch.solve(psi0, **opts)
+        assert all(s["converged"] for s in st), st
+        out[mode] = psi
+    assert np.array_equal(out[0], out[1])
+    print("[persistent large grid: forced barrier time-out]", flush=True)
+    os.environ["NOSH_BARRIER_TIMEOUT_S"] = "0"
+    try:
+        with pytest.raises(nosh.NoshError):
+            batch.solve(psi0, **opts)
+    finally:
+        del os.environ["NOSH_BARRIER_TIMEOUT_S"]
+    psi, st = batch.solve(psi0, **opts)
     assert all(s["converged"] for s in st)
+    assert np.array_equal(psi, out[0])
 
 
 def test_full_group_matches_padded_group():
