@@ -356,6 +356,17 @@ int nosh_mlp_destroy(nosh_mlp_t mlp);
  * sum_k c_k Re(conj(phi_k) psi_k) / sum_k c_k. */
 int nosh_nls_gibbs_energy(nosh_nls_t nls, const double* psi, double* energy);
 int nosh_nls_inner_product(nosh_nls_t nls, const double* phi, const double* psi, double* value);
+/* The Gibbs energy without any allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). psi: 2n
+ * doubles of device memory in reference vertex order; energy: one double of
+ * device memory, written on `stream` (the caller synchronizes). The partial
+ * sums live in a fixed device array of the library (256 KiB per GPU, kept
+ * until the process ends), so meshes of more than 4,194,304 vertices are
+ * refused (NOSH_ERR_INVALID), and calls that overlap on one GPU race. Same
+ * value, bit for bit, as nosh_nls_gibbs_energy. */
+int nosh_nls_gibbs_energy_noalloc(nosh_nls_t nls, const double* psi, double* energy,
+                                  void* stream);
 /* ------------------------------------------------ measurement helper -----
  * No reference counterpart. Average seconds per launch of the library's own
  * streaming kernel `kind` (0 read, 1 write, 2 copy) over buffers of `bytes`

@@ -596,6 +596,7 @@ def _load_more():
     sig = {
         "nosh_nls_gibbs_energy": (i32, [vp, vp, C.POINTER(d)]),
         "nosh_nls_inner_product": (i32, [vp, vp, vp, C.POINTER(d)]),
+        "nosh_nls_gibbs_energy_noalloc": (i32, [vp, vp, vp, vp]),
         "nosh_bench_stream": (i32, [i32, C.c_int64, i32, i32, i32, C.POINTER(d)]),
         "nosh_continuation_default_options": (i32, [C.POINTER(ContinuationOptions)]),
         "nosh_continuation_run": (i32, [vp, vp, i32, cpp, vp, C.POINTER(ContinuationOptions),
@@ -616,6 +617,16 @@ def gibbs_energy(model: Nls, psi) -> float:
     v = C.c_double()
     _check(lib.nosh_nls_gibbs_energy(model._h, _ptr(x), C.byref(v)))
     return v.value
+
+
+def gibbs_energy_noalloc(model: Nls, psi_ptr: int, energy_ptr: int, stream: int = 0) -> None:
+    """The Gibbs energy with no allocation: psi (2n doubles, reference order)
+    and energy (1 double) are device pointers; the value is written on
+    `stream`. Meshes above 4,194,304 vertices are refused; calls that overlap
+    on one GPU race on the library's scratch array."""
+    lib = _load_more()
+    _check(lib.nosh_nls_gibbs_energy_noalloc(model._h, C.c_void_p(psi_ptr),
+                                             C.c_void_p(energy_ptr), C.c_void_p(stream)))
 
 
 def bench_stream(kind: str, nbytes: int, reps: int = 20, blocks_per_cu: int = 8,

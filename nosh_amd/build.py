@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libnosh_amd.so")
 
 SOURCES = ["mesh_cache.hip", "operators.hip", "solver.hip", "batched.hip", "capi.cpp",
            "continuation.cpp", "batch_capi.cpp", "batch_continuation.cpp", "mlp.hip",
-           "mlp_capi.cpp", "eigen.hip", "arclength.cpp", "stream.hip"]
+           "mlp_capi.cpp", "eigen.hip", "arclength.cpp", "stream.hip", "noalloc.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
