@@ -94,12 +94,19 @@ def parse():
                    help="branches run with the plain FP64 continuation for the ratio")
     p.add_argument("--no-ablation", action="store_true",
                    help="skip the single-instance ablation rows")
+    p.add_argument("--no-persistent", action="store_true",
+                   help="skip the persistent-vs-graph MINRES comparison")
+    p.add_argument("--no-extras", action="store_true",
+                   help="only the timed continuation (no second step size, no surrogate-free "
+                        "run, no ablation; for profilers)")
     p.add_argument("--cont-tuning", default="",
                    help="block_threads,fold[,mode] for the batched continuation (skips the "
                         "autotuner; mode 0 by default)")
     p.add_argument("--no-surrogate", action="store_true")
     p.add_argument("--cont-smooth", type=int, default=4,
                    help="Jacobi sweeps on the residual of each surrogate guess")
+    p.add_argument("--cont-accept", type=float, default=0.5,
+                   help="surrogate guess used when ||F|| <= this times the secant guess's")
     p.add_argument("--workload", default="cont96",
                    choices=["cont96", "batched", "cont", "disk", "stability"],
                    help="cont96: the headline (continuation, side-by-side solves, roofline, "
@@ -366,7 +373,7 @@ def continuation_phase(args, model, n):
     sur = None if args.no_surrogate else dict(
         width=256, depth=4, train_after=args.cont_train_after, train_steps=args.cont_train_steps,
         batch=args.cont_batch, lr=1e-3, newton_budget=6, retrain_steps=args.cont_retrain,
-        smooth_sweeps=args.cont_smooth)
+        smooth_sweeps=args.cont_smooth, accept_ratio=args.cont_accept)
     if args.warmup > 0:
         t0 = time.perf_counter()
         batch.continuation(psi0, span * args.warmup / steps, args.warmup, **kw)
@@ -406,33 +413,71 @@ def continuation_phase(args, model, n):
         fp64_err.append(float(np.linalg.norm(_phase_align(finals[b], p) - p) /
                               np.linalg.norm(p)))
     say(f"continuation: mixed vs FP64-polished final states, rel. err. max {max(fp64_err):.2e}")
+    # Persistent (cooperative) MINRES against the tuned hipGraph mode: one
+    # batched Newton solve of the 8 branches at mu_0 + span from their final
+    # states perturbed by 1e-4, same block size and fold (bitwise equal
+    # results expected).
+    pers = None
+    if not args.no_persistent:
+        from nosh_amd import meshgen as mg
+        bp = nosh.Batch([(model, nb)], mu0 + span, g)
+        p_start = psi + 1e-4 * np.concatenate([mg.random_state(n, 900 + b) for b in range(nb)])
+        kw_p = dict(kw, forcing=1)
+        res = {}
+        for mode in sorted({0, 1, tuned[1]}):
+            bp.set_tuning(tuned[0], mode, 4, 16, fold=tuned[2])
+            bp.solve(p_start, **kw_p)  # warm-up (graph capture, first launch)
+            t1 = time.perf_counter()
+            p_m, st_m = bp.solve(p_start, **kw_p)
+            res[mode] = (time.perf_counter() - t1, p_m, st_m)
+        t_g, p_g, st_g = res[0]
+        t_p, p_p, st_p = res[1]
+        pers = {"block_threads": tuned[0], "fold": tuned[2],
+                "graph_seconds": round(t_g, 4), "persistent_seconds": round(t_p, 4),
+                "tuned_mode": MODE_NAMES[tuned[1]],
+                "tuned_mode_seconds": round(res[tuned[1]][0], 4),
+                "bitwise_equal_persistent_vs_graph": bool(np.array_equal(p_g, p_p)),
+                "newton_iters": max(s["newton_iters"] for s in st_g),
+                "minres_iters": max(s["minres_iters"] for s in st_g),
+                "converged": all(s["converged"] for s in st_p),
+                "note": "one batched Newton solve of the 8 branches at the end of the span; "
+                        "graph = hipGraph mode 0 (one lane per thread), persistent = mode 1, "
+                        "same block size and fold, so the same arithmetic"}
+        say(f"continuation: persistent MINRES {t_p:.3f}s vs hipGraph {t_g:.3f}s "
+            f"({MODE_NAMES[tuned[1]]} {res[tuned[1]][0]:.3f}s), bitwise equal "
+            f"{pers['bitwise_equal_persistent_vs_graph']}")
+        del bp
     # The batched run without the surrogate (the ablation's batching row, and
     # the surrogate's net gain in this same process).
-    t_ns, rec_ns, _, _ = _batched_run(batch, psi0, span, steps, kw, None)
-    no_sur = _run_summary(nb, steps, t_ns, rec_ns)
-    say(f"continuation: batched without surrogate {t_ns:.2f}s -> {no_sur['solves_per_s']:.3f}/s")
-    # The other step size of the workload.
     other_steps = 50 if steps != 50 else 20
-    t_o, rec_o, sst_o, _ = _batched_run(batch, psi0, span, other_steps, kw, sur)
-    other = _run_summary(nb, other_steps, t_o, rec_o)
-    other["surrogate"] = sst_o
-    say(f"continuation: combined at {other_steps} steps {t_o:.2f}s -> "
-        f"{other['solves_per_s']:.3f}/s")
+    no_sur, other = {"solves_per_s": None}, None
+    if not args.no_extras:
+        t_ns, rec_ns, _, _ = _batched_run(batch, psi0, span, steps, kw, None)
+        no_sur = _run_summary(nb, steps, t_ns, rec_ns)
+        say(f"continuation: batched without surrogate {t_ns:.2f}s -> "
+            f"{no_sur['solves_per_s']:.3f}/s")
+        # The other step size of the workload.
+        t_o, rec_o, sst_o, _ = _batched_run(batch, psi0, span, other_steps, kw, sur)
+        other = _run_summary(nb, other_steps, t_o, rec_o)
+        other["surrogate"] = sst_o
+        say(f"continuation: combined at {other_steps} steps {t_o:.2f}s -> "
+            f"{other['solves_per_s']:.3f}/s")
     del batch
     # Plain FP64 single-instance continuation of every branch.
     branches = list(range(min(args.cont_plain_branches, nb)))
     plain_rate, plain_rec = _plain_continuation(model, n, psi0, g, mu0, span, steps, branches,
                                                 args) if branches else (None, [])
     two = sorted({0, nb - 1})
-    plain_o_rate, plain_o_rec = _plain_continuation(model, n, psi0, g, mu0, span, other_steps,
-                                                    two, args)
-    other["plain_fp64_solves_per_s"] = plain_o_rate
-    other["plain_sample"] = plain_o_rec
-    other["speedup_vs_plain_fp64"] = (other["solves_per_s"] / plain_o_rate
-                                      if plain_o_rate else None)
+    if other is not None:
+        plain_o_rate, plain_o_rec = _plain_continuation(model, n, psi0, g, mu0, span,
+                                                        other_steps, two, args)
+        other["plain_fp64_solves_per_s"] = plain_o_rate
+        other["plain_sample"] = plain_o_rec
+        other["speedup_vs_plain_fp64"] = (other["solves_per_s"] / plain_o_rate
+                                          if plain_o_rate else None)
     # Ablation at the headline step size (single-instance rows on two branches).
     abl = {"plain FP64 (tangent predictor, MINRES rtol 1e-10)": plain_rate}
-    if not args.no_ablation:
+    if not args.no_ablation and not args.no_extras:
         r_ew, _ = _plain_continuation(model, n, psi0, g, mu0, span, steps, two, args, forcing=1,
                                       forcing_min=1e-6, forcing_max=args.cont_forcing_max)
         abl["+ Eisenstat-Walker forcing (FP64 inner)"] = r_ew
@@ -459,8 +504,9 @@ def continuation_phase(args, model, n):
                                            if seeded else None),
         "without_surrogate": no_sur,
         "surrogate_net_gain": (comb_rate / no_sur["solves_per_s"]
-                               if no_sur["solves_per_s"] else None),
+                               if no_sur.get("solves_per_s") else None),
         "dominant_kernel_roofline": cheb_roof,
+        "persistent_vs_graph": pers,
         "plain_fp64_solves_per_s": plain_rate, "plain_sample": plain_rec,
         "speedup_vs_plain_fp64": comb_rate / plain_rate if plain_rate else None,
         f"at_{other_steps}_steps": other,
@@ -486,23 +532,22 @@ def _event_time(stream, fn, reps, warm=3):
 
 
 def _cold_time(stream, fn, reps, flush):
-    """Median seconds of fn() with a 1 GiB read (flush.sum()) before each
-    call: the read replaces the 256 MB Infinity Cache and the L2s with clean
-    lines, so fn streams its operands from HBM (a write-based flush would
-    leave dirty lines whose write-back the timed call would pay for)."""
+    """Seconds of fn() with cold caches: each call follows a 1 GiB read
+    (flush.sum()), which replaces the 256 MB Infinity Cache and the L2s with
+    clean lines, so fn streams its operands from HBM (a write-based flush
+    would leave dirty lines whose write-back fn would pay for). Measured as
+    the difference of two back-to-back sequences timed with HIP events on
+    `stream`, reps x (read, fn) and reps x (read) alone, so the launch gaps
+    of single timed calls do not count (they add ~6 us to a 57 us kernel)."""
     import torch
-    ev = []
-    with torch.cuda.stream(stream):
-        for _ in range(reps):
-            flush.sum()
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            fn()
-            e1.record(stream)
-            ev.append((e0, e1))
-    torch.cuda.synchronize()
-    return float(np.median([a.elapsed_time(b) for a, b in ev])) / 1e3
+
+    def pair():
+        flush.sum()
+        fn()
+
+    t_pair = _event_time(stream, pair, reps, warm=2)
+    t_read = _event_time(stream, lambda: flush.sum(), reps, warm=2)
+    return t_pair - t_read
 
 
 def stream_ceiling():
@@ -682,18 +727,31 @@ def single_solves(args, model, params, psi0, stream, dev):
     Newton-MINRES, from two starts: the seeded random state (|psi| ~
     U(0.5,1), arg ~ U(0, 2 pi); on this fine mesh the random phases cancel
     and both variants converge to the trivial state psi = 0, which the
-    record names), and a superconducting start (psi = 1 plus 10 % seeded
-    noise), which both take to the same nontrivial state. Reported per
-    variant: ||F||, ||psi|| and the phase-aligned ||psi - psi_FP64||
-    (absolute, and relative when psi_FP64 is nontrivial)."""
+    record names), and a nontrivial vortex state perturbed by 1e-3, which
+    both take back to that state. Reported per variant: ||F||, ||psi|| and
+    the phase-aligned ||psi - psi_FP64|| (absolute, and relative when
+    psi_FP64 is nontrivial)."""
     import torch
     import nosh_amd as nosh
     from nosh_amd import meshgen as mg
     n = psi0.size // 2
     s1 = 0.1 * mg.random_state(n, 5)
     s1[0::2] += 1.0
-    out = {}
-    for start, p0 in (("random", psi0), ("superconducting", s1)):
+    # A nontrivial solution (plain FP64 Newton from the superconducting
+    # start lands on a vortex state), perturbed by 1e-3 seeded noise: from
+    # there both variants converge back to it. (From the far start itself
+    # they wander to different vortex states.)
+    t0 = time.perf_counter()
+    p_sc, st_sc = nosh.newton_solve(model, s1, params, tol=1e-10, max_iters=60, minres_rtol=1e-10,
+                                    minres_max_iters=args.minres_maxit, cheb_iters=args.cheb,
+                                    cheb_ratio=args.ratio)
+    say(f"side by side: FP64 from the superconducting start {time.perf_counter() - t0:.2f}s, "
+        f"newton {st_sc['newton_iters']}, ||psi|| {np.linalg.norm(p_sc):.3e}")
+    out = {"vortex_state_from_superconducting_start": {
+        "newton_iters": st_sc["newton_iters"], "converged": bool(st_sc["converged"]),
+        "norm_psi": float(np.linalg.norm(p_sc))}}
+    for start, p0 in (("random", psi0),
+                      ("perturbed vortex state", p_sc + 1e-3 * mg.random_state(n, 6))):
         psi_dev0 = torch.from_numpy(p0).to(dev)
         psi = torch.empty_like(psi_dev0)
         results, sols = {}, {}

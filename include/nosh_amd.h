@@ -220,8 +220,12 @@ int nosh_nls_create_w_prec(nosh_nls_t nls, nosh_prec_t* out);
 int nosh_nls_destroy(nosh_nls_t nls);
 
 /* --------------------------------------------------------- solvers -------
- * The in-repo replacement of Belos MINRES (selected at examples/conf.xml:105,
- * :117-123) and the NOX full-step Newton (examples/conf.xml:77-190). */
+ * The in-repo replacement of the Belos linear solver and the NOX full-step
+ * Newton (examples/conf.xml:77-190). The reference's example selects Pseudo
+ * Block GMRES (examples/conf.xml:104; the MINRES line :105 is commented
+ * out, its parameters :117-123 remain); J is real symmetric and
+ * keo_regularized SPD, so MINRES is the short-recurrence Krylov method for
+ * the same systems, with the same tolerance 1e-10 and limit 1000. */
 typedef struct {
   double tol;           /* stop when ||F(psi)||_2 <= tol (default 1e-10) */
   int max_iters;        /* Newton steps (default 30) */
@@ -446,6 +450,8 @@ typedef struct {
   int smooth_sweeps;    /* batched driver: damped Jacobi sweeps on the residual
                            of every surrogate guess before it is compared with
                            the secant guess (default 0) */
+  double accept_ratio;  /* a surrogate guess is used when its ||F|| is at most
+                           this fraction of the predictor guess's (0.5) */
 } nosh_surrogate_options;
 
 typedef struct {

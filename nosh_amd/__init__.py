@@ -920,7 +920,8 @@ class SurrogateOptions(C.Structure):
     _fields_ = [("width", C.c_int), ("depth", C.c_int), ("train_after", C.c_int),
                 ("train_steps", C.c_int), ("batch", C.c_int), ("lr", C.c_float),
                 ("newton_budget", C.c_int), ("energy_tol", C.c_double), ("seed", C.c_uint64),
-                ("retrain_steps", C.c_int), ("smooth_sweeps", C.c_int)]
+                ("retrain_steps", C.c_int), ("smooth_sweeps", C.c_int),
+                ("accept_ratio", C.c_double)]
 
 
 class SurrogateStats(C.Structure):

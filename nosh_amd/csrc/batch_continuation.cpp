@@ -9,9 +9,12 @@
 // With a surrogate, after `train_after` steps a per-vertex MLP (BF16 MFMA,
 // trained on the run's own solutions) seeds Newton with
 //   psi_guess = psi_sec + h^2 sigma Y(x, psi_k, h s, mu, g),   h s = psi_sec - psi_k,
-// sigma the RMS of the training targets (psi_{k+1} - psi_sec) / h^2; a
-// branch starts from the surrogate guess when its ||F|| is at most half the
-// secant guess's, else from the secant guess. A
+// sigma the RMS of the training targets (psi_{k+1} - psi_sec) / h^2, then
+// smoothed by smooth_sweeps damped Jacobi sweeps on its residual; a branch
+// starts from the surrogate guess when its ||F|| is at most accept_ratio
+// times the secant guess's (inexact Newton takes as many steps, and often
+// more MINRES iterations, from a guess that is only slightly better), else
+// from the secant guess. A
 // branch whose Newton from the surrogate guess does not converge within the
 // budget, or whose Gibbs energy leaves its linear trend, is solved again
 // from psi_sec; at a branch transition, where Newton from the secant guess
@@ -75,11 +78,6 @@ constexpr int kMaxHalvings = 3;   // step splits (2, 4, 8 sub-steps) for failed 
 constexpr double kPtcShift0 = 0.1;
 constexpr double kPtcForcing = 1e-4;
 constexpr int kPtcCheb = 6;
-// A surrogate guess is used only when its residual is at most this fraction
-// of the secant guess's: inexact Newton (Eisenstat-Walker forcing) takes the
-// same number of steps, and often more MINRES iterations, from a guess that
-// is only slightly better.
-constexpr double kSurrogateMargin = 0.5;
 // Damping of the Jacobi sweeps that smooth the surrogate's guess.
 constexpr double kSmoothOmega = 0.6;
 constexpr int kRed = 256;       // threads of the per-lane reduction blocks
@@ -278,7 +276,8 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
   int rc = bcguard([&] {
     NOSH_CHECK(bt && psi && record && steps >= 1, kInvalid, "invalid batch continuation call");
     NOSH_CHECK(!so || (so->train_after >= 1 && so->batch >= 64 && so->newton_budget >= 1 &&
-                       so->retrain_steps >= 0 && so->smooth_sweeps >= 0),
+                       so->retrain_steps >= 0 && so->smooth_sweeps >= 0 &&
+                       so->accept_ratio > 0.0),
                kInvalid, "bad surrogate options");
     nosh_newton_options o;
     if (opts)
@@ -521,7 +520,7 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
                                "[%.3g, %.3g], ||F(sec)|| %.3g\n", k, rmin, rmax, nf_sec[0]);
         }
         for (int i = 0; i < ni; ++i) {
-          use_sur[i] = nf_sur[i] < kSurrogateMargin * nf_sec[i];
+          use_sur[i] = nf_sur[i] < so->accept_ratio * nf_sec[i];
           maxit[i] = use_sur[i] ? so->newton_budget : std::min(o.max_iters, kShortNewton);
           tried_sec[i] = !use_sur[i];
           nsur += use_sur[i];

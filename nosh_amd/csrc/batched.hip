@@ -177,8 +177,11 @@ __device__ __forceinline__ void slot_value(float al, float at, float mu, float& 
   vr = -al * cs;
   vi = al * sn;
 }
+// FP64: also explicit, so that the residual from precomputed slot values
+// and the per-lane rebuild give the same bits.
 __device__ __forceinline__ void slot_value(double al, double at, double mu, double& vr,
                                            double& vi) {
+#pragma clang fp contract(off)
   double sn, cs;
   sc(mu * at, &sn, &cs);
   vr = -al * cs;
@@ -191,8 +194,9 @@ __device__ __forceinline__ void cmac(float& ar, float& ai, float vr, float vi, f
   ai += __builtin_fmaf(vi, x.x, vr * x.y);
 }
 __device__ __forceinline__ void cmac(double& ar, double& ai, double vr, double vi, double2 x) {
-  ar += vr * x.x - vi * x.y;
-  ai += vr * x.y + vi * x.x;
+#pragma clang fp contract(off)
+  ar += __builtin_fma(vr, x.x, -(vi * x.y));
+  ai += __builtin_fma(vi, x.x, vr * x.y);
 }
 // acc + (b0 b2; b2 b1) z.
 __device__ __forceinline__ float2 blk_add(float2 acc, float b0, float b1, float b2, float2 z) {
@@ -433,6 +437,18 @@ __device__ __forceinline__ double2 brow_val64(const BGroup& G, int64_t row, int 
 
 // F and |F|^2 of one virtual block; VAL64: every group has kval64 (a kernel
 // of its own, so the sine/cosine path does not set its register count).
+// f = acc + (diag + ct (V + g |x|^2)) x and |f|^2, every multiply-add an
+// explicit fused operation (the same bits whichever row product gave acc).
+__device__ __forceinline__ double residual_row(double diag, double ct, double V, double g,
+                                               double2 xi, double2 acc, double2* f) {
+#pragma clang fp contract(off)
+  const double r2 = __builtin_fma(xi.x, xi.x, xi.y * xi.y);
+  const double d = __builtin_fma(ct, __builtin_fma(g, r2, V), diag);
+  const double2 r = make_double2(__builtin_fma(d, xi.x, acc.x), __builtin_fma(d, xi.y, acc.y));
+  *f = r;
+  return __builtin_fma(r.x, r.x, r.y * r.y);
+}
+
 template <bool VAL64>
 __device__ double ph_compute_f(const BatchArgs& a, int64_t vb) {
   const Lane L = lane_ctx(a, vb);
@@ -446,10 +462,7 @@ __device__ double ph_compute_f(const BatchArgs& a, int64_t vb) {
     else
       acc = brow<double>(G, L.row, L.b, mu, G.psi);
     const double2 xi = G.psi[L.idx];
-    const double d = G.diag[L.row] + G.ct[L.row] * (G.V[L.row] + g * (xi.x * xi.x + xi.y * xi.y));
-    const double2 r = make_double2(acc.x + d * xi.x, acc.y + d * xi.y);
-    G.f[L.idx] = r;
-    s2 = r.x * r.x + r.y * r.y;
+    s2 = residual_row(G.diag[L.row], G.ct[L.row], G.V[L.row], g, xi, acc, G.f + L.idx);
   }
   return s2;
 }
@@ -1308,7 +1321,15 @@ int Batch::persistent_grid() {
   NOSH_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kb_minres_persistent)));
   NOSH_CHECK(fa.maxThreadsPerBlock >= block_threads, kUnsupported,
              "persistent kernel: block size above the kernel's limit");
-  const int per_cu = block_threads <= 256 ? kPersistentPerCu : 1;
+  // Workgroups per CU the kernel's registers allow (512 VGPRs per SIMD lane,
+  // allocated in granules of 8, at most 8 waves per SIMD, 4 SIMDs per CU),
+  // capped at kPersistentPerCu.
+  const int vgpr = std::max(8, (std::max(fa.numRegs, 1) + 7) / 8 * 8);
+  const int waves_simd = std::min(8, 512 / vgpr);
+  const int waves_block = (block_threads + 63) / 64;
+  const int fit = (4 * waves_simd) / waves_block;
+  // (A register count below 16 cannot be this kernel's: one per CU then.)
+  const int per_cu = fa.numRegs < 16 ? 1 : std::max(1, std::min(kPersistentPerCu, fit));
   return int(std::min<int64_t>(int64_t(per_cu) * cus, args.total_blocks / args.fold));
 }
 

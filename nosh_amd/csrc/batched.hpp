@@ -97,8 +97,8 @@ struct GroupHost {
   BGroup dev{};
 };
 
-// Workgroups per CU of the persistent (cooperative) kernel's grid: one, or
-// two for blocks of at most 256 threads. The kernel strides over all row
+// Workgroups per CU of the persistent (cooperative) kernel's grid: at most
+// two, and only as many as its registers allow. The kernel strides over all row
 // tiles, so any grid size is correct as long as every workgroup is resident
 // at once; a grid sized to the occupancy limit (occupancy x CUs) leaves no
 // margin, since the dispatcher hands workgroups to the XCDs round-robin and

@@ -298,7 +298,6 @@ int nosh_continuation_run(nosh_nls_t nls, double* psi, int np, const char* const
 
 namespace {
 
-constexpr double kSurrogateMargin = 0.5;
 
 // X[i] = (x*cs, y*cs[, z*cs], Re psi, Im psi, h Re t, h Im t, mu) (internal order).
 __global__ void k_features(int64_t n, int dim, const int32_t* __restrict__ perm,
@@ -390,6 +389,7 @@ int nosh_surrogate_default_options(nosh_surrogate_options* o) {
     o->seed = 3;
     o->retrain_steps = 0;
     o->smooth_sweeps = 0;
+    o->accept_ratio = 0.5;
   });
 }
 
@@ -403,7 +403,7 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int np, const c
   int rc = cguard([&] {
     NOSH_CHECK(nls && psi && opts && so && record, kInvalid, "null argument");
     NOSH_CHECK(opts->steps >= 1 && so->train_after >= 1 && so->batch >= 64 &&
-                   so->retrain_steps >= 0,
+                   so->retrain_steps >= 0 && so->accept_ratio > 0.0,
                kInvalid, "bad continuation / surrogate options");
     const std::string param = opts->parameter ? opts->parameter : "mu";
     NOSH_CHECK(param == "mu" || param == "g", kInvalid, "continuation parameter must be mu or g");
@@ -529,7 +529,7 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int np, const c
         p[param] = target;
         const double nf_g = residual_norm(nls, guess.p, p, s);
         const double nf_t = residual_norm(nls, tanbuf.p, p, s);
-        use_guess = nf_g <= kSurrogateMargin * nf_t;
+        use_guess = nf_g <= so->accept_ratio * nf_t;
       }
       if (use_guess) {
         NOSH_HIP(hipMemcpyAsync(x, guess.p, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, s));

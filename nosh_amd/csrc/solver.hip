@@ -1,7 +1,10 @@
 // solver.hip -- the preconditioner (Chebyshev iteration with 2x2 block-Jacobi
 // scaling, standing in for the MueLu AMG cycle of keo_regularized::apply,
 // src/keo_regularized.cpp:88-165) and preconditioned MINRES (Paige-Saunders),
-// the in-repo replacement of Belos MINRES (examples/conf.xml:105,117-123).
+// the in-repo replacement of the Belos solver (the reference example selects
+// Pseudo Block GMRES at examples/conf.xml:104 with its MINRES line :105
+// commented out; J is symmetric and the preconditioner SPD, so MINRES solves
+// the same systems with a short recurrence, tolerance and limit as :117-123).
 //
 // MINRES runs as a fixed sequence of fused kernels per iteration with every
 // scalar of the Lanczos/Givens recurrences kept on the device:

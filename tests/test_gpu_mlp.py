@@ -89,3 +89,33 @@ def test_fused_inference_matches_forward(width, M):
     if M <= 1000:
         y_bf, _, _ = MO.forward(net.get_params(), X, net.dims, bf=True)
         assert np.linalg.norm(y_i - y_bf) / np.linalg.norm(y_bf) < 5e-3
+
+
+def test_first_adam_step_closed_form():
+    """Adam's first step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected)
+    has a closed form: m_hat = g and v_hat = g^2, so every parameter moves by
+    -lr g / (|g| + eps), i.e. -lr sign(g) where |g| >> eps. The step is
+    checked against the gradient the library reports for the same minibatch
+    (FP32 master weights: within a few ulps of the parameter)."""
+    import torch
+    net = _net(seed=5)
+    rng = np.random.default_rng(6)
+    M = 2048
+    X = rng.normal(0, 1, (M, 7)).astype(np.float32)
+    T = rng.normal(0, 1, (M, 2)).astype(np.float32)
+    p0 = net.get_params().astype(np.float64)
+    _, g = net.gradients(X, T)
+    g = g.astype(np.float64)
+    dev = torch.device("cuda:0")
+    Xd = torch.from_numpy(X).to(dev)
+    Td = torch.from_numpy(T).to(dev)
+    lr = 1e-3
+    net.train_step_device(Xd.data_ptr(), Td.data_ptr(), M, lr)
+    torch.cuda.synchronize()
+    d = net.get_params().astype(np.float64) - p0
+    want = -lr * g / (np.abs(g) + 1e-8)
+    tol = 4 * np.finfo(np.float32).eps * np.maximum(np.abs(p0), lr) + 1e-3 * lr
+    assert np.all(np.abs(d - want) <= tol), np.max(np.abs(d - want) - tol)
+    big = np.abs(g) > 1e-5
+    assert big.sum() > 0.9 * g.size
+    np.testing.assert_allclose(d[big], -lr * np.sign(g[big]), rtol=0, atol=2e-3 * lr + 1e-6)

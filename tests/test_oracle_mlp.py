@@ -38,3 +38,18 @@ def test_gradient_matches_finite_differences():
         lp, _ = M.loss_and_grad(p + e, X, T, dims)
         lm, _ = M.loss_and_grad(p - e, X, T, dims)
         assert abs((lp - lm) / (2 * h) - g[i]) < 1e-7 * max(1.0, abs(g[i]))
+
+
+def test_gelu_matches_library_fixture():
+    """The oracle's GELU and its derivative against PyTorch's tanh-form GELU
+    and autograd derivative (tests/golden/gelu.npz, written in this
+    repository by tests/golden/make_gelu.py): a dropped term or a wrong
+    constant (0.044715, sqrt(2/pi)) would miss by far more than 1e-12."""
+    import os
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", "gelu.npz"))
+    z = f["z"]
+    assert np.max(np.abs(M.gelu(z) - f["gelu"])) < 1e-12
+    assert np.max(np.abs(M.dgelu(z) - f["dgelu"])) < 1e-12
+    # Limits: GELU(z) -> z for large z, -> 0 for very negative z, GELU(0) = 0.
+    assert abs(M.gelu(np.array([0.0]))[0]) == 0.0
+    assert abs(M.gelu(np.array([8.0]))[0] - 8.0) < 1e-12
