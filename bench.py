@@ -103,6 +103,8 @@ def parse():
                    help="block_threads,fold[,mode] for the batched continuation (skips the "
                         "autotuner; mode 0 by default)")
     p.add_argument("--no-surrogate", action="store_true")
+    p.add_argument("--cont-predictor", default="tangent", choices=["tangent", "secant"],
+                   help="predictor of the batched continuation")
     p.add_argument("--cont-smooth", type=int, default=4,
                    help="Jacobi sweeps on the residual of each surrogate guess")
     p.add_argument("--cont-accept", type=float, default=0.5,
@@ -318,10 +320,11 @@ def _plain_continuation(model, n, psi0, g, mu0, span, steps, branches, args, **e
     return (len(branches) * steps / t_all if ok and t_all > 0 else 0.0), recs
 
 
-def _batched_run(batch, psi0, span, steps, kw, sur):
+def _batched_run(batch, psi0, span, steps, kw, sur, predictor):
     """One batched continuation; returns (seconds, records, surrogate stats, psi)."""
     t0 = time.perf_counter()
-    psi, rec, sst = batch.continuation(psi0, span, steps, surrogate=sur, **kw)
+    psi, rec, sst = batch.continuation(psi0, span, steps, surrogate=sur, predictor=predictor,
+                                       **kw)
     return time.perf_counter() - t0, rec, sst, psi
 
 
@@ -376,10 +379,11 @@ def continuation_phase(args, model, n):
         smooth_sweeps=args.cont_smooth, accept_ratio=args.cont_accept)
     if args.warmup > 0:
         t0 = time.perf_counter()
-        batch.continuation(psi0, span * args.warmup / steps, args.warmup, **kw)
+        batch.continuation(psi0, span * args.warmup / steps, args.warmup,
+                           predictor=args.cont_predictor, **kw)
         say(f"continuation: warmup {args.warmup} steps {time.perf_counter() - t0:.2f}s")
     # ---- timed region
-    t_comb, rec, sst, psi = _batched_run(batch, psi0, span, steps, kw, sur)
+    t_comb, rec, sst, psi = _batched_run(batch, psi0, span, steps, kw, sur, args.cont_predictor)
     # ----
     head = _run_summary(nb, steps, t_comb, rec)
     comb_rate = head["solves_per_s"]
@@ -452,12 +456,14 @@ def continuation_phase(args, model, n):
     other_steps = 50 if steps != 50 else 20
     no_sur, other = {"solves_per_s": None}, None
     if not args.no_extras:
-        t_ns, rec_ns, _, _ = _batched_run(batch, psi0, span, steps, kw, None)
+        t_ns, rec_ns, _, _ = _batched_run(batch, psi0, span, steps, kw, None,
+                                          args.cont_predictor)
         no_sur = _run_summary(nb, steps, t_ns, rec_ns)
         say(f"continuation: batched without surrogate {t_ns:.2f}s -> "
             f"{no_sur['solves_per_s']:.3f}/s")
         # The other step size of the workload.
-        t_o, rec_o, sst_o, _ = _batched_run(batch, psi0, span, other_steps, kw, sur)
+        t_o, rec_o, sst_o, _ = _batched_run(batch, psi0, span, other_steps, kw, sur,
+                                            args.cont_predictor)
         other = _run_summary(nb, other_steps, t_o, rec_o)
         other["surrogate"] = sst_o
         say(f"continuation: combined at {other_steps} steps {t_o:.2f}s -> "
@@ -485,7 +491,7 @@ def continuation_phase(args, model, n):
                                       forcing_min=1e-6, forcing_max=args.cont_forcing_max,
                                       precision=1)
         abl["+ FP32 inner MINRES"] = r_32
-    abl["+ batching (8 branches in lockstep, secant predictor, Chebyshev "
+    abl[f"+ batching (8 branches in lockstep, {args.cont_predictor} predictor, Chebyshev "
         f"{args.cont_cheb})"] = no_sur["solves_per_s"]
     abl["+ surrogate (the headline)"] = comb_rate
     seeded = [r for r in rec if r["seeded"]]
@@ -531,25 +537,6 @@ def _event_time(stream, fn, reps, warm=3):
     return e0.elapsed_time(e1) / 1e3 / reps
 
 
-def _cold_time(stream, fn, reps, flush):
-    """Seconds of fn() with cold caches: each call follows a 1 GiB read
-    (flush.sum()), which replaces the 256 MB Infinity Cache and the L2s with
-    clean lines, so fn streams its operands from HBM (a write-based flush
-    would leave dirty lines whose write-back fn would pay for). Measured as
-    the difference of two back-to-back sequences timed with HIP events on
-    `stream`, reps x (read, fn) and reps x (read) alone, so the launch gaps
-    of single timed calls do not count (they add ~6 us to a 57 us kernel)."""
-    import torch
-
-    def pair():
-        flush.sum()
-        fn()
-
-    t_pair = _event_time(stream, pair, reps, warm=2)
-    t_read = _event_time(stream, lambda: flush.sum(), reps, warm=2)
-    return t_pair - t_read
-
-
 def stream_ceiling():
     """Attainable HBM bandwidth from the library's own streaming kernels
     (read, write, copy over 2 GiB buffers; non-temporal and ordinary
@@ -569,10 +556,12 @@ def stream_ceiling():
     return out
 
 
-def jphi_timings(args, mesh, model, params, psi0, phi, stream, dev, flush, fp32=True):
-    """FP64 (and FP32) J(psi) phi on the library's launch stream: warm
-    (back-to-back) and cold (1 GiB read before each launch) average launch
-    times against the canonical bytes 40E + 68n (FP32: 24E + 36n)."""
+def jphi_timings(args, mesh, model, params, psi0, phi, stream, dev, fp32=True):
+    """FP64 (and FP32) J(psi) phi on the library's launch stream: average
+    kernel durations, stamped by the runtime on each kernel dispatch (HIP
+    events; launch gaps excluded), warm (back-to-back) and cold (a 1 GiB read
+    before each launch), against the canonical bytes 40E + 68n (FP32:
+    24E + 36n)."""
     import torch
     E, n = mesh.num_edges, mesh.num_vertices
     J = model.create_W_op()
@@ -580,19 +569,19 @@ def jphi_timings(args, mesh, model, params, psi0, phi, stream, dev, flush, fp32=
     x_ref = torch.from_numpy(phi).to(dev)
     x = torch.empty_like(x_ref)
     y = torch.empty_like(x_ref)
-    mesh.to_internal(x_ref.data_ptr(), x.data_ptr(), stream.cuda_stream)
-    f64 = lambda: J.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)  # noqa: E731
+    s = stream.cuda_stream
+    mesh.to_internal(x_ref.data_ptr(), x.data_ptr(), s)
     res = {"n": n, "E": E, "bytes64": 40 * E + 68 * n, "bytes32": 24 * E + 36 * n,
-           "warm64": _event_time(stream, f64, args.spmv_reps, warm=5),
-           "cold64": _cold_time(stream, f64, 20, flush)}
+           "warm64": J.bench_apply(x.data_ptr(), y.data_ptr(), reps=args.spmv_reps, stream=s),
+           "cold64": J.bench_apply(x.data_ptr(), y.data_ptr(), cold=True, reps=20, stream=s)}
     if fp32:
         x32 = x.float()
         y32 = torch.empty_like(x32)
-        f32 = lambda: J.apply_internal_fp32(x32.data_ptr(), y32.data_ptr(),  # noqa: E731
-                                            stream.cuda_stream)
-        f32()  # the FP32 block copy is made once per rebuild, here
-        res["warm32"] = _event_time(stream, f32, args.spmv_reps)
-        res["cold32"] = _cold_time(stream, f32, 20, flush)
+        res["warm32"] = J.bench_apply(x32.data_ptr(), y32.data_ptr(), fp32=True,
+                                      reps=args.spmv_reps, stream=s)
+        res["cold32"] = J.bench_apply(x32.data_ptr(), y32.data_ptr(), fp32=True, cold=True,
+                                      reps=20, stream=s)
+    torch.cuda.synchronize()
     return res, J, x, y
 
 
@@ -606,8 +595,7 @@ def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev, A):
     import torch
     import nosh_amd as nosh
     E, n = mesh.num_edges, mesh.num_vertices
-    flush = torch.ones(1 << 27, dtype=torch.float64, device=dev)
-    t, J, x, y = jphi_timings(args, mesh, model, params, psi0, phi, stream, dev, flush)
+    t, J, x, y = jphi_timings(args, mesh, model, params, psi0, phi, stream, dev)
     bytes_canon = t["bytes64"]
     gbs = lambda b, s: b / s / 1e9  # noqa: E731
     ceiling = stream_ceiling()
@@ -618,7 +606,11 @@ def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev, A):
     copy_gbs = 2 * cbuf.numel() * 8 / t_copy / 1e9
     del cbuf, cdst
     layout = mesh.layout()
-    bytes_layout = 20 * layout["nnz_pad"] + 4 * (layout["nslices"] + 1) + 56 * n
+    # Split layout: 20 B per upper slot (column, value), 8 B per lower slot
+    # (column, position of the mirrored value, which comes from L2), two
+    # slice pointers per slice, 56 B per vertex.
+    bytes_layout = (20 * layout["nnz_upper"] + 8 * layout["nnz_lower"] +
+                    8 * (layout["nslices"] + 1) + 56 * n)
     # K.phi and F(psi) (FP64, internal order), canonical bytes 40E + 44n and
     # 40E + 52n.
     others = {}
@@ -628,14 +620,14 @@ def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev, A):
     fk = lambda: keo.apply_internal(x.data_ptr(), y.data_ptr(), stream.cuda_stream)  # noqa
     ff = lambda: model.eval_f_internal(x.data_ptr(), params, y.data_ptr(),  # noqa: E731
                                        stream.cuda_stream)
+    # Warm only, HIP events around back-to-back launches (launch gaps
+    # included).
     for name, fn, b_ in (("keo_apply (K.phi FP64)", fk, 40 * E + 44 * n),
                          ("compute_f (F(psi) FP64)", ff, 40 * E + 52 * n)):
-        tw, tc = _event_time(stream, fn, args.spmv_reps), _cold_time(stream, fn, 10, flush)
-        others[name] = {"us_per_launch": round(tw * 1e6, 2), "cold_us_per_launch": round(tc * 1e6, 2),
-                        "bytes_per_launch": b_, "achieved": round(gbs(b_, tw), 1),
-                        "cold_achieved": round(gbs(b_, tc), 1),
-                        "frac": round(gbs(b_, tw) / HBM_PEAK_GBS, 4),
-                        "cold_frac": round(gbs(b_, tc) / HBM_PEAK_GBS, 4)}
+        tw = _event_time(stream, fn, args.spmv_reps)
+        others[name] = {"us_per_launch": round(tw * 1e6, 2), "bytes_per_launch": b_,
+                        "achieved": round(gbs(b_, tw), 1),
+                        "frac": round(gbs(b_, tw) / HBM_PEAK_GBS, 4)}
     others["k_block_apply<float> (J.phi FP32, blocks converted once per rebuild)"] = {
         "us_per_launch": round(t["warm32"] * 1e6, 2),
         "cold_us_per_launch": round(t["cold32"] * 1e6, 2), "bytes_per_launch": t["bytes32"],
@@ -655,7 +647,7 @@ def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev, A):
                          nosh.ScalarFieldConstant(m160, 1.0))
         n1 = c.shape[0]
         t1, J1, _, _ = jphi_timings(args, m160, md160, params, mg.random_state(n1, 0),
-                                    mg.random_state(n1, 1), stream, dev, flush, fp32=False)
+                                    mg.random_state(n1, 1), stream, dev, fp32=False)
         k160 = {"n_vertices": n1, "n_edges": t1["E"], "bytes_per_launch": t1["bytes64"],
                 "us_per_launch": round(t1["warm64"] * 1e6, 2),
                 "cold_us_per_launch": round(t1["cold64"] * 1e6, 2),
@@ -666,7 +658,6 @@ def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev, A):
         say(f"J.phi 160^3: warm {t1['warm64']*1e6:.1f} us, cold {t1['cold64']*1e6:.1f} us -> "
             f"{k160['achieved']:.0f} GB/s")
         del J1, md160, m160
-    del flush
     traffic, traffic_src = _profiled_traffic(args.ncell)
     ach_cold = gbs(bytes_canon, t["cold64"])
     ach_warm = gbs(bytes_canon, t["warm64"])
@@ -678,6 +669,9 @@ def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev, A):
             "frac": round(ach_cold / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": traffic_src,
             "kernel": "k_block_apply<double> (J.phi FP64, 96^3, cold caches)",
+            "timing": "average kernel duration from start/end events that the HIP runtime "
+                      "stamps on each dispatch (hipExtLaunchKernelGGL) on the launch stream; "
+                      "cold: a 1 GiB read before each launch",
             "us_per_launch": round(t["cold64"] * 1e6, 2), "bytes_per_launch": bytes_canon,
             "attainable": {"unit": "GB/s", "own_stream_kernels": ceiling,
                            "torch_copy": round(copy_gbs, 1)},
@@ -688,10 +682,13 @@ def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev, A):
                              "by the 256 MB Infinity Cache"},
             "kuhn160": k160,
             "layout_bytes_per_launch": bytes_layout,
-            "layout_note": "sliced ELLPACK (64-row slices) of the off-diagonal entries: "
-                           "16 B value + 4 B column per padded slot, 4 B per slice, "
-                           "24 B 2x2 block + 16 B x + 16 B y per vertex",
-            "nnz_pad": layout["nnz_pad"], "other_kernels": others}
+            "layout_note": "split sliced ELLPACK (64-row slices): upper slots (column above "
+                           "the row) 4 B column + 16 B value, lower slots 4 B column + 4 B "
+                           "position of the mirrored upper slot (conjugate value, gathered "
+                           "from L2), 8 B per slice, 24 B 2x2 block + 16 B x + 16 B y per "
+                           "vertex",
+            "nnz_upper": layout["nnz_upper"], "nnz_lower": layout["nnz_lower"],
+            "other_kernels": others}
 
 
 def surrogate_mfma(rows, stream, dev, nin=9, width=256, depth=4, reps=5):

@@ -90,6 +90,11 @@ int nosh_mesh_permutation(nosh_mesh_t mesh, int32_t* perm);
 /* Internal sparse layout: padded sliced-ELLPACK slots, slices of 64 rows,
  * widest row. */
 int nosh_mesh_layout(nosh_mesh_t mesh, int64_t* nnz_pad, int64_t* nslices, int* max_width);
+/* The split layout the operator kernels read: padded slots of the upper part
+ * (column above the row: 4 B column + the value) and of the lower part (4 B
+ * column + 4 B position of the mirrored upper slot, whose conjugate value it
+ * uses). */
+int nosh_mesh_layout_split(nosh_mesh_t mesh, int64_t* nnz_upper, int64_t* nnz_lower);
 /* Device vectors between reference order and the internal order (stream
  * may be NULL). Callers that keep state on the device in internal order use
  * the *_apply_internal entry points and skip the permutation per apply. */
@@ -158,6 +163,16 @@ int nosh_jac_apply_device(nosh_jac_t jac, const double* x, double* y, void* stre
  * the operator (values and blocks rounded to float) to FP32 vectors. */
 int nosh_jac_apply_internal(nosh_jac_t jac, const double* x, double* y, void* stream);
 int nosh_jac_apply_internal_fp32(nosh_jac_t jac, const float* x, float* y, void* stream);
+/* Measurement helper, no reference counterpart: the average duration of the
+ * internal-order J.phi kernel (FP64, or FP32 with fp32 != 0; x and y device
+ * vectors of that type) over `reps` launches on `stream`, each timed by start
+ * and end events that the runtime stamps on the kernel dispatch itself, so
+ * launch gaps do not count. cold != 0: before each launch, a read of a 1 GiB
+ * device buffer (allocated by the call) replaces the contents of the L2s and
+ * the 256 MB Infinity Cache with clean lines, so the kernel streams its
+ * operands from HBM. Two untimed launches come first. */
+int nosh_jac_bench_apply(nosh_jac_t jac, const void* x, void* y, int fp32, int cold, int reps,
+                         void* stream, double* seconds_per_launch);
 int nosh_jac_destroy(nosh_jac_t jac);
 
 /* ----------------------------------------------------- keo_regularized ---
@@ -476,14 +491,18 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int nparams,
  * (executables/nosh-cont/nosh-cont.cpp:226-290,344; step control
  * examples/conf.xml:68-74). Instance i of `batch` runs mu from its mu_0 to
  * mu_0 + span in `steps` equal steps; every step solves all branches with
- * the lockstep batched Newton (FP32 inner MINRES, FP64 outer) from a secant
- * predictor or, once trained, from the MLP surrogate guess (sopts != NULL),
- * with per-branch fallback to the secant guess and to psi_k (at most 20
- * lockstep Newton steps each), then, one branch at a time, to the
- * single-instance FP64 Newton-MINRES (inner solves to minres_rtol) from the
- * secant guess, from psi_k, and to the FP64 tangent continuation over the
- * step (up to 3 step halvings), and last to 2, 4, 8 batched sub-steps of the
- * branches that still fail. psi
+ * the lockstep batched Newton (FP32 inner MINRES, FP64 outer) from the
+ * predictor guess -- predictor 1: the tangent psi_k + h dpsi/dmu, with
+ * J(psi_k) dpsi/dmu = -dF/dmu solved for all branches in one lockstep FP32
+ * MINRES (LOCA's "Tangent", examples/conf.xml:38-41); 2: the secant through
+ * the last two states -- or, once trained, from the MLP surrogate guess
+ * (sopts != NULL). Branches that miss are retried in lockstep: from the
+ * predictor guess and from psi_k (at most 20 Newton steps each), then by
+ * pseudo-transient continuation ((J + sigma c t) steps) from both; then, one
+ * branch at a time, by the single-instance FP64 Newton-MINRES (inner solves
+ * to minres_rtol) from the predictor guess, from psi_k, and by the FP64
+ * tangent continuation over the step (up to 3 step halvings), and last by 2,
+ * 4, 8 batched sub-steps of the branches that still fail. psi
  * (host, the instances' reference-order vectors concatenated) holds the
  * start states and receives the last accepted states. record has
  * steps + 1 entries (entry 0: the solve at mu_0). Statuses as for
@@ -499,7 +518,8 @@ typedef struct {
   double seconds;
   double pred_error;    /* mean over branches of ||surrogate guess - psi|| where the
                            surrogate was evaluated, else equal to sec_error */
-  double sec_error;     /* mean over branches of ||secant guess - psi|| */
+  double sec_error;     /* mean over branches of ||predictor guess - psi|| (tangent or
+                           secant, as chosen) */
   int halvings;         /* step halvings needed by the branches no guess reached (FP64
                            fallback or batched sub-steps) */
   int retries;          /* branch retries by pseudo-transient continuation */
@@ -507,7 +527,7 @@ typedef struct {
 } nosh_batch_continuation_step;
 
 int nosh_batch_continuation_run(nosh_batch_t batch, double* psi, double span, int steps,
-                                const nosh_newton_options* opts,
+                                int predictor, const nosh_newton_options* opts,
                                 const nosh_surrogate_options* sopts,
                                 nosh_batch_continuation_step* record, int* steps_done,
                                 nosh_surrogate_stats* stats);

@@ -109,6 +109,7 @@ def _load():
         "nosh_mesh_control_volumes": (i32, [vp, vp]),
         "nosh_mesh_permutation": (i32, [vp, vp]),
         "nosh_mesh_layout": (i32, [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i32)]),
+        "nosh_mesh_layout_split": (i32, [vp, C.POINTER(i64), C.POINTER(i64)]),
         "nosh_mesh_to_internal": (i32, [vp, vp, vp, vp]),
         "nosh_mesh_to_reference": (i32, [vp, vp, vp, vp]),
         "nosh_vfield_explicit_values_create": (i32, [vp, vp, d, pvp]),
@@ -130,6 +131,7 @@ def _load():
         "nosh_jac_apply_device": (i32, [vp, vp, vp, vp]),
         "nosh_jac_apply_internal": (i32, [vp, vp, vp, vp]),
         "nosh_jac_apply_internal_fp32": (i32, [vp, vp, vp, vp]),
+        "nosh_jac_bench_apply": (i32, [vp, vp, vp, i32, i32, i32, vp, C.POINTER(d)]),
         "nosh_jac_destroy": (i32, [vp]),
         "nosh_prec_create": (i32, [vp, vp, vp, pvp]),
         "nosh_prec_set_chebyshev": (i32, [vp, i32, d, d]),
@@ -266,10 +268,14 @@ class Mesh(_Handle):
         return out
 
     def layout(self) -> dict:
-        """Internal sliced-ELLPACK layout sizes."""
+        """Internal sliced-ELLPACK layout sizes (nnz_pad: the full structure;
+        nnz_upper / nnz_lower: the split layout the operator kernels read)."""
         nnz, ns, w = C.c_int64(), C.c_int64(), C.c_int()
         _check(_lib.nosh_mesh_layout(self._h, C.byref(nnz), C.byref(ns), C.byref(w)))
-        return {"nnz_pad": nnz.value, "nslices": ns.value, "max_width": w.value}
+        nu, nl = C.c_int64(), C.c_int64()
+        _check(_lib.nosh_mesh_layout_split(self._h, C.byref(nu), C.byref(nl)))
+        return {"nnz_pad": nnz.value, "nslices": ns.value, "max_width": w.value,
+                "nnz_upper": nu.value, "nnz_lower": nl.value}
 
     def to_internal(self, x_ref_ptr: int, x_int_ptr: int, stream: int = 0):
         _check(_lib.nosh_mesh_to_internal(self._h, C.c_void_p(x_ref_ptr), C.c_void_p(x_int_ptr),
@@ -421,6 +427,17 @@ class JacobianOperator(_Operator):
     def apply_internal_fp32(self, x_ptr: int, y_ptr: int, stream: int = 0):
         _check(_lib.nosh_jac_apply_internal_fp32(self._h, C.c_void_p(x_ptr),
                                                  C.c_void_p(y_ptr), C.c_void_p(stream)))
+
+    def bench_apply(self, x_ptr: int, y_ptr: int, *, fp32: bool = False, cold: bool = False,
+                    reps: int = 20, stream: int = 0) -> float:
+        """Average kernel duration (seconds) of the internal-order apply,
+        stamped on the kernel dispatch itself; cold: a 1 GiB read before
+        each launch evicts the caches (measurement helper)."""
+        out = C.c_double()
+        _check(_lib.nosh_jac_bench_apply(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr),
+                                         int(fp32), int(cold), int(reps), C.c_void_p(stream),
+                                         C.byref(out)))
+        return out.value
 
 
 class KeoRegularized(_Operator):
@@ -788,18 +805,22 @@ class Batch(_Handle):
                 for i in range(self.ninst)]
 
     def continuation(self, psi0, span: float, steps: int, surrogate: Optional[dict] = None,
-                     options: Optional[NewtonOptions] = None, **kw):
+                     options: Optional[NewtonOptions] = None, predictor: str = "tangent",
+                     **kw):
         """Steps every instance (a branch) from its mu_0 to mu_0 + span in
-        ``steps`` equal steps, all branches solved together. With
-        ``surrogate`` (SurrogateOptions fields; ``{}`` for the defaults) the
-        MLP seeds Newton once trained. Returns (psi, step records, surrogate
-        stats or None)."""
+        ``steps`` equal steps, all branches solved together, each step from
+        the ``predictor`` guess ("tangent": psi_k + h dpsi/dmu from one
+        lockstep solve of all branches; "secant": through the last two
+        states). With ``surrogate`` (SurrogateOptions fields; ``{}`` for the
+        defaults) the MLP seeds Newton once trained. Returns (psi, step
+        records, surrogate stats or None)."""
+        pred = {"tangent": 1, "secant": 2}[predictor]
         lib = _load_mlp()
         if not getattr(lib, "_nosh_bcont", False):
             _load_surrogate(lib)
             lib.nosh_batch_continuation_run.restype = C.c_int
             lib.nosh_batch_continuation_run.argtypes = [
-                C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.POINTER(NewtonOptions),
+                C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(NewtonOptions),
                 C.POINTER(SurrogateOptions), C.POINTER(BatchContinuationStep), C.POINTER(C.c_int),
                 C.POINTER(SurrogateStats)]
             lib._nosh_bcont = True
@@ -814,7 +835,8 @@ class Batch(_Handle):
         rec = (BatchContinuationStep * (steps + 1))()
         done = C.c_int()
         st = SurrogateStats()
-        rc = lib.nosh_batch_continuation_run(self._h, _ptr(psi), span, steps, C.byref(o),
+        rc = lib.nosh_batch_continuation_run(self._h, _ptr(psi), span, steps, pred,
+                                             C.byref(o),
                                              C.byref(so) if so is not None else None, rec,
                                              C.byref(done), C.byref(st))
         if rc not in (_OK, _NOTCONV):

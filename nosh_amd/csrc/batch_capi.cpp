@@ -180,8 +180,16 @@ int newton_batch(nosh_batch_s* h, const nosh_newton_options& o, nosh_batch_insta
         if (B.active_h[i]) newton[i] = it;
       break;
     }
-    if (std::getenv("NOSH_PROGRESS") && (it < 8 || it % 5 == 0 || nactive < 4))
-      std::fprintf(stderr, "[batch newton] step %d active %d of %d\n", it, nactive, ni);
+    if (std::getenv("NOSH_PROGRESS") && (it < 8 || it % 5 == 0 || nactive < 4)) {
+      double fmin = 1e300, fmax = 0.0;
+      for (int i = 0; i < ni; ++i)
+        if (B.active_h[i]) {
+          fmin = std::min(fmin, nf[i]);
+          fmax = std::max(fmax, nf[i]);
+        }
+      std::fprintf(stderr, "[batch newton] step %d active %d of %d, ||F|| in [%.3e, %.3e]\n", it,
+                   nactive, ni, fmin, fmax);
+    }
     NOSH_HIP(hipMemcpyAsync(B.active.p, B.active_h.data(), sizeof(int) * ni,
                             hipMemcpyHostToDevice, s));
     NOSH_HIP(hipMemcpyAsync(B.rtol.p, rt.data(), sizeof(double) * ni, hipMemcpyHostToDevice, s));
@@ -208,6 +216,24 @@ int newton_batch(nosh_batch_s* h, const nosh_newton_options& o, nosh_batch_insta
       st[i].converged = (final_nf[i] <= o.tol) ? 1 : 0;
     }
   return lockstep;
+}
+
+int tangent_batch(nosh_batch_s* h, const nosh_newton_options& o, double scale, double rtol,
+                  hipStream_t s) {
+  Batch& B = h->b;
+  const int ni = B.args.ninst;
+  B.args.iters = o.cheb_iters;
+  B.args.ratio = o.cheb_ratio;
+  B.args.upper = 0.0;
+  B.args.z0 = std::max(o.cheb_iters - 1, 0) % 2;
+  B.drop_graph();
+  B.active_h.assign(ni, 1);
+  const std::vector<double> rt(ni, rtol);
+  NOSH_HIP(hipMemcpyAsync(B.active.p, B.active_h.data(), sizeof(int) * ni, hipMemcpyHostToDevice,
+                          s));
+  NOSH_HIP(hipMemcpyAsync(B.rtol.p, rt.data(), sizeof(double) * ni, hipMemcpyHostToDevice, s));
+  B.dfdmu_into_f(s, scale);
+  return B.minres(s, o.minres_max_iters);
 }
 
 }  // namespace nosh

@@ -78,6 +78,10 @@ constexpr int kMaxHalvings = 3;   // step splits (2, 4, 8 sub-steps) for failed 
 constexpr double kPtcShift0 = 0.1;
 constexpr double kPtcForcing = 1e-4;
 constexpr int kPtcCheb = 6;
+// Relative MINRES tolerance of the tangent solve: the guess then errs by
+// about kTangentRtol h |dpsi/dmu| from the solve against ~h^2 |psi''| / 2 of
+// the tangent itself.
+constexpr double kTangentRtol = 1e-3;
 // Damping of the Jacobi sweeps that smooth the surrogate's guess.
 constexpr double kSmoothOmega = 0.6;
 constexpr int kRed = 256;       // threads of the per-lane reduction blocks
@@ -267,7 +271,7 @@ struct GroupCont {
 extern "C" {
 
 int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int steps,
-                                const nosh_newton_options* opts,
+                                int predictor, const nosh_newton_options* opts,
                                 const nosh_surrogate_options* so,
                                 nosh_batch_continuation_step* record, int* steps_done,
                                 nosh_surrogate_stats* stats) {
@@ -275,6 +279,8 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
   nosh_surrogate_stats sst{};
   int rc = bcguard([&] {
     NOSH_CHECK(bt && psi && record && steps >= 1, kInvalid, "invalid batch continuation call");
+    NOSH_CHECK(predictor == 1 || predictor == 2, kInvalid,
+               "batch continuation: predictor must be 1 (tangent) or 2 (secant)");
     NOSH_CHECK(!so || (so->train_after >= 1 && so->batch >= 64 && so->newton_budget >= 1 &&
                        so->retrain_steps >= 0 && so->smooth_sweeps >= 0 &&
                        so->accept_ratio > 0.0),
@@ -482,15 +488,26 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
       nosh_batch_continuation_step& r = record[k];
       r = nosh_batch_continuation_step{};
       r.param = k * h;
+      if (predictor == 1) {
+        // Tangent guess psi_k + h dpsi/dmu, J(psi_k) dpsi/dmu = -dF/dmu at
+        // mu_k, all branches in one lockstep solve.
+        for (int i = 0; i < ni; ++i) mu[i] = bt->mu_h[i] + (k - 1) * h;
+        set_mu();
+        copy_state(&GroupCont::cur, false);
+        r.minres_iters += tangent_batch(bt, o, h, kTangentRtol, s);
+        copy_state(&GroupCont::sec, true);
+      }
       for (int i = 0; i < ni; ++i) mu[i] = bt->mu_h[i] + k * h;
       set_mu();
-      // Secant guess (psi_k on the first step).
-      const double ratio = hprev > 0.0 ? h / hprev : 0.0;
-      for (size_t q = 0; q < ng; ++q) {
-        GroupHost& G = B.ghost[q];
-        const int64_t m = G.mesh->n * G.dev.bp;
-        k_bc_secant<<<grid_for(m, 256), 256, 0, s>>>(m, ratio, gc[q].cur.p, gc[q].prev.p,
-                                                     gc[q].sec.p);
+      if (predictor == 2) {
+        // Secant guess (psi_k on the first step).
+        const double ratio = hprev > 0.0 ? h / hprev : 0.0;
+        for (size_t q = 0; q < ng; ++q) {
+          GroupHost& G = B.ghost[q];
+          const int64_t m = G.mesh->n * G.dev.bp;
+          k_bc_secant<<<grid_for(m, 256), 256, 0, s>>>(m, ratio, gc[q].cur.p, gc[q].prev.p,
+                                                       gc[q].sec.p);
+        }
       }
       // The surrogate is evaluated while it earns its cost: when it gives the
       // smaller residual for fewer than a quarter of the branches, its next

@@ -34,6 +34,13 @@ int newton_batch(nosh_batch_s* h, const nosh_newton_options& o, nosh_batch_insta
                  hipStream_t s, const int* skip = nullptr, const int* max_iters = nullptr,
                  int stop_below = 0, const double* ptc0 = nullptr);
 
+// Tangent predictor of every instance: rebuilds J at the current psi, solves
+// J z = -scale dF/dmu (FP32 MINRES to relative tolerance rtol, all instances
+// in lockstep) and leaves psi + z = psi + scale dpsi/dmu in the state.
+// Returns the lockstep MINRES iterations.
+int tangent_batch(nosh_batch_s* h, const nosh_newton_options& o, double scale, double rtol,
+                  hipStream_t s);
+
 // ||F(psi)|| of every instance at the batch's current state and mu, g.
 void batch_residuals(nosh_batch_s* h, std::vector<double>& norms, hipStream_t s);
 

@@ -467,6 +467,33 @@ __device__ double ph_compute_f(const BatchArgs& a, int64_t vb) {
   return s2;
 }
 
+// d/dmu of the off-diagonal row product: with K_ij = -alpha e^{-i mu a~},
+// dK_ij/dmu = -i a~ K_ij, i.e. (vr, vi) -> (a~ vi, -a~ vr); the diagonal sum
+// alpha, V and the nonlinear term do not depend on mu. Padding slots have
+// alpha 0.
+__device__ __forceinline__ double2 brow_dmu(const BGroup& G, int64_t row, int b, double mu,
+                                            const double2* __restrict__ x) {
+  const int32_t s = uniform(int32_t(row >> 6));
+  const int32_t beg = ld_uniform(G.slice_ptr + s), end = ld_uniform(G.slice_ptr + s + 1);
+  const int32_t w = (end - beg) / kSliceH;
+  double ar = 0.0, ai = 0.0;
+  int32_t j = beg + int32_t(row & 63);
+  for (int32_t k = 0; k < w; ++k, j += kSliceH) {
+    const double at = G.t64[j];
+    double vr, vi;
+    slot_value(G.a64[j], at, mu, vr, vi);
+    cmac(ar, ai, at * vi, -at * vr, x[int64_t(G.scol[j]) * G.bp + b]);
+  }
+  return make_double2(ar, ai);
+}
+
+__device__ void ph_dfdmu(const BatchArgs& a, int64_t vb, double scale) {
+  const Lane L = lane_ctx(a, vb);
+  if (!L.valid) return;
+  const double2 d = brow_dmu(L.G, L.row, L.b, a.mu[L.inst], L.G.psi);
+  L.G.f[L.idx] = make_double2(scale * d.x, scale * d.y);
+}
+
 // J and P blocks (FP32, including diag K) and the per-instance Chebyshev bound.
 __device__ void ph_rebuild(const BatchArgs& a, int64_t vb) {
   const Lane L = lane_ctx(a, vb);
@@ -595,19 +622,48 @@ __device__ __forceinline__ Ctx make_ctx(const BatchArgs& a, int64_t v0) {
 struct WUpd {
   float oe, de, dn;
   double phi;
+  double xphi;  // phi of the previous w update if its x update was deferred, else 0
 };
 
+// w_k = (v_k - oldeps w_{k-2} - delta w_{k-1}) / gamma, every operation
+// explicit (the one-lane and the L-lane kernels give the same bits).
+__device__ __forceinline__ float2 w_next(const WUpd& q, float2 vk, float2 w1, float2 w2) {
+#pragma clang fp contract(off)
+  return make_float2(
+      __builtin_fmaf(-q.de, w2.x, __builtin_fmaf(-q.oe, w1.x, vk.x)) * q.dn,
+      __builtin_fmaf(-q.de, w2.y, __builtin_fmaf(-q.oe, w1.y, vk.y)) * q.dn);
+}
+// x + phi_{k-1} w_{k-1} (a deferred update, when xphi != 0) + phi_k w_k.
+__device__ __forceinline__ double2 x_next(const WUpd& q, double2 xv, float2 w2, float2 w) {
+#pragma clang fp contract(off)
+  if (q.xphi != 0.0) {
+    xv.x = __builtin_fma(q.xphi, double(w2.x), xv.x);
+    xv.y = __builtin_fma(q.xphi, double(w2.y), xv.y);
+  }
+  xv.x = __builtin_fma(q.phi, double(w.x), xv.x);
+  xv.y = __builtin_fma(q.phi, double(w.y), xv.y);
+  return xv;
+}
+// r = u - (alfa / beta) r2.
+__device__ __forceinline__ float2 r_next(float2 uu, float2 r2v, float cab) {
+#pragma clang fp contract(off)
+  return make_float2(__builtin_fmaf(-cab, r2v.x, uu.x), __builtin_fmaf(-cab, r2v.y, uu.y));
+}
+
+// The update of w_k and of the FP64 accumulator x += phi_k w_k. The x update
+// of an odd k is deferred to k + 1 (defer), which then adds phi_k w_k and
+// phi_{k+1} w_{k+1} in that order with w_k = w_{(k+1)-1} already loaded: x
+// is read and written every other iteration (16 of the 64 bytes per row and
+// lane of the update, on average), with the bits of an update every
+// iteration.
 __device__ __forceinline__ void w_update(const BGroup& G, const WUpd& q, int k, int64_t idx,
-                                         float2 vk) {
+                                         float2 vk, bool defer) {
   float2* wk = wbuf(G, k);
   const float2 w1 = ldn(wbuf(G, k - 2) + idx), w2 = ldn(wbuf(G, k - 1) + idx);
-  const float2 w = make_float2((vk.x - q.oe * w1.x - q.de * w2.x) * q.dn,
-                               (vk.y - q.oe * w1.y - q.de * w2.y) * q.dn);
+  const float2 w = w_next(q, vk, w1, w2);
   stn(wk + idx, w);
-  double2 xv = ldn(G.x + idx);
-  xv.x += q.phi * double(w.x);
-  xv.y += q.phi * double(w.y);
-  stn(G.x + idx, xv);
+  if (defer) return;
+  stn(G.x + idx, x_next(q, ldn(G.x + idx), w2, w));
 }
 
 __device__ __forceinline__ WUpd wupd_of(const MinresScalars& S) {
@@ -616,6 +672,7 @@ __device__ __forceinline__ WUpd wupd_of(const MinresScalars& S) {
   q.de = float(S.p_delta);
   q.dn = float(S.p_denom);
   q.phi = S.p_phi;
+  q.xphi = S.x_phi;
   return q;
 }
 
@@ -646,11 +703,10 @@ __device__ double rows_cheb_first(const BatchArgs& a, const Ctx& c, int mode) {
     const int64_t row = c.row0 + int64_t(t) * c.tstride;
     if (row >= G.n) break;
     const int64_t idx = row * G.bp + c.b;
-    if (pending) w_update(G, q, it, idx, ldn(vprev + idx));
+    if (pending) w_update(G, q, it, idx, ldn(vprev + idx), (it & 1) != 0);
     float2 rv;
     if (mode == 2) {
-      const float2 uu = ldn(G.u + idx), r2v = ldn(r2 + idx);
-      rv = make_float2(uu.x - cab * r2v.x, uu.y - cab * r2v.y);
+      rv = r_next(ldn(G.u + idx), ldn(r2 + idx), cab);
       stn(r + idx, rv);
     } else {
       rv = ldn(r + idx);
@@ -1047,12 +1103,110 @@ __device__ void rows_minres_a_v(const BatchArgs& a, const VCtx<L>& c, double (&s
   }
 }
 
+// Chebyshev step 0 (and the previous iteration's w/x update) for L lanes of
+// a row per thread: the arithmetic of rows_cheb_first per (row, lane), with
+// 16-byte (w, v, u, r, z) and 32-byte (x) accesses.
+template <int L>
+__device__ void rows_cheb_first_v(const BatchArgs& a, const VCtx<L>& c, int mode,
+                                  double (&s)[L]) {
+  const BGroup& G = c.G;
+  bool act[L];
+  WUpd q[L];
+  float cab[L], th[L];
+  bool any = false, all = true, pending = false;
+  int it = 0;
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    s[l] = 0.0;
+    act[l] = false;
+    q[l] = WUpd{0.f, 0.f, 0.f, 0.0, 0.0};
+    cab[l] = 0.0f;
+    th[l] = 1.0f;
+    const int b = c.b0 + l;
+    if (b < G.B) {
+      const int inst = G.inst0 + b;
+      const MinresScalars& S = a.S[inst];
+      if (!S.done) {
+        act[l] = true;
+        any = true;
+        // The same for every active instance (lockstep).
+        it = mode == 2 ? S.it : 0;
+        pending = mode == 2 && S.pending;
+        q[l] = wupd_of(S);
+        cab[l] = mode == 2 ? float(S.alfa / S.beta) : 0.0f;
+        const double hi = hi_of(a, inst);
+        th[l] = float(0.5 * (hi + hi / a.ratio));
+      }
+    }
+    all = all && act[l];
+  }
+  if (!any) return;
+  const bool defer = (it & 1) != 0;
+  const float2* vprev = vbuf(G, it - 1);
+  const float2* w1p = wbuf(G, it - 2);
+  const float2* w2p = wbuf(G, it - 1);
+  float2* wk = wbuf(G, it);
+  float2* r = rbuf(G, it);
+  const float2* r2 = rbuf(G, it + 1);
+  float2* z0 = zbuf(G, a.z0);
+  const int64_t nb3 = G.n * G.bp;
+  for (int t = 0; t < a.fold; ++t) {
+    const int64_t row = c.row0 + int64_t(t) * c.tstride;
+    if (row >= G.n) break;
+    const int64_t idx0 = row * G.bp + c.b0;
+    if (pending) {
+      float2 vk[L], w1[L], w2[L], w[L];
+      load_lanes_nt<L>(vprev + idx0, vk);
+      load_lanes_nt<L>(w1p + idx0, w1);
+      load_lanes_nt<L>(w2p + idx0, w2);
+#pragma unroll
+      for (int l = 0; l < L; ++l) w[l] = w_next(q[l], vk[l], w1[l], w2[l]);
+      store_lanes_nt<L>(wk + idx0, w, all, act);
+      if (!defer) {
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+          if (act[l]) stn(G.x + idx0 + l, x_next(q[l], ldn(G.x + idx0 + l), w2[l], w[l]));
+      }
+    }
+    float2 rv[L], z[L];
+    if (mode == 2) {
+      float2 uu[L], r2v[L];
+      load_lanes_nt<L>(G.u + idx0, uu);
+      load_lanes_nt<L>(r2 + idx0, r2v);
+#pragma unroll
+      for (int l = 0; l < L; ++l) rv[l] = r_next(uu[l], r2v[l], cab[l]);
+      store_lanes_nt<L>(r + idx0, rv, all, act);
+    } else {
+      load_lanes_nt<L>(r + idx0, rv);
+    }
+    if (a.iters == 0) {
+#pragma unroll
+      for (int l = 0; l < L; ++l) z[l] = rv[l];
+    } else {
+      float b0[L], b1[L], b2[L];
+      load_floats_nt<L>(G.pblk + idx0, b0);
+      load_floats_nt<L>(G.pblk + nb3 + idx0, b1);
+      load_floats_nt<L>(G.pblk + 2 * nb3 + idx0, b2);
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        float o0, o1;
+        dinv32(b0[l], b1[l], b2[l], rv[l].x, rv[l].y, &o0, &o1);
+        z[l] = make_float2(o0 / th[l], o1 / th[l]);
+      }
+    }
+    store_lanes_nt<L>(z0 + idx0, z, all, act);
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+      if (act[l]) s[l] += dot2(rv[l], z[l]);
+  }
+}
+
 __device__ void ph_flush(const BatchArgs& a, int64_t vb) {
   const Lane L = lane_ctx(a, vb);
   if (!L.valid || !a.active[L.inst]) return;
   const BGroup& G = L.G;
   const MinresScalars& S = a.S[L.inst];
-  if (S.pending) w_update(G, wupd_of(S), S.it, L.idx, vbuf(G, S.it - 1)[L.idx]);
+  if (S.pending) w_update(G, wupd_of(S), S.it, L.idx, vbuf(G, S.it - 1)[L.idx], false);
   const double2 d = G.x[L.idx];
   double2 p = G.psi[L.idx];
   p.x += d.x;
@@ -1077,8 +1231,12 @@ __device__ void ph_finalize(const BatchArgs& a, int inst, int what, double* out)
       init_update(S, v[0]);
     else if (what == 2)
       S->alfa = v[0];  // the pending w/x update is applied by the next phase
-    else
+    else {
+      // The update the first Chebyshev phase of this iteration applied
+      // (index it) deferred its x part when it is odd.
+      S->x_phi = (S->pending && (S->it & 1)) ? S->p_phi : 0.0;
       givens_update(S, v[0]);
+    }
   }
   __syncthreads();
 }
@@ -1112,6 +1270,10 @@ __device__ __forceinline__ void blk_minres_a(const BatchArgs& a, int64_t rb) {
 // range of row tiles and its L2 holds the x entries those rows gather.
 __global__ void kb_compute_f(BatchArgs a) { blk_compute_f<false>(a, xcd_block(true)); }
 __global__ void kb_compute_f64v(BatchArgs a) { blk_compute_f<true>(a, xcd_block(true)); }
+__global__ void kb_dfdmu(BatchArgs a, double scale) {
+  const int64_t rb = xcd_block(true);
+  for (int r = 0; r < a.fold; ++r) ph_dfdmu(a, tile_vb(a, rb, r), scale);
+}
 __global__ void kb_rebuild(BatchArgs a) {
   const int64_t rb = xcd_block(true);
   for (int r = 0; r < a.fold; ++r) ph_rebuild(a, tile_vb(a, rb, r));
@@ -1134,6 +1296,14 @@ __global__ void kb_cheb_step_v(BatchArgs a, int mode, int j) {
   double s[L];
   rows_cheb_step_v<L>(a, c, mode, j, s);
   if (j == a.iters - 1) lane_partials_v<L>(a, v0, c, s);
+}
+template <int L>
+__global__ void kb_cheb_first_v(BatchArgs a, int mode) {
+  const int64_t v0 = xcd_block(true) * a.fold;
+  const VCtx<L> c = make_vctx<L>(a, v0);
+  double s[L];
+  rows_cheb_first_v<L>(a, c, mode, s);
+  if (a.iters <= 1) lane_partials_v<L>(a, v0, c, s);
 }
 template <int L>
 __global__ void kb_minres_a_v(BatchArgs a) {
@@ -1159,6 +1329,7 @@ __global__ void kb_reset(BatchArgs a, int maxit) {
   S->done = a.active[i] ? 0 : 1;
   S->it = 0;
   S->pending = 0;
+  S->x_phi = 0.0;
   S->rtol = a.rtol[i];
   S->maxit = maxit;
   S->relres = 1.0;
@@ -1293,7 +1464,10 @@ void Batch::launch_phases_iteration(hipStream_t s) {
   const unsigned nb = unsigned(args.total_blocks / args.fold), bt = unsigned(block_threads);
   launch_minres_a(s);
   kb_finalize<<<args.ninst, 256, 0, s>>>(args, 2, nullptr);
-  kb_cheb_first<<<nb, bt, 0, s>>>(args, 2);
+  if (lanes == kVecLanes)
+    kb_cheb_first_v<kVecLanes><<<nb, bt / kVecLanes, 0, s>>>(args, 2);
+  else
+    kb_cheb_first<<<nb, bt, 0, s>>>(args, 2);
   for (int j = 1; j < args.iters; ++j) launch_cheb_step(s, 2, j);
   kb_finalize<<<args.ninst, 256, 0, s>>>(args, 3, nullptr);
 }
@@ -1527,6 +1701,11 @@ void Batch::compute_norms(hipStream_t s, std::vector<double>& nrm) {
                           hipMemcpyDeviceToHost, s));
   NOSH_HIP(hipStreamSynchronize(s));
   for (double& v : nrm) v = std::sqrt(v);
+}
+
+void Batch::dfdmu_into_f(hipStream_t s, double scale) {
+  const unsigned nb = unsigned(args.total_blocks / args.fold), bt = unsigned(block_threads);
+  kb_dfdmu<<<nb, bt, 0, s>>>(args, scale);
 }
 
 void Batch::relayout(int threads, int fold_) {

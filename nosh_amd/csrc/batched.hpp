@@ -157,6 +157,9 @@ struct Batch {
   // applies psi += dx; returns the largest MINRES iteration count.
   int minres(hipStream_t s, int maxit);
   void compute_norms(hipStream_t s, std::vector<double>& nrm);
+  // f := scale * dF/dmu at psi and each instance's mu, for every instance
+  // (then minres() gives psi + scale * dpsi/dmu: the tangent predictor).
+  void dfdmu_into_f(hipStream_t s, double scale);
   // Average duration (us) of one Chebyshev step launch (kb_cheb_step,
   // j = min(2, iters - 1)) with every instance active, timed with events on s over reps launches
   // after rebuilding at the current psi. Leaves the MINRES state reset.

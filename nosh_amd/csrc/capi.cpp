@@ -304,6 +304,14 @@ int nosh_mesh_layout(nosh_mesh_t mesh, int64_t* nnz_pad, int64_t* nslices, int* 
   });
 }
 
+int nosh_mesh_layout_split(nosh_mesh_t mesh, int64_t* nnz_upper, int64_t* nnz_lower) {
+  return guard([&] {
+    NOSH_CHECK(mesh, kInvalid, "null mesh");
+    if (nnz_upper) *nnz_upper = mesh->m->nnz_u;
+    if (nnz_lower) *nnz_lower = mesh->m->nnz_l;
+  });
+}
+
 int nosh_mesh_to_internal(nosh_mesh_t mesh, const double* x_ref, double* x_int, void* stream) {
   return guard([&] {
     NOSH_CHECK(mesh && x_ref && x_int, kInvalid, "null argument");
@@ -544,21 +552,86 @@ int nosh_jac_apply_internal(nosh_jac_t jac, const double* x, double* y, void* st
   });
 }
 
+namespace {
+// The FP32 values of K and the FP32 copy of the blocks, made once per rebuild.
+void jac_prepare_fp32(nosh_jac_t jac, hipStream_t s) {
+  MeshCache& m = *jac->mesh->m;
+  if (!jac->keo->kv.valid32) launch_keo_to_fp32(m, jac->keo->kv, s);
+  if (!jac->blk32_valid || jac->blk32.n != size_t(3 * m.n)) {
+    if (jac->blk32.n != size_t(3 * m.n)) jac->blk32.alloc(3 * m.n);
+    convert_to<float>(jac->blk.p, jac->blk32.p, 3 * m.n, s);
+    jac->blk32_valid = true;
+  }
+}
+}  // namespace
+
 int nosh_jac_apply_internal_fp32(nosh_jac_t jac, const float* x, float* y, void* stream) {
   return guard([&] {
     NOSH_CHECK(jac && x && y, kInvalid, "null argument");
     NOSH_CHECK(jac->built, kNotBuilt, "jacobian_operator: rebuild has not been called");
+    hipStream_t s = stream_of(stream);
+    jac_prepare_fp32(jac, s);
+    launch_block_apply32(*jac->mesh->m, jac->keo->kv, jac->blk32.p, x, y, s);
+  });
+}
+
+int nosh_jac_bench_apply(nosh_jac_t jac, const void* x, void* y, int fp32, int cold, int reps,
+                         void* stream, double* seconds_per_launch) {
+  hipEvent_t flush_done = nullptr;
+  std::vector<hipEvent_t> ev;
+  int rc = guard([&] {
+    NOSH_CHECK(jac && x && y && seconds_per_launch && reps >= 1, kInvalid,
+               "invalid J.phi benchmark arguments");
+    NOSH_CHECK(jac->built, kNotBuilt, "jacobian_operator: rebuild has not been called");
     MeshCache& m = *jac->mesh->m;
     hipStream_t s = stream_of(stream);
-    if (!jac->keo->kv.valid32) launch_keo_to_fp32(m, jac->keo->kv, s);
-    // The FP32 copy of the blocks is made once per rebuild.
-    if (!jac->blk32_valid || jac->blk32.n != size_t(3 * m.n)) {
-      if (jac->blk32.n != size_t(3 * m.n)) jac->blk32.alloc(3 * m.n);
-      convert_to<float>(jac->blk.p, jac->blk32.p, 3 * m.n, s);
-      jac->blk32_valid = true;
+    if (fp32) jac_prepare_fp32(jac, s);
+    auto apply = [&](hipEvent_t t0, hipEvent_t t1) {
+      if (fp32)
+        launch_block_apply32(m, jac->keo->kv, jac->blk32.p, static_cast<const float*>(x),
+                             static_cast<float*>(y), s, t0, t1);
+      else
+        launch_block_apply(m, jac->keo->kv, jac->blk.p, static_cast<const double*>(x),
+                           static_cast<double*>(y), s, t0, t1);
+    };
+    // 1 GiB, four times the Infinity Cache, read before each cold launch.
+    const int64_t flush_bytes = int64_t(1) << 30;
+    DBuf<unsigned char> flush;
+    DBuf<float> part;
+    int dev = 0, cus = 0;
+    NOSH_HIP(hipGetDevice(&dev));
+    NOSH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const unsigned grid = unsigned(std::max(1, cus) * 8);
+    if (cold) {
+      flush.alloc(size_t(flush_bytes));
+      part.alloc(grid);
+      NOSH_HIP(hipMemsetAsync(flush.p, 0, size_t(flush_bytes), s));
     }
-    launch_block_apply32(m, jac->keo->kv, jac->blk32.p, x, y, s);
+    ev.resize(size_t(2 * reps), nullptr);
+    for (auto& e : ev) NOSH_HIP(hipEventCreate(&e));
+    for (int r = 0; r < 2; ++r) {
+      if (cold) launch_cache_flush(flush.p, flush_bytes, part.p, grid, s);
+      apply(nullptr, nullptr);
+    }
+    for (int r = 0; r < reps; ++r) {
+      if (cold) launch_cache_flush(flush.p, flush_bytes, part.p, grid, s);
+      apply(ev[size_t(2 * r)], ev[size_t(2 * r + 1)]);
+    }
+    NOSH_HIP(hipEventCreate(&flush_done));
+    NOSH_HIP(hipEventRecord(flush_done, s));
+    NOSH_HIP(hipEventSynchronize(flush_done));
+    double sum = 0.0;
+    for (int r = 0; r < reps; ++r) {
+      float ms = 0.f;
+      NOSH_HIP(hipEventElapsedTime(&ms, ev[size_t(2 * r)], ev[size_t(2 * r + 1)]));
+      sum += double(ms);
+    }
+    *seconds_per_launch = sum * 1e-3 / reps;
   });
+  for (hipEvent_t e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (flush_done) (void)hipEventDestroy(flush_done);
+  return rc;
 }
 
 int nosh_jac_destroy(nosh_jac_t jac) {

@@ -152,8 +152,7 @@ constexpr int kSB = 256;
 
 struct BlkOp {
   int64_t n;
-  const int32_t* slice_ptr;
-  const int32_t* scol;
+  SellView sv;          // split off-diagonal layout
   const double2* val;   // off-diagonal values of K (shared by J and P)
   const double* blk;    // 3 n per-vertex blocks (J or P), diagonal of K included
   const double* bound;  // Chebyshev bound from the rebuild
@@ -166,13 +165,13 @@ template <int NV>
 __device__ __forceinline__ void blk_row(const BlkOp& o, int64_t row, int nv, const ColsC& X,
                                         double2 (&acc)[NV]) {
   const int32_t sl = uniform(int32_t(row >> 6));
-  const int32_t b = uniform(o.slice_ptr[sl]);
-  const int32_t e = uniform(o.slice_ptr[sl + 1]);
   const int lane = int(row & 63);
 #pragma unroll
   for (int q = 0; q < NV; ++q) acc[q] = make_double2(0.0, 0.0);
-  for (int32_t j = b + lane; j < e; j += kSliceH) {
-    const int32_t c = __builtin_nontemporal_load(o.scol + j);
+  // Upper part: values in place.
+  for (int32_t j = uniform(o.sv.uptr[sl]) + lane, e = uniform(o.sv.uptr[sl + 1]); j < e;
+       j += kSliceH) {
+    const int32_t c = __builtin_nontemporal_load(o.sv.ucol + j);
     const double2 v = o.val[j];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -180,6 +179,20 @@ __device__ __forceinline__ void blk_row(const BlkOp& o, int64_t row, int nv, con
         const double2 xv = reinterpret_cast<const double2*>(X.p[q])[c];
         acc[q].x += v.x * xv.x - v.y * xv.y;
         acc[q].y += v.x * xv.y + v.y * xv.x;
+      }
+    }
+  }
+  // Lower part: the conjugate of the mirrored slot's value.
+  for (int32_t j = uniform(o.sv.lptr[sl]) + lane, e = uniform(o.sv.lptr[sl + 1]); j < e;
+       j += kSliceH) {
+    const int32_t c = __builtin_nontemporal_load(o.sv.lcol + j);
+    const double2 v = o.val[__builtin_nontemporal_load(o.sv.lpos + j)];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      if (q < nv) {
+        const double2 xv = reinterpret_cast<const double2*>(X.p[q])[c];
+        acc[q].x += v.x * xv.x + v.y * xv.y;
+        acc[q].y += v.x * xv.y - v.y * xv.x;
       }
     }
   }
@@ -553,7 +566,7 @@ int lobpcg_jacobian(nosh_nls_s* nls, const double* psi_int, const std::map<std::
   launch_rebuild_blocks(m, nls->keo->kv, nls->ct.p, V, g, psi_int, nls->jblk.p, nls->pblk.p,
                         nls->bound.p, s);
   int applies = 0;
-  BlkOp jop{n, m.slice_ptr.p, m.scol.p, nls->keo->kv.val.p, nls->jblk.p, nls->bound.p, 0.0,
+  BlkOp jop{n, sell_view(m), nls->keo->kv.val.p, nls->jblk.p, nls->bound.p, 0.0,
             o.cheb_ratio, o.cheb_iters};
   BlkOp pop = jop;
   pop.blk = nls->pblk.p;

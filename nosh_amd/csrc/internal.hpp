@@ -39,7 +39,31 @@ struct MeshCache {
   DBuf<int32_t> scol;       // nnz_pad
   DBuf<int32_t> sedge;      // nnz_pad
   DBuf<double> cv_int;      // n, internal order
+  // Split layout of the same graph, read by the single-instance operator
+  // kernels: each undirected edge appears in two rows, and K_ji = conj(K_ij).
+  // The slot in the row with the smaller internal index (the "upper" part:
+  // column above the row) holds the value; the slot in the other row (the
+  // "lower" part) holds the position of that value. Each part is a sliced
+  // ELLPACK with its own slice widths, so a wave reads either contiguous
+  // values or contiguous positions. Padding: column = own row; upper edge
+  // code -1 (value 0); lower position nnz_u (the zero entry after the
+  // upper values).
+  int64_t nnz_u = 0, nnz_l = 0;
+  DBuf<int32_t> uptr, ucol, uedge;  // nslices + 1, nnz_u, nnz_u
+  DBuf<int32_t> lptr, lcol, lpos;   // nslices + 1, nnz_l, nnz_l
 };
+
+// Device view of the split layout (kernel argument).
+struct SellView {
+  const int32_t* uptr;
+  const int32_t* ucol;
+  const int32_t* lptr;
+  const int32_t* lcol;
+  const int32_t* lpos;
+};
+inline SellView sell_view(const MeshCache& m) {
+  return SellView{m.uptr.p, m.ucol.p, m.lptr.p, m.lcol.p, m.lpos.p};
+}
 
 std::unique_ptr<MeshCache> build_mesh_cache(const double* coords, int64_t n, const int32_t* cells,
                                             int64_t T, int nv, bool reorder, hipStream_t s);
@@ -94,11 +118,11 @@ struct Vec2<float> {
   using type = float2;
 };
 
-// Assembled off-diagonal values of K (and of dK/dmu) in the SELL layout,
+// Assembled off-diagonal values of K (and of dK/dmu) in the split layout,
 // with the row sums of alpha (the diagonal of K) and of |alpha|.
 struct KeoValues {
-  DBuf<double2> val;   // nnz_pad
-  DBuf<double2> dval;  // nnz_pad (dK/dp), allocated on demand
+  DBuf<double2> val;   // nnz_u + 1 (upper slots of the split layout, then 0)
+  DBuf<double2> dval;  // nnz_u + 1 (dK/dp), allocated on demand
   DBuf<float2> val32;  // FP32 copy for the mixed-precision inner solver
   DBuf<double> diag;   // n: sum of alpha over the row
   DBuf<double> rowabs; // n: sum of |alpha| over the row
@@ -135,9 +159,11 @@ void launch_compute_f(const MeshCache& m, const KeoValues& kv, const double* ct,
 void launch_rebuild_blocks(const MeshCache& m, const KeoValues& kv, const double* ct,
                            const double* V, double g, const double* psi, double* jblk,
                            double* pblk, double* bound, hipStream_t s);
-// y = Koff x + B x with 3-component blocks B (J or P).
+// y = Koff x + B x with 3-component blocks B (J or P). With t0/t1 set, the
+// runtime stamps the kernel's own start and end into those events.
 void launch_block_apply(const MeshCache& m, const KeoValues& kv, const double* blk,
-                        const double* x, double* y, hipStream_t s);
+                        const double* x, double* y, hipStream_t s, hipEvent_t t0 = nullptr,
+                        hipEvent_t t1 = nullptr);
 
 // Elementwise helpers.
 void launch_fill(double* a, int64_t n, double v, hipStream_t s);
@@ -145,7 +171,13 @@ void launch_mul(const double* a, const double* b, double* c, int64_t n, hipStrea
 void launch_scale(const double* a, double v, double* c, int64_t n, hipStream_t s);
 // y = K x (internal order), FP32 values and vectors.
 void launch_block_apply32(const MeshCache& m, const KeoValues& kv, const float* blk,
-                          const float* x, float* y, hipStream_t s);
+                          const float* x, float* y, hipStream_t s, hipEvent_t t0 = nullptr,
+                          hipEvent_t t1 = nullptr);
+// One pass of ordinary (cache-allocating) 16-byte loads over `bytes` bytes of
+// `buf`, one float per workgroup into out[grid]: replaces the contents of the
+// L2s and the Infinity Cache with clean lines.
+void launch_cache_flush(const void* buf, int64_t bytes, float* out, unsigned grid,
+                        hipStream_t s);
 
 }  // namespace nosh
 

@@ -35,25 +35,43 @@ __device__ __forceinline__ int32_t ld_uniform(const int32_t* p) {
 #endif
 }
 
-// Off-diagonal product of one internal row: sum_k val_k * x[col_k]
-// (complex multiply), in slot order.
+// Off-diagonal product of one internal row over the split layout: the upper
+// part reads its values in place, sum val_k x[col_k]; the lower part reads
+// each value at its position in the upper part (a gather that the
+// neighbouring rows' waves have just brought into L2) and uses its
+// conjugate, sum conj(val_pos_k) x[col_k]. Upper slots first, each part in
+// slot order.
 template <class T>
-__device__ __forceinline__ V2<T> sell_row(int64_t row, const int32_t* __restrict__ slice_ptr,
-                                          const int32_t* __restrict__ scol,
+__device__ __forceinline__ V2<T> sell_row(int64_t row, const SellView& S,
                                           const V2<T>* __restrict__ val,
                                           const V2<T>* __restrict__ x) {
   const int32_t s = uniform(int32_t(row >> 6));
-  const int32_t b = uniform(slice_ptr[s]);
-  const int32_t e = uniform(slice_ptr[s + 1]);
   const int lane = int(row & 63);
   T ar = T(0), ai = T(0);
+  {
+    const int32_t b = uniform(S.uptr[s]);
+    const int32_t e = uniform(S.uptr[s + 1]);
 #pragma unroll 4
-  for (int32_t j = b + lane; j < e; j += kSliceH) {
-    const int32_t c = __builtin_nontemporal_load(scol + j);
-    const V2<T> v = val[j];
-    const V2<T> xv = x[c];
-    ar += v.x * xv.x - v.y * xv.y;
-    ai += v.x * xv.y + v.y * xv.x;
+    for (int32_t j = b + lane; j < e; j += kSliceH) {
+      const int32_t c = __builtin_nontemporal_load(S.ucol + j);
+      const V2<T> v = val[j];
+      const V2<T> xv = x[c];
+      ar += v.x * xv.x - v.y * xv.y;
+      ai += v.x * xv.y + v.y * xv.x;
+    }
+  }
+  {
+    const int32_t b = uniform(S.lptr[s]);
+    const int32_t e = uniform(S.lptr[s + 1]);
+#pragma unroll 4
+    for (int32_t j = b + lane; j < e; j += kSliceH) {
+      const int32_t c = __builtin_nontemporal_load(S.lcol + j);
+      const int32_t p = __builtin_nontemporal_load(S.lpos + j);
+      const V2<T> v = val[p];
+      const V2<T> xv = x[c];
+      ar += v.x * xv.x + v.y * xv.y;
+      ai += v.x * xv.y - v.y * xv.x;
+    }
   }
   V2<T> r;
   r.x = ar;

@@ -422,6 +422,96 @@ __global__ void k_sell_fill(const uint32_t* __restrict__ rows, const int32_t* __
   sedge[dst] = vals[i];
 }
 
+// Split layout (see MeshCache): per row, the number of columns below the row
+// (CSR columns are sorted, so these come first) and above it.
+__global__ void k_count_split(const int32_t* __restrict__ rowstart, const int32_t* __restrict__ cols,
+                              int64_t n, int32_t* __restrict__ nlow, int32_t* __restrict__ nup) {
+  const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int32_t k = rowstart[r];
+  const int32_t e = rowstart[r + 1];
+  while (k < e && cols[k] < r) ++k;
+  nlow[r] = k - rowstart[r];
+  nup[r] = e - k;
+}
+
+__global__ void k_split_width(const int32_t* __restrict__ nlow, const int32_t* __restrict__ nup,
+                              int64_t n, int64_t nslices, int32_t* __restrict__ wl64,
+                              int32_t* __restrict__ wu64) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= nslices) return;
+  int wl = 0, wu = 0;
+  for (int64_t r = s * kSliceH; r < std::min<int64_t>(n, (s + 1) * kSliceH); ++r) {
+    wl = max(wl, nlow[r]);
+    wu = max(wu, nup[r]);
+  }
+  wl64[s] = wl * kSliceH;
+  wu64[s] = wu * kSliceH;
+}
+
+// Padding of both parts: column = own row; upper edge code -1 (value 0),
+// lower position = zpos (the zero entry after the upper values).
+__global__ void k_split_pad(const int32_t* __restrict__ uptr, const int32_t* __restrict__ lptr,
+                            int64_t nslices, int64_t n, int32_t zpos, int32_t* __restrict__ ucol,
+                            int32_t* __restrict__ uedge, int32_t* __restrict__ lcol,
+                            int32_t* __restrict__ lpos) {
+  const int64_t s = blockIdx.x;
+  if (s >= nslices) return;
+  for (int64_t j = uptr[s] + threadIdx.x; j < uptr[s + 1]; j += blockDim.x) {
+    const int64_t r = s * kSliceH + ((j - uptr[s]) & (kSliceH - 1));
+    ucol[j] = int32_t(r < n ? r : 0);
+    uedge[j] = -1;
+  }
+  for (int64_t j = lptr[s] + threadIdx.x; j < lptr[s + 1]; j += blockDim.x) {
+    const int64_t r = s * kSliceH + ((j - lptr[s]) & (kSliceH - 1));
+    lcol[j] = int32_t(r < n ? r : 0);
+    lpos[j] = zpos;
+  }
+}
+
+// Every CSR entry to its part: upper entries (column above the row) keep the
+// edge code and record where the edge's value will live; lower entries
+// keep the code for now (k_split_lpos turns it into that position).
+__global__ void k_split_fill(const uint32_t* __restrict__ rows, const int32_t* __restrict__ cols,
+                             const int32_t* __restrict__ codes,
+                             const int32_t* __restrict__ rowstart,
+                             const int32_t* __restrict__ nlow, const int32_t* __restrict__ uptr,
+                             const int32_t* __restrict__ lptr, int64_t N,
+                             int32_t* __restrict__ ucol, int32_t* __restrict__ uedge,
+                             int32_t* __restrict__ lcol, int32_t* __restrict__ lcode,
+                             int32_t* __restrict__ upos_of_edge) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t r = rows[i];
+  const int64_t k = i - rowstart[r];
+  const int64_t s = r / kSliceH, lane = r % kSliceH;
+  const int32_t c = cols[i], code = codes[i];
+  if (c > r) {
+    const int64_t dst = uptr[s] + (k - nlow[r]) * kSliceH + lane;
+    ucol[dst] = c;
+    uedge[dst] = code;
+    upos_of_edge[code >> 1] = int32_t(dst);
+  } else {
+    const int64_t dst = lptr[s] + k * kSliceH + lane;
+    lcol[dst] = c;
+    lcode[dst] = code;
+  }
+}
+
+__global__ void k_split_lpos(const int32_t* __restrict__ lcode,
+                             const int32_t* __restrict__ upos_of_edge, int64_t nnz_l,
+                             int32_t* __restrict__ lpos) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= nnz_l) return;
+  const int32_t code = lcode[j];
+  if (code >= 0) lpos[j] = upos_of_edge[code >> 1];
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ a, int64_t n, int32_t v) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
 __global__ void k_gather_f64(const int32_t* __restrict__ perm, const double* __restrict__ src,
                              int64_t n, double* __restrict__ dst) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -702,6 +792,33 @@ std::unique_ptr<MeshCache> build_mesh_cache(const double* coords, int64_t n, con
     k_sell_fill<<<grid_for(ND, kBlock), kBlock, 0, s>>>(rows.p, cols.p, sdv.p, rstart.p,
                                                         m->slice_ptr.p, ND, m->scol.p,
                                                         m->sedge.p);
+    // 8. The split layout of the same graph for the operator kernels.
+    DBuf<int32_t> nlow(n), nup(n), wl64(ns + 1), wu64(ns + 1);
+    wl64.zero(s);
+    wu64.zero(s);
+    k_count_split<<<grid_for(n, kBlock), kBlock, 0, s>>>(rstart.p, cols.p, n, nlow.p, nup.p);
+    k_split_width<<<grid_for(ns, kBlock), kBlock, 0, s>>>(nlow.p, nup.p, n, ns, wl64.p, wu64.p);
+    m->uptr.alloc(ns + 1);
+    m->lptr.alloc(ns + 1);
+    exclusive_sum(wu64.p, m->uptr.p, ns + 1, s);
+    exclusive_sum(wl64.p, m->lptr.p, ns + 1, s);
+    int32_t tu = 0, tl = 0;
+    NOSH_HIP(hipMemcpy(&tu, m->uptr.p + ns, 4, hipMemcpyDeviceToHost));
+    NOSH_HIP(hipMemcpy(&tl, m->lptr.p + ns, 4, hipMemcpyDeviceToHost));
+    m->nnz_u = tu;
+    m->nnz_l = tl;
+    m->ucol.alloc(std::max<int64_t>(tu, 1));
+    m->uedge.alloc(std::max<int64_t>(tu, 1));
+    m->lcol.alloc(std::max<int64_t>(tl, 1));
+    m->lpos.alloc(std::max<int64_t>(tl, 1));
+    DBuf<int32_t> lcode(std::max<int64_t>(tl, 1)), upos(std::max<int64_t>(E, 1));
+    k_fill_i32<<<grid_for(tl, kBlock), kBlock, 0, s>>>(lcode.p, tl, -1);
+    k_split_pad<<<unsigned(ns), kBlock, 0, s>>>(m->uptr.p, m->lptr.p, ns, n, tu, m->ucol.p,
+                                                m->uedge.p, m->lcol.p, m->lpos.p);
+    k_split_fill<<<grid_for(ND, kBlock), kBlock, 0, s>>>(
+        rows.p, cols.p, sdv.p, rstart.p, nlow.p, m->uptr.p, m->lptr.p, ND, m->ucol.p,
+        m->uedge.p, m->lcol.p, lcode.p, upos.p);
+    k_split_lpos<<<grid_for(tl, kBlock), kBlock, 0, s>>>(lcode.p, upos.p, tl, m->lpos.p);
     NOSH_HIP(hipStreamSynchronize(s));
   }
   return m;

@@ -13,6 +13,8 @@
 //   jacobian_operator          src/jacobian_operator.cpp:38-200
 //   keo_regularized::rebuild   src/keo_regularized.cpp:181-264
 
+#include <hip/hip_ext.h>
+
 #include "batched.hpp"
 #include "kernels.hpp"
 
@@ -49,9 +51,11 @@ __global__ void k_alpha(const double* __restrict__ covol, const double* __restri
 __global__ void k_keo_fill(const int32_t* __restrict__ sedge, const double* __restrict__ alpha,
                            const double* __restrict__ atilde, double mu, int64_t nnz,
                            double2* __restrict__ val, double2* __restrict__ dval, int dmode) {
+  // Slots 0 .. nnz-1 of the upper part, then the zero entry at nnz that the
+  // lower part's padding points to.
   const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= nnz) return;
-  const int32_t code = sedge[j];
+  if (j > nnz) return;
+  const int32_t code = j < nnz ? sedge[j] : -1;
   if (code < 0) {
     val[j] = make_double2(0.0, 0.0);
     if (dval) dval[j] = make_double2(0.0, 0.0);
@@ -118,12 +122,12 @@ __global__ void k_gather(int64_t n, const int32_t* __restrict__ perm,
 }
 
 __global__ __launch_bounds__(kBlock) void k_keo_apply(
-    int64_t n, const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ scol,
+    int64_t n, SellView S,
     const double2* __restrict__ val, const double* __restrict__ diag,
     const double2* __restrict__ x, double2* __restrict__ y) {
   const int64_t row = xcd_block(true) * kBlock + threadIdx.x;
   if (row >= n) return;
-  const double2 acc = sell_row<double>(row, slice_ptr, scol, val, x);
+  const double2 acc = sell_row<double>(row, S, val, x);
   const double2 xi = x[row];
   const double d = diag[row];
   y[row] = make_double2(acc.x + d * xi.x, acc.y + d * xi.y);
@@ -131,14 +135,14 @@ __global__ __launch_bounds__(kBlock) void k_keo_apply(
 
 // f = K x + ct (V + g |x|^2) x  and  sum |f|^2 (when nrm2 != nullptr).
 __global__ __launch_bounds__(kBlock) void k_compute_f(
-    int64_t n, const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ scol,
+    int64_t n, SellView S,
     const double2* __restrict__ val, const double* __restrict__ diag,
     const double* __restrict__ ct, const double* __restrict__ V, double g,
     const double2* __restrict__ x, double2* __restrict__ f, double* __restrict__ partials) {
   const int64_t row = xcd_block(true) * kBlock + threadIdx.x;
   double s2[1] = {0.0};
   if (row < n) {
-    const double2 acc = sell_row<double>(row, slice_ptr, scol, val, x);
+    const double2 acc = sell_row<double>(row, S, val, x);
     const double2 xi = x[row];
     const double a = ct[row] * (V[row] + g * (xi.x * xi.x + xi.y * xi.y));
     const double d = diag[row];
@@ -158,12 +162,12 @@ __global__ void k_finalize_store(const double* __restrict__ partials, int nb,
 
 // dF/dp column: dK x + ct |x|^2 x (parameter "g") or ct dV/dp x.
 __global__ __launch_bounds__(kBlock) void k_dfdp(
-    int64_t n, const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ scol,
+    int64_t n, SellView S,
     const double2* __restrict__ dval, const double* __restrict__ ct, int is_g, double dvdp,
     const double* __restrict__ dvdp_vec, const double2* __restrict__ x, double2* __restrict__ y) {
   const int64_t row = xcd_block(true) * kBlock + threadIdx.x;
   if (row >= n) return;
-  const double2 acc = sell_row<double>(row, slice_ptr, scol, dval, x);
+  const double2 acc = sell_row<double>(row, S, dval, x);
   const double2 xi = x[row];
   const double dv = dvdp_vec ? dvdp + dvdp_vec[row] : dvdp;
   const double a = is_g ? ct[row] * (xi.x * xi.x + xi.y * xi.y) : ct[row] * dv;
@@ -226,12 +230,12 @@ __global__ __launch_bounds__(kBlock) void k_rebuild_blocks(
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_block_apply(
-    int64_t n, const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ scol,
+    int64_t n, SellView S,
     const V2<T>* __restrict__ val, const T* __restrict__ blk, const V2<T>* __restrict__ x,
     V2<T>* __restrict__ y) {
   const int64_t row = xcd_block(true) * kBlock + threadIdx.x;
   if (row >= n) return;
-  const V2<T> acc = sell_row<T>(row, slice_ptr, scol, val, x);
+  const V2<T> acc = sell_row<T>(row, S, val, x);
   const V2<T> xi = x[row];
   const T b0 = blk[row], b1 = blk[n + row], b2 = blk[2 * n + row];
   V2<T> r;
@@ -253,12 +257,13 @@ void launch_alpha(const MeshCache& m, const double* thickness_ref, double* alpha
 
 void launch_keo_fill(const MeshCache& m, const double* alpha, const double* atilde, double mu,
                      bool dkeo, bool deriv_is_mu, KeoValues& kv, hipStream_t s) {
-  if (kv.val.n != size_t(m.nnz_pad)) kv.val.alloc(m.nnz_pad);
-  if (dkeo && kv.dval.n != size_t(m.nnz_pad)) kv.dval.alloc(m.nnz_pad);
+  const size_t nv = size_t(m.nnz_u) + 1;
+  if (kv.val.n != nv) kv.val.alloc(nv);
+  if (dkeo && kv.dval.n != nv) kv.dval.alloc(nv);
   if (kv.diag.n != size_t(m.n)) kv.diag.alloc(m.n);
   if (kv.rowabs.n != size_t(m.n)) kv.rowabs.alloc(m.n);
-  k_keo_fill<<<grid_for(m.nnz_pad, kBlock), kBlock, 0, s>>>(
-      m.sedge.p, alpha, atilde, mu, m.nnz_pad, kv.val.p, dkeo ? kv.dval.p : nullptr,
+  k_keo_fill<<<grid_for(int64_t(nv), kBlock), kBlock, 0, s>>>(
+      m.uedge.p, alpha, atilde, mu, m.nnz_u, kv.val.p, dkeo ? kv.dval.p : nullptr,
       deriv_is_mu ? 1 : 0);
   k_keo_diag<<<grid_for(m.n, kBlock), kBlock, 0, s>>>(m.slice_ptr.p, m.sedge.p, alpha, m.n,
                                                       kv.diag.p, kv.rowabs.p);
@@ -269,8 +274,9 @@ void launch_keo_fill(const MeshCache& m, const double* alpha, const double* atil
 }
 
 void launch_keo_to_fp32(const MeshCache& m, KeoValues& kv, hipStream_t s) {
-  if (kv.val32.n != size_t(m.nnz_pad)) kv.val32.alloc(m.nnz_pad);
-  k_to_fp32<<<grid_for(m.nnz_pad, kBlock), kBlock, 0, s>>>(kv.val.p, m.nnz_pad, kv.val32.p);
+  const int64_t nv = m.nnz_u + 1;
+  if (kv.val32.n != size_t(nv)) kv.val32.alloc(nv);
+  k_to_fp32<<<grid_for(nv, kBlock), kBlock, 0, s>>>(kv.val.p, nv, kv.val32.p);
   kv.valid32 = true;
 }
 
@@ -294,14 +300,14 @@ void launch_gather_ref(int64_t n, const int32_t* perm, const double* v_ref, doub
 void launch_keo_apply(const MeshCache& m, const KeoValues& kv, const double* x, double* y,
                       hipStream_t s) {
   k_keo_apply<<<grid_for(m.n, kBlock), kBlock, 0, s>>>(
-      m.n, m.slice_ptr.p, m.scol.p, kv.val.p, kv.diag.p, reinterpret_cast<const double2*>(x),
+      m.n, sell_view(m), kv.val.p, kv.diag.p, reinterpret_cast<const double2*>(x),
       reinterpret_cast<double2*>(y));
 }
 
 void launch_dfdp(const MeshCache& m, const KeoValues& kv, const double* ct, const double* x,
                  bool is_g, double dvdp, const double* dvdp_vec, double* y, hipStream_t s) {
   k_dfdp<<<grid_for(m.n, kBlock), kBlock, 0, s>>>(
-      m.n, m.slice_ptr.p, m.scol.p, kv.dval.p, ct, is_g ? 1 : 0, dvdp, dvdp_vec,
+      m.n, sell_view(m), kv.dval.p, ct, is_g ? 1 : 0, dvdp, dvdp_vec,
       reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y));
 }
 
@@ -322,7 +328,7 @@ void launch_compute_f(const MeshCache& m, const KeoValues& kv, const double* ct,
                       hipStream_t s) {
   const unsigned nb = grid_for(m.n, kBlock);
   double* partials = nrm2 ? fscratch(nb).p : nullptr;
-  k_compute_f<<<nb, kBlock, 0, s>>>(m.n, m.slice_ptr.p, m.scol.p, kv.val.p, kv.diag.p, ct, V, g,
+  k_compute_f<<<nb, kBlock, 0, s>>>(m.n, sell_view(m), kv.val.p, kv.diag.p, ct, V, g,
                                     reinterpret_cast<const double2*>(x),
                                     reinterpret_cast<double2*>(f), partials);
   if (nrm2) k_finalize_store<<<1, kFinalizeThreads, 0, s>>>(partials, int(nb), nrm2);
@@ -338,10 +344,16 @@ void launch_rebuild_blocks(const MeshCache& m, const KeoValues& kv, const double
 }
 
 void launch_block_apply(const MeshCache& m, const KeoValues& kv, const double* blk,
-                        const double* x, double* y, hipStream_t s) {
-  k_block_apply<double><<<grid_for(m.n, kBlock), kBlock, 0, s>>>(
-      m.n, m.slice_ptr.p, m.scol.p, kv.val.p, blk, reinterpret_cast<const double2*>(x),
-      reinterpret_cast<double2*>(y));
+                        const double* x, double* y, hipStream_t s, hipEvent_t t0,
+                        hipEvent_t t1) {
+  const auto xv = reinterpret_cast<const double2*>(x);
+  const auto yv = reinterpret_cast<double2*>(y);
+  if (t0 || t1)
+    hipExtLaunchKernelGGL(k_block_apply<double>, dim3(grid_for(m.n, kBlock)), dim3(kBlock), 0, s,
+                          t0, t1, 0, m.n, sell_view(m), static_cast<const double2*>(kv.val.p), blk, xv, yv);
+  else
+    k_block_apply<double><<<grid_for(m.n, kBlock), kBlock, 0, s>>>(m.n, sell_view(m), kv.val.p,
+                                                                   blk, xv, yv);
 }
 
 namespace {
@@ -374,10 +386,16 @@ void launch_mul(const double* a, const double* b, double* c, int64_t n, hipStrea
 }
 
 void launch_block_apply32(const MeshCache& m, const KeoValues& kv, const float* blk,
-                          const float* x, float* y, hipStream_t s) {
-  k_block_apply<float><<<grid_for(m.n, kBlock), kBlock, 0, s>>>(
-      m.n, m.slice_ptr.p, m.scol.p, kv.val32.p, blk, reinterpret_cast<const float2*>(x),
-      reinterpret_cast<float2*>(y));
+                          const float* x, float* y, hipStream_t s, hipEvent_t t0,
+                          hipEvent_t t1) {
+  const auto xv = reinterpret_cast<const float2*>(x);
+  const auto yv = reinterpret_cast<float2*>(y);
+  if (t0 || t1)
+    hipExtLaunchKernelGGL(k_block_apply<float>, dim3(grid_for(m.n, kBlock)), dim3(kBlock), 0, s,
+                          t0, t1, 0, m.n, sell_view(m), kv.val32.p, blk, xv, yv);
+  else
+    k_block_apply<float><<<grid_for(m.n, kBlock), kBlock, 0, s>>>(m.n, sell_view(m),
+                                                                  kv.val32.p, blk, xv, yv);
 }
 
 }  // namespace nosh

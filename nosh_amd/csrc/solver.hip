@@ -35,8 +35,7 @@ constexpr int kBlock = 256;
 template <class T>
 struct ChebArgs {
   int64_t n;
-  const int32_t* slice_ptr;
-  const int32_t* scol;
+  SellView sv;  // split off-diagonal layout
   const V2<T>* pval;   // P off-diagonal values
   const T* pblk;       // P diagonal blocks (3 n)
   const double* bound; // max ratio from the rebuild
@@ -104,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(ChebArgs<T> a, int mode, i
   double s2[1] = {0.0};
   if (row < a.n) {
     const Cheb c = cheb_coef(cheb_hi(a.bound, a.upper), a.ratio, j);
-    const V2<T> acc = sell_row<T>(row, a.slice_ptr, a.scol, a.pval, zin);
+    const V2<T> acc = sell_row<T>(row, a.sv, a.pval, zin);
     const V2<T> zi = zin[row];
     const T b0 = a.pblk[row], b1 = a.pblk[a.n + row], b2 = a.pblk[2 * a.n + row];
     const V2<T> rv = r[row];
@@ -154,8 +153,7 @@ __global__ void k_fin_alfa(MinresScalars* S, const double* __restrict__ partials
 template <class T>
 struct MinresArgs {
   int64_t n;
-  const int32_t* slice_ptr;
-  const int32_t* scol;
+  SellView sv;  // split off-diagonal layout
   const V2<T>* jval;
   const T* jblk;
   const V2<T>* y;  // preconditioned residual of the previous iteration
@@ -207,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_minres_a(MinresArgs<T> a) {
   if (row < a.n) {
     const V2<T> vold = a.v[row];
     if (S->pending) w_update<T>(a, S, it, row, vold);
-    const V2<T> acc = sell_row<T>(row, a.slice_ptr, a.scol, a.jval, a.y);
+    const V2<T> acc = sell_row<T>(row, a.sv, a.jval, a.y);
     const V2<T> yi = a.y[row];
     const T b0 = a.jblk[row], b1 = a.jblk[a.n + row], b2 = a.jblk[2 * a.n + row];
     const T ib = T(1.0 / beta);
@@ -307,8 +305,7 @@ static ChebArgs<T> cheb_args(const MeshCache& m, const PrecView<T>& P, MinresWor
                              V2<T>* R0, V2<T>* R1) {
   ChebArgs<T> a;
   a.n = m.n;
-  a.slice_ptr = m.slice_ptr.p;
-  a.scol = m.scol.p;
+  a.sv = sell_view(m);
   a.pval = P.val;
   a.pblk = P.blk;
   a.bound = P.bound;
@@ -361,8 +358,7 @@ int minres_solve(const MeshCache& m, const JacView<T>& J, const PrecView<T>& P,
   cheb_chain<T>(m, ca, 1, w.Y.p, w.Z2.p, s);
   MinresArgs<T> ma;
   ma.n = m.n;
-  ma.slice_ptr = m.slice_ptr.p;
-  ma.scol = m.scol.p;
+  ma.sv = sell_view(m);
   ma.jval = J.val;
   ma.jblk = J.blk;
   ma.y = w.Y.p;

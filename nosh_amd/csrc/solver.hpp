@@ -9,6 +9,9 @@ namespace nosh {
 struct MinresScalars {
   double beta1, beta, oldb, alfa, dbar, epsln, phibar, cs, sn;
   double p_oldeps, p_delta, p_denom, p_phi;  // pending w/x update
+  // Batched solver: phi of the last applied w update whose x update was
+  // deferred to the next one (0: none).
+  double x_phi;
   double relres, rtol;
   int it, maxit, done, pending;
 };

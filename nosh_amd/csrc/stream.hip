@@ -111,6 +111,13 @@ void launch(int kind, unsigned grid, f4v* a, f4v* b, int64_t nv, float* out, hip
 
 }  // namespace
 
+namespace nosh {
+void launch_cache_flush(const void* buf, int64_t bytes, float* out, unsigned grid,
+                        hipStream_t s) {
+  k_stream_read<false><<<grid, kThreads, 0, s>>>(static_cast<const f4v*>(buf), bytes / 16, out);
+}
+}  // namespace nosh
+
 extern "C" {
 
 // Average seconds per launch of the streaming kernel `kind` (0 read, 1

@@ -41,3 +41,46 @@ def test_internal_order_entries_match_reference_order():
     assert np.array_equal(out.cpu().numpy(), k_ref)
     with pytest.raises(KeyError):
         model.eval_f_internal(x.data_ptr(), {"mu": 0.4}, y.data_ptr())
+
+
+def test_jacobian_timing_entry_leaves_the_apply_result():
+    """The measurement entry (kernel-stamped events, optional cache eviction)
+    launches the same kernel: y holds J.phi afterwards, FP64 bitwise equal to
+    the reference-order apply and FP32 equal to the FP32 internal apply."""
+    import torch
+    import nosh_amd as nosh
+    c, t = mg.kuhn_cube(8, jitter=0.2, seed=4)
+    mesh = nosh.Mesh(c, t)
+    A = mg.magnetic_potential(c)
+    model = nosh.Nls(mesh, nosh.VectorFieldExplicitValues(mesh, A, 0.0),
+                     nosh.ScalarFieldConstant(mesh, -1.0), 1.0,
+                     nosh.ScalarFieldConstant(mesh, 1.0))
+    n = c.shape[0]
+    psi, phi = mg.random_state(n, 0), mg.random_state(n, 1)
+    p = {"g": 1.0, "mu": 0.3}
+    J = model.create_W_op()
+    J.rebuild(p, psi)
+    want = J.apply(phi)
+    dev = torch.device("cuda:0")
+    x_ref = torch.from_numpy(phi).to(dev)
+    x, y, out = torch.empty_like(x_ref), torch.empty_like(x_ref), torch.empty_like(x_ref)
+    torch.cuda.synchronize()
+    mesh.to_internal(x_ref.data_ptr(), x.data_ptr())
+    for cold in (False, True):
+        y.zero_()
+        torch.cuda.synchronize()
+        sec = J.bench_apply(x.data_ptr(), y.data_ptr(), cold=cold, reps=3)
+        assert 0.0 < sec < 1.0
+        mesh.to_reference(y.data_ptr(), out.data_ptr())
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), want)
+    x32 = x.float()
+    y32, z32 = torch.empty_like(x32), torch.empty_like(x32)
+    torch.cuda.synchronize()
+    J.apply_internal_fp32(x32.data_ptr(), z32.data_ptr())
+    sec = J.bench_apply(x32.data_ptr(), y32.data_ptr(), fp32=True, cold=True, reps=2)
+    torch.cuda.synchronize()
+    assert 0.0 < sec < 1.0
+    assert torch.equal(y32, z32)
+    with pytest.raises(ValueError):
+        J.bench_apply(x.data_ptr(), y.data_ptr(), reps=0)
