@@ -7,28 +7,32 @@ vertices, 5,308,416 tets, 6,276,384 edges); A = mu*0.5*(-y, x, 0); V = -1;
 thickness 1.
 
 Headline workload (``value``; one step = one continuation step of every
-branch): mu-continuation from mu = 0 to 1 in 50 equal steps of 8 branches
-(mu_0 = 0, g_b = 0.6 + 0.2 b, psi_0 = 1/sqrt(g_b), exact at mu = 0), each step
-solved to ||F|| <= 1e-10. Combined method: the branches batched in one
-instance-interleaved state (one launch per phase, hipGraph-captured MINRES
-iterations, autotuned block size), FP32 inner MINRES with FP64 Newton
-(Eisenstat-Walker forcing), secant predictor, and the BF16 MFMA MLP surrogate
-trained online on the run's own solutions seeding Newton where its guess has
-the smaller residual (tangent-free fallback to the secant). ``value`` = 8 x
-50 / wall time of the run (training and inference included). Plain
-baseline: the single-instance FP64 continuation (MINRES rtol 1e-10, tangent
-predictor, Chebyshev-Jacobi(6, 300), its fastest setting) of branch 0; the
-ratio is value / its solves/sec.
+branch): mu-continuation from mu = 0 to 1 in ``--steps`` equal steps (default
+50; 20 in the round-end run) of 8 branches (mu_0 = 0, g_b = 0.6 + 0.2 b,
+psi_0 = 1/sqrt(g_b), exact at mu = 0), each step solved to ||F|| <= 1e-10.
+Combined method: the branches batched in one instance-interleaved state (one
+launch per phase, hipGraph-captured MINRES iterations, autotuned block size),
+FP32 inner MINRES with FP64 Newton (Eisenstat-Walker forcing), the tangent
+predictor (one lockstep solve of J dpsi/dmu = -dF/dmu for all branches per
+step; ``--cont-predictor secant`` for the secant), lockstep retries at branch
+transitions, and the BF16 MFMA MLP surrogate trained online on the run's own
+solutions seeding Newton where its guess has the smaller residual (the
+predictor guess otherwise). ``value`` = 8 x steps / wall time of the run
+(training and inference included). Plain baseline: the single-instance FP64
+continuation (MINRES rtol 1e-10, tangent predictor, Chebyshev-Jacobi(6, 300),
+its fastest setting) of every branch; the ratio is value / its solves/sec.
 
 Side by side (the 96^3 random-start solve): plain FP64 and FP32-inner
 Newton-MINRES from psi_0 seeded random (|psi| ~ U(0.5,1), arg ~ U(0, 2 pi),
 seed 0) at mu = 0.4, g = 1, with ||F|| and ||psi - psi_FP64||.
 
-Roofline: the FP64 Jacobian apply J(psi) phi, timed with HIP events on the
-stream it is launched on; algorithmic bytes per launch 40E + 68n (complex CSR
-storing both directions: 16 B value + 4 B column per off-diagonal entry, 4 B
-row pointer and 8 B diagonal per row, phi / y / psi 16 B and c*t 8 B per
-vertex) against 8 TB/s.
+Roofline: the FP64 Jacobian apply J(psi) phi with cold caches (a 1 GiB read
+before each launch), its duration from the start/end events the HIP runtime
+stamps on each dispatch on the launch stream; algorithmic bytes per launch
+40E + 68n (complex CSR storing both directions: 16 B value + 4 B column per
+off-diagonal entry, 4 B row pointer and 8 B diagonal per row, phi / y / psi
+16 B and c*t 8 B per vertex) against 8 TB/s, beside the library's own
+streaming-kernel ceiling.
 
 CPU baseline: the oracle (a C++ restatement of the reference operators and
 solver, OpenMP) on a bounded sample on this host: the cost of F, the
@@ -454,7 +458,7 @@ def continuation_phase(args, model, n):
     # The batched run without the surrogate (the ablation's batching row, and
     # the surrogate's net gain in this same process).
     other_steps = 50 if steps != 50 else 20
-    no_sur, other = {"solves_per_s": None}, None
+    no_sur, other, alt_pred = {"solves_per_s": None}, None, None
     if not args.no_extras:
         t_ns, rec_ns, _, _ = _batched_run(batch, psi0, span, steps, kw, None,
                                           args.cont_predictor)
@@ -468,6 +472,12 @@ def continuation_phase(args, model, n):
         other["surrogate"] = sst_o
         say(f"continuation: combined at {other_steps} steps {t_o:.2f}s -> "
             f"{other['solves_per_s']:.3f}/s")
+        # The headline run with the other predictor (same process, same
+        # surrogate settings).
+        alt = "secant" if args.cont_predictor == "tangent" else "tangent"
+        t_a, rec_a, _, _ = _batched_run(batch, psi0, span, steps, kw, sur, alt)
+        alt_pred = {"predictor": alt, **_run_summary(nb, steps, t_a, rec_a)}
+        say(f"continuation: {alt} predictor {t_a:.2f}s -> {alt_pred['solves_per_s']:.3f}/s")
     del batch
     # Plain FP64 single-instance continuation of every branch.
     branches = list(range(min(args.cont_plain_branches, nb)))
@@ -516,6 +526,8 @@ def continuation_phase(args, model, n):
         "plain_fp64_solves_per_s": plain_rate, "plain_sample": plain_rec,
         "speedup_vs_plain_fp64": comb_rate / plain_rate if plain_rate else None,
         f"at_{other_steps}_steps": other,
+        "predictor": args.cont_predictor,
+        "other_predictor": alt_pred,
         "ablation_solves_per_s": abl,
     })
     return out

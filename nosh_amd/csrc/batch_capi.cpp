@@ -118,7 +118,7 @@ void batch_residuals(nosh_batch_s* h, std::vector<double>& norms, hipStream_t s)
 
 int newton_batch(nosh_batch_s* h, const nosh_newton_options& o, nosh_batch_instance_stats* st,
                  hipStream_t s, const int* skip, const int* max_iters, int stop_below,
-                 const double* ptc0) {
+                 const double* ptc0, int stall) {
   Batch& B = h->b;
   const int ni = B.args.ninst;
   B.args.iters = o.cheb_iters;
@@ -127,7 +127,8 @@ int newton_batch(nosh_batch_s* h, const nosh_newton_options& o, nosh_batch_insta
   B.args.z0 = std::max(o.cheb_iters - 1, 0) % 2;
   B.drop_graph();
   std::vector<double> nf, nf_prev(ni, -1.0), eta_prev(ni, o.forcing_max), rt(ni);
-  std::vector<int> done(ni, 0), newton(ni, 0), mres(ni, 0);
+  std::vector<int> done(ni, 0), newton(ni, 0), mres(ni, 0), last_gain(ni, 0);
+  std::vector<double> best(ni, -1.0), f1(ni, -1.0), f2(ni, -1.0);
   if (skip)
     for (int i = 0; i < ni; ++i) done[i] = skip[i] != 0;
   std::vector<double> final_nf(ni, 0.0);
@@ -148,6 +149,20 @@ int newton_batch(nosh_batch_s* h, const nosh_newton_options& o, nosh_batch_insta
       final_nf[i] = nf[i];
       if (!(nf[i] > o.tol) || !std::isfinite(nf[i]) ||
           it >= (max_iters ? max_iters[i] : o.max_iters)) {
+        done[i] = 1;
+        B.active_h[i] = 0;
+        continue;
+      }
+      // Stagnation: ||F|| not below half its best value for `stall` steps,
+      // and not halving on each of the last two steps either (a run that
+      // descends steadily from a jump up goes on).
+      const bool descending = f2[i] > 0.0 && nf[i] < 0.5 * f1[i] && f1[i] < 0.5 * f2[i];
+      f2[i] = f1[i];
+      f1[i] = nf[i];
+      if (best[i] < 0.0 || nf[i] < 0.5 * best[i]) {
+        best[i] = nf[i];
+        last_gain[i] = it;
+      } else if (stall > 0 && it - last_gain[i] >= stall && !descending) {
         done[i] = 1;
         B.active_h[i] = 0;
         continue;

@@ -65,6 +65,14 @@ int bcguard(F&& f) {
 // Newton steps of the first try from each guess (12 measured slower at 96^3:
 // more branches reach the full-length tries).
 constexpr int kShortNewton = 20;
+// A short try stops a branch whose ||F|| has not fallen below half its best
+// value for this many steps and is not halving step after step: at the 96^3
+// transitions Newton from a wrong guess oscillates between 5e-3 and 3e-2 for
+// all 20 steps, while the tries that converge descend steadily, also after a
+// jump up (96^3, 20 steps: mu 0.30 -> 0.35 from the tangent guess in 14
+// steps via ||F|| = 3.7e3, falling 3.4x a step; mu 0.95 -> 1.0 from psi_k in
+// 16 steps via ||F|| = 4).
+constexpr int kStallSteps = 8;
 // Chebyshev degree of the single-instance FP64 fallback solves (the plain
 // solver's best).
 constexpr int kFallbackCheb = 6;
@@ -608,7 +616,7 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
           sig0.assign(ni, kPtcShift0);
         }
         const int it = newton_batch(bt, oa, st.data(), s, skip.data(), maxit.data(), 0,
-                                    ptc ? sig0.data() : nullptr);
+                                    ptc ? sig0.data() : nullptr, ptc ? 0 : kStallSteps);
         int mx = 0, nc = 0;
         for (int i = 0; i < ni; ++i)
           if (!skip[i]) {

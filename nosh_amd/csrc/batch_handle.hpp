@@ -29,10 +29,13 @@ void gather_all(nosh_batch_s* h, double* dst_dev, hipStream_t s);
 // steps).
 // stop_below > 0: return as soon as at most stop_below instances are still
 // iterating (they come back unconverged with newton_iters < their limit and
-// their current iterate in the state).
+// their current iterate in the state). ptc0: initial pseudo-transient shifts
+// per instance. stall > 0: an instance whose ||F|| has not fallen below half
+// its best value for `stall` steps stops, unconverged (Newton wandering at a
+// branch transition).
 int newton_batch(nosh_batch_s* h, const nosh_newton_options& o, nosh_batch_instance_stats* st,
                  hipStream_t s, const int* skip = nullptr, const int* max_iters = nullptr,
-                 int stop_below = 0, const double* ptc0 = nullptr);
+                 int stop_below = 0, const double* ptc0 = nullptr, int stall = 0);
 
 // Tangent predictor of every instance: rebuilds J at the current psi, solves
 // J z = -scale dF/dmu (FP32 MINRES to relative tolerance rtol, all instances

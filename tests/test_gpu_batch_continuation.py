@@ -72,7 +72,9 @@ def test_batch_tangent_guess_halves_the_secant_error():
     of the tangent run are about half the secant run's, on steps 2-6 of the
     same branches (mu <= 0.24; on later steps of this run psi'' changes
     from step to step, and the ratio leaves 1/2: 0.55 and 0.6 at steps 7
-    and 8, 0.96 and 0.28 at steps 9 and 10). Both runs end on the same solutions (oracle)."""
+    and 8, 0.96 and 0.28 at steps 9 and 10). Both runs end on the same
+    solutions (oracle), up to the global phase."""
+    from conftest import phase_align
     c, t, model, batch, psi0 = _setup()
     span, steps = 0.2, 10
     errs, finals = {}, {}
@@ -85,7 +87,7 @@ def test_batch_tangent_guess_halves_the_secant_error():
     print("tangent / secant guess error per step", np.round(q, 3))
     assert np.all(q > 0.45) and np.all(q < 0.6)
     for a, b in zip(finals["tangent"], finals["secant"]):
-        assert np.linalg.norm(a - b) <= 1e-8 * np.linalg.norm(b)
+        assert np.linalg.norm(phase_align(a, b) - b) <= 1e-8 * np.linalg.norm(b)
     _oracle_check(c, t, MU0 + span, finals["tangent"])
 
 
