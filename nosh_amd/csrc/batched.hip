@@ -481,7 +481,14 @@ __device__ __forceinline__ double2 brow_dmu(const BGroup& G, int64_t row, int b,
   for (int32_t k = 0; k < w; ++k, j += kSliceH) {
     const double at = G.t64[j];
     double vr, vi;
-    slot_value(G.a64[j], at, mu, vr, vi);
+    if (G.kval64) {
+      // The values at the group's shared mu, the bits of slot_value.
+      const double2 v = G.kval64[j];
+      vr = v.x;
+      vi = v.y;
+    } else {
+      slot_value(G.a64[j], at, mu, vr, vi);
+    }
     cmac(ar, ai, at * vi, -at * vr, x[int64_t(G.scol[j]) * G.bp + b]);
   }
   return make_double2(ar, ai);
@@ -584,24 +591,49 @@ struct Ctx {
   bool lane_ok;  // a real instance (b < B)
 };
 
-// Row tiles of launched blocks. A group's launched blocks take its tiles
-// strided by their count (block k: tiles k, k + nl, k + 2 nl, ... of the
-// group, nl = nblocks / fold), so that the blocks resident at one time work
-// on one band of rows, whose gathers fit an XCD's L2, rather than each on its
-// own stretch of fold consecutive tiles (with fold 8 and 512 threads that put
-// an XCD's whole eighth of a 96^3 mesh in flight and the gathers missed L2).
-// v0 = rb * fold of launched block rb; first_tile is the block's index k in
-// its group, which also numbers its partial sums.
-__device__ __forceinline__ int64_t first_tile(const BatchArgs& a, const BGroup& G, int64_t v0) {
-  return (v0 - G.block0) / a.fold;
-}
-__device__ __forceinline__ int64_t tile_stride(const BatchArgs& a, const BGroup& G) {
-  return G.nblocks / a.fold;
+// Row tiles of launched blocks. The launched blocks form 8 contiguous ranges,
+// one per XCD (the order xcd_block gives them), and within a group each
+// range's blocks take the range's own contiguous share of the group's tiles,
+// strided by their count (block j of a range of c blocks: tiles s + j,
+// s + j + c, s + j + 2c, ...). The blocks resident on one XCD then sweep
+// one band of rows through their own stretch of the mesh, so the halo their
+// gathers read beyond the band was mostly read by that XCD's previous band
+// and is still in its L2 (each XCD's stretch has two ends, where the
+// neighbouring XCD's rows are read again). Before, the eight XCDs took
+// adjacent bands of one sweep of the whole group (block k: tiles k, k + nl,
+// ...), and every band's halo was the neighbouring XCDs' rows, read again
+// (96^3 x 8 lanes: ~2x the gathered vector's bytes). The mapping is a
+// function of the logical block index alone (the persistent kernel, which
+// does not remap, takes the same tiles and partial-sum slots). v0 = rb * fold
+// of launched block rb.
+struct TileMap {
+  int64_t first;   // the block's first tile in its group
+  int64_t stride;  // tiles from one of its tiles to the next
+};
+__device__ __forceinline__ TileMap tile_map(const BatchArgs& a, const BGroup& G, int64_t v0) {
+  const int64_t rb = v0 / a.fold;
+  const int64_t nb = a.total_blocks / a.fold;
+  const int64_t q = nb / 8, r = nb % 8;
+  int64_t xs, xc;
+  if (rb < r * (q + 1)) {
+    xs = rb / (q + 1) * (q + 1);
+    xc = q + 1;
+  } else {
+    xs = r * (q + 1) + (rb - r * (q + 1)) / q * q;
+    xc = q;
+  }
+  const int64_t g0 = G.block0 / a.fold, nl = G.nblocks / a.fold;
+  const int64_t s0 = max(g0, xs), s1 = min(g0 + nl, xs + xc);
+  TileMap m;
+  m.first = (s0 - g0) * a.fold + (rb - s0);
+  m.stride = s1 - s0;
+  return m;
 }
 // Virtual block (tile) t of launched block rb.
 __device__ __forceinline__ int64_t tile_vb(const BatchArgs& a, int64_t rb, int t) {
   const BGroup& G = a.groups[group_of(a, rb * a.fold)];
-  return G.block0 + first_tile(a, G, rb * a.fold) + int64_t(t) * tile_stride(a, G);
+  const TileMap m = tile_map(a, G, rb * a.fold);
+  return G.block0 + m.first + int64_t(t) * m.stride;
 }
 
 __device__ __forceinline__ Ctx make_ctx(const BatchArgs& a, int64_t v0) {
@@ -611,8 +643,9 @@ __device__ __forceinline__ Ctx make_ctx(const BatchArgs& a, int64_t v0) {
   c.b = threadIdx.x & (bp - 1);
   c.inst = c.G.inst0 + c.b;
   c.rows_per_vb = int(blockDim.x) / bp;
-  c.row0 = first_tile(a, c.G, v0) * c.rows_per_vb + threadIdx.x / bp;
-  c.tstride = tile_stride(a, c.G) * c.rows_per_vb;
+  const TileMap m = tile_map(a, c.G, v0);
+  c.row0 = m.first * c.rows_per_vb + threadIdx.x / bp;
+  c.tstride = m.stride * c.rows_per_vb;
   c.lane_ok = c.b < c.G.B;
   return c;
 }
@@ -626,7 +659,7 @@ struct WUpd {
 };
 
 // w_k = (v_k - oldeps w_{k-2} - delta w_{k-1}) / gamma, every operation
-// explicit (the one-lane and the L-lane kernels give the same bits).
+// explicit, no contraction left to the compiler.
 __device__ __forceinline__ float2 w_next(const WUpd& q, float2 vk, float2 w1, float2 w2) {
 #pragma clang fp contract(off)
   return make_float2(
@@ -660,10 +693,11 @@ __device__ __forceinline__ void w_update(const BGroup& G, const WUpd& q, int k, 
                                          float2 vk, bool defer) {
   float2* wk = wbuf(G, k);
   const float2 w1 = ldn(wbuf(G, k - 2) + idx), w2 = ldn(wbuf(G, k - 1) + idx);
+  const double2 xv = defer ? make_double2(0.0, 0.0) : ldn(G.x + idx);
   const float2 w = w_next(q, vk, w1, w2);
   stn(wk + idx, w);
   if (defer) return;
-  stn(G.x + idx, x_next(q, ldn(G.x + idx), w2, w));
+  stn(G.x + idx, x_next(q, xv, w2, w));
 }
 
 __device__ __forceinline__ WUpd wupd_of(const MinresScalars& S) {
@@ -888,8 +922,9 @@ __device__ __forceinline__ VCtx<L> make_vctx(const BatchArgs& a, int64_t v0) {
   c.T = c.G.bp / L;
   c.R = int(blockDim.x) / c.T;
   c.b0 = int(threadIdx.x % c.T) * L;
-  c.row0 = first_tile(a, c.G, v0) * c.R + threadIdx.x / c.T;
-  c.tstride = tile_stride(a, c.G) * c.R;
+  const TileMap m = tile_map(a, c.G, v0);
+  c.row0 = m.first * c.R + threadIdx.x / c.T;
+  c.tstride = m.stride * c.R;
   return c;
 }
 
@@ -1103,104 +1138,6 @@ __device__ void rows_minres_a_v(const BatchArgs& a, const VCtx<L>& c, double (&s
   }
 }
 
-// Chebyshev step 0 (and the previous iteration's w/x update) for L lanes of
-// a row per thread: the arithmetic of rows_cheb_first per (row, lane), with
-// 16-byte (w, v, u, r, z) and 32-byte (x) accesses.
-template <int L>
-__device__ void rows_cheb_first_v(const BatchArgs& a, const VCtx<L>& c, int mode,
-                                  double (&s)[L]) {
-  const BGroup& G = c.G;
-  bool act[L];
-  WUpd q[L];
-  float cab[L], th[L];
-  bool any = false, all = true, pending = false;
-  int it = 0;
-#pragma unroll
-  for (int l = 0; l < L; ++l) {
-    s[l] = 0.0;
-    act[l] = false;
-    q[l] = WUpd{0.f, 0.f, 0.f, 0.0, 0.0};
-    cab[l] = 0.0f;
-    th[l] = 1.0f;
-    const int b = c.b0 + l;
-    if (b < G.B) {
-      const int inst = G.inst0 + b;
-      const MinresScalars& S = a.S[inst];
-      if (!S.done) {
-        act[l] = true;
-        any = true;
-        // The same for every active instance (lockstep).
-        it = mode == 2 ? S.it : 0;
-        pending = mode == 2 && S.pending;
-        q[l] = wupd_of(S);
-        cab[l] = mode == 2 ? float(S.alfa / S.beta) : 0.0f;
-        const double hi = hi_of(a, inst);
-        th[l] = float(0.5 * (hi + hi / a.ratio));
-      }
-    }
-    all = all && act[l];
-  }
-  if (!any) return;
-  const bool defer = (it & 1) != 0;
-  const float2* vprev = vbuf(G, it - 1);
-  const float2* w1p = wbuf(G, it - 2);
-  const float2* w2p = wbuf(G, it - 1);
-  float2* wk = wbuf(G, it);
-  float2* r = rbuf(G, it);
-  const float2* r2 = rbuf(G, it + 1);
-  float2* z0 = zbuf(G, a.z0);
-  const int64_t nb3 = G.n * G.bp;
-  for (int t = 0; t < a.fold; ++t) {
-    const int64_t row = c.row0 + int64_t(t) * c.tstride;
-    if (row >= G.n) break;
-    const int64_t idx0 = row * G.bp + c.b0;
-    if (pending) {
-      float2 vk[L], w1[L], w2[L], w[L];
-      load_lanes_nt<L>(vprev + idx0, vk);
-      load_lanes_nt<L>(w1p + idx0, w1);
-      load_lanes_nt<L>(w2p + idx0, w2);
-#pragma unroll
-      for (int l = 0; l < L; ++l) w[l] = w_next(q[l], vk[l], w1[l], w2[l]);
-      store_lanes_nt<L>(wk + idx0, w, all, act);
-      if (!defer) {
-#pragma unroll
-        for (int l = 0; l < L; ++l)
-          if (act[l]) stn(G.x + idx0 + l, x_next(q[l], ldn(G.x + idx0 + l), w2[l], w[l]));
-      }
-    }
-    float2 rv[L], z[L];
-    if (mode == 2) {
-      float2 uu[L], r2v[L];
-      load_lanes_nt<L>(G.u + idx0, uu);
-      load_lanes_nt<L>(r2 + idx0, r2v);
-#pragma unroll
-      for (int l = 0; l < L; ++l) rv[l] = r_next(uu[l], r2v[l], cab[l]);
-      store_lanes_nt<L>(r + idx0, rv, all, act);
-    } else {
-      load_lanes_nt<L>(r + idx0, rv);
-    }
-    if (a.iters == 0) {
-#pragma unroll
-      for (int l = 0; l < L; ++l) z[l] = rv[l];
-    } else {
-      float b0[L], b1[L], b2[L];
-      load_floats_nt<L>(G.pblk + idx0, b0);
-      load_floats_nt<L>(G.pblk + nb3 + idx0, b1);
-      load_floats_nt<L>(G.pblk + 2 * nb3 + idx0, b2);
-#pragma unroll
-      for (int l = 0; l < L; ++l) {
-        float o0, o1;
-        dinv32(b0[l], b1[l], b2[l], rv[l].x, rv[l].y, &o0, &o1);
-        z[l] = make_float2(o0 / th[l], o1 / th[l]);
-      }
-    }
-    store_lanes_nt<L>(z0 + idx0, z, all, act);
-#pragma unroll
-    for (int l = 0; l < L; ++l)
-      if (act[l]) s[l] += dot2(rv[l], z[l]);
-  }
-}
-
 __device__ void ph_flush(const BatchArgs& a, int64_t vb) {
   const Lane L = lane_ctx(a, vb);
   if (!L.valid || !a.active[L.inst]) return;
@@ -1296,14 +1233,6 @@ __global__ void kb_cheb_step_v(BatchArgs a, int mode, int j) {
   double s[L];
   rows_cheb_step_v<L>(a, c, mode, j, s);
   if (j == a.iters - 1) lane_partials_v<L>(a, v0, c, s);
-}
-template <int L>
-__global__ void kb_cheb_first_v(BatchArgs a, int mode) {
-  const int64_t v0 = xcd_block(true) * a.fold;
-  const VCtx<L> c = make_vctx<L>(a, v0);
-  double s[L];
-  rows_cheb_first_v<L>(a, c, mode, s);
-  if (a.iters <= 1) lane_partials_v<L>(a, v0, c, s);
 }
 template <int L>
 __global__ void kb_minres_a_v(BatchArgs a) {
@@ -1464,10 +1393,10 @@ void Batch::launch_phases_iteration(hipStream_t s) {
   const unsigned nb = unsigned(args.total_blocks / args.fold), bt = unsigned(block_threads);
   launch_minres_a(s);
   kb_finalize<<<args.ninst, 256, 0, s>>>(args, 2, nullptr);
-  if (lanes == kVecLanes)
-    kb_cheb_first_v<kVecLanes><<<nb, bt / kVecLanes, 0, s>>>(args, 2);
-  else
-    kb_cheb_first<<<nb, bt, 0, s>>>(args, 2);
+  // One lane per thread in every mode: a two-lane form of this elementwise
+  // kernel (16- and 32-byte accesses, 92 VGPRs) measured 131.6 against
+  // 115.8 us per launch at 96^3 x 8 lanes.
+  kb_cheb_first<<<nb, bt, 0, s>>>(args, 2);
   for (int j = 1; j < args.iters; ++j) launch_cheb_step(s, 2, j);
   kb_finalize<<<args.ninst, 256, 0, s>>>(args, 3, nullptr);
 }
@@ -1705,6 +1634,7 @@ void Batch::compute_norms(hipStream_t s, std::vector<double>& nrm) {
 
 void Batch::dfdmu_into_f(hipStream_t s, double scale) {
   const unsigned nb = unsigned(args.total_blocks / args.fold), bt = unsigned(block_threads);
+  prepare_kval64(s);
   kb_dfdmu<<<nb, bt, 0, s>>>(args, scale);
 }
 

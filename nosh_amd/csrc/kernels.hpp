@@ -124,20 +124,35 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* __restri
   }
 }
 
-// Sums nb partials per value in fixed order; totals valid in thread 0.
+constexpr int kFinalizeThreads = 256;
+
+// Sums nb partials per value in fixed order; totals valid in thread 0. The
+// order is that of a block of kFinalizeThreads threads whatever the block
+// size (a multiple of 64): virtual thread t adds partials t, t + 256, ...,
+// each virtual wave of 64 is reduced by lane shuffles, and the 4 wave sums
+// by a last one, so the persistent kernel (blocks of 128 to 1024 threads)
+// and the separate finalize kernels (256) give the same bits.
 template <int NV>
 __device__ __forceinline__ void finalize_sum(double (&v)[NV], const double* __restrict__ partials,
                                              int nb) {
   __shared__ double lds[16 * NV];
+  constexpr int kVw = kFinalizeThreads / 64;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = max(1, int(blockDim.x) >> 6);
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partials[q * nb + i];
-    v[q] = s;
+    for (int vw = wid; vw < kVw; vw += nw) {
+      double s = 0.0;
+      for (int i = vw * 64 + lane; i < nb; i += kFinalizeThreads) s += partials[q * nb + i];
+      s = wave_sum(s);
+      if (lane == 0) lds[q * 16 + vw] = s;
+    }
   }
-  block_sum<NV>(v, lds);
+  __syncthreads();
+  if (wid == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = wave_sum(lane < kVw ? lds[q * 16 + lane] : 0.0);
+  }
+  __syncthreads();
 }
-
-constexpr int kFinalizeThreads = 256;
 
 }  // namespace nosh

@@ -147,14 +147,20 @@ def test_persistent_mode_large_grid_matches_graph():
     opts = dict(tol=1e-10, max_iters=60, minres_rtol=1e-6, minres_max_iters=2000, cheb_iters=4,
                 cheb_ratio=100.0, forcing=0)
     out = {}
-    for mode in (0, 1):
-        print(f"[persistent large grid: mode {mode}, 256 threads, 35937 rows x 8 lanes]",
-              flush=True)
-        batch.set_tuning(256, mode, 4, 16, fold=1)
-        psi, st = batch.solve(psi0, **opts)
-        assert all(s["converged"] for s in st), st
-        out[mode] = psi
-    assert np.array_equal(out[0], out[1])
+    # 512 threads: the persistent kernel's blocks are twice the size of the
+    # finalize kernels' (whose sums of the > 256 block partials must not
+    # depend on it); mode 2 runs the two-lane kernels.
+    for bt, modes in ((512, (0, 1, 2)), (256, (0, 1))):
+        for mode in modes:
+            print(f"[persistent large grid: mode {mode}, {bt} threads, 35937 rows x 8 lanes]",
+                  flush=True)
+            batch.set_tuning(bt, mode, 4, 16, fold=1)
+            psi, st = batch.solve(psi0, **opts)
+            assert all(s["converged"] for s in st), st
+            out[bt, mode] = psi
+        for mode in modes[1:]:
+            assert np.array_equal(out[bt, 0], out[bt, mode]), (bt, mode)
+    out[0] = out[256, 0]
     print("[persistent large grid: forced barrier time-out]", flush=True)
     os.environ["NOSH_BARRIER_TIMEOUT_S"] = "0"
     try:
