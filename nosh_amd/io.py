@@ -25,8 +25,8 @@ The reference writes MOAB .h5m files (HDF5); HDF5 is not available here, so
 state files are VTK XML unstructured grids (.vtu, raw little-endian appended
 data, readable by ParaView) with point data "psi" (Re, Im) and field data
 with the parameters, and checkpoints are NumPy .npz archives (no pickles).
-Meshes are read from .vtu and legacy ASCII .vtk files; cells keep their file
-order, so ``nosh_amd.Mesh`` numbers edges as the reference does for the same
+Meshes are read from .vtu, legacy ASCII .vtk and Exodus II (netCDF classic)
+files; cells keep their file order, so ``nosh_amd.Mesh`` numbers edges as the reference does for the same
 cell order (first appearance, src/mesh.cpp:629-691).
 """
 from __future__ import annotations
@@ -369,15 +369,178 @@ def read_vtk_legacy(file_name: str):
     return pts, cells, pd
 
 
+# ------------------------------------------------------------- Exodus II
+# Exodus II meshes (the reference's test meshes are .e files, read through
+# MOAB, src/mesh_reader.cpp:19-162) in the netCDF classic formats (CDF-1 and
+# the 64-bit-offset CDF-2), through scipy.io.netcdf_file. netCDF-4 (HDF5)
+# Exodus files need HDF5, which is not available. Vertices keep their file
+# order and cells the order of their element blocks, block by block, the
+# order MOAB gives handles in, so nosh_amd.Mesh numbers the edges as the
+# reference does (first appearance over the cells in that order).
+_EXO_TRI = ("TRI", "TRI3", "TRIANGLE", "TRIANGLE3")
+_EXO_TET = ("TETRA", "TETRA4", "TET", "TET4")
+_EXO_COMPONENTS = ("_X", "_Y", "_Z")
+
+
+def _exo_text(a) -> str:
+    """A netCDF char array (or attribute bytes) as a stripped str."""
+    if isinstance(a, (bytes, bytearray)):
+        b = bytes(a)
+    else:
+        b = b"".join(np.asarray(a).ravel().tolist())
+    return b.split(b"\0", 1)[0].decode("ascii", "replace").strip()
+
+
+def read_exodus(file_name: str, step: int = -1):
+    """(coords n x 3, cells T x 3 or T x 4, point_data) of an Exodus II file.
+
+    point_data holds the nodal variables at time step `step` (default: the
+    last); variables named <base>_X, <base>_Y[, <base>_Z] become one n x 2 or
+    n x 3 array <base> (so "A_X", "A_Y", "A_Z" give the magnetic potential
+    "A"), <base>_real / <base>_imag one n x 2 array. Element blocks must all
+    be linear triangles or all linear tetrahedra; connectivity is 1-based in
+    the file and 0-based here."""
+    from scipy.io import netcdf_file
+    with netcdf_file(file_name, "r", mmap=False) as f:
+        dims, var = f.dimensions, f.variables
+        nn = int(dims["num_nodes"])
+        nd = int(dims["num_dim"])
+        coords = np.zeros((nn, 3))
+        if "coord" in var:
+            coords[:, :nd] = np.array(var["coord"][:], dtype=np.float64).T
+        else:
+            for d, nm in enumerate(("coordx", "coordy", "coordz")[:nd]):
+                coords[:, d] = np.array(var[nm][:], dtype=np.float64)
+        nblk = int(dims.get("num_el_blk", 0) or 0)
+        status = np.array(var["eb_status"][:]) if "eb_status" in var else np.ones(nblk, int)
+        blocks, kind = [], None
+        for b in range(1, nblk + 1):
+            if status[b - 1] == 0 or f"connect{b}" not in var:
+                continue
+            v = var[f"connect{b}"]
+            et = _exo_text(getattr(v, "elem_type", b"")).upper()
+            k = "tri" if et in _EXO_TRI else "tet" if et in _EXO_TET else None
+            if k is None:
+                raise ValueError(f"{file_name}: element block {b} has type {et!r}; only "
+                                 f"linear triangles (TRI3) and tetrahedra (TETRA4) are supported")
+            if kind is not None and k != kind:
+                raise ValueError(f"{file_name}: mixed triangle and tetrahedron blocks")
+            kind = k
+            blocks.append(np.array(v[:], dtype=np.int64) - 1)
+        if not blocks:
+            raise ValueError(f"{file_name}: no element blocks")
+        cells = np.ascontiguousarray(np.concatenate(blocks), dtype=np.int32)
+        if cells.min() < 0 or cells.max() >= nn:
+            raise ValueError(f"{file_name}: connectivity outside the {nn} nodes")
+        names = []
+        if "name_nod_var" in var:
+            names = [_exo_text(row) for row in np.asarray(var["name_nod_var"][:])]
+        raw = {}
+        for i, nm in enumerate(names):
+            if f"vals_nod_var{i + 1}" in var:
+                raw[nm] = np.array(var[f"vals_nod_var{i + 1}"][step], dtype=np.float64)
+            elif "vals_nod_var" in var:
+                raw[nm] = np.array(var["vals_nod_var"][step, i], dtype=np.float64)
+    pd: Dict[str, np.ndarray] = {}
+    used = set()
+    for nm in names:
+        up = nm.upper()
+        for suf in (_EXO_COMPONENTS[0], "_REAL"):
+            if up.endswith(suf):
+                base = nm[:-len(suf)]
+                parts = ([base + c for c in ("_X", "_Y", "_Z")] if suf == "_X" else
+                         [base + "_real", base + "_imag"])
+                found = [next((m for m in names if m.upper() == q.upper()), None) for q in parts]
+                if suf == "_X" and found[2] is None:
+                    found = found[:2]
+                if all(m is not None for m in found) and len(found) >= 2:
+                    pd[base] = np.stack([raw[m] for m in found], axis=1)
+                    used.update(found)
+    for nm in names:
+        if nm not in used:
+            pd[nm] = raw[nm]
+    return coords, cells, pd
+
+
+def write_exodus(file_name: str, coords, cells, point_data: Optional[Dict[str, np.ndarray]] = None,
+                 time: float = 0.0, title: str = "nosh_amd") -> None:
+    """Writes a one-block, one-time-step Exodus II file (netCDF CDF-2). Point
+    data of shape n become one nodal variable, n x 2 or n x 3 arrays the
+    variables <name>_X, <name>_Y[, <name>_Z] (read back by read_exodus as one
+    array). 2D meshes (coords n x 2, or n x 3 with z = 0, and triangles) are
+    written with num_dim 2."""
+    from scipy.io import netcdf_file
+    coords = np.asarray(coords, dtype=np.float64)
+    cells = np.asarray(cells, dtype=np.int64)
+    if cells.ndim != 2 or cells.shape[1] not in (3, 4):
+        raise ValueError("cells must be T x 3 (triangles) or T x 4 (tetrahedra)")
+    nn = coords.shape[0]
+    nd = 2 if cells.shape[1] == 3 and (coords.shape[1] == 2 or not np.any(coords[:, 2])) else 3
+    names, vals = [], []
+    for k, a in (point_data or {}).items():
+        a = np.asarray(a, dtype=np.float64)
+        if a.ndim == 1:
+            names.append(k)
+            vals.append(a)
+        else:
+            for j in range(a.shape[1]):
+                names.append(k + _EXO_COMPONENTS[j])
+                vals.append(a[:, j])
+    with netcdf_file(file_name, "w", version=2) as f:
+        f.api_version = np.float32(5.0)
+        f.version = np.float32(5.0)
+        f.floating_point_word_size = np.int32(8)
+        f.file_size = np.int32(1)
+        f.title = title
+        f.createDimension("time_step", None)  # the record dimension comes first
+        f.createDimension("len_string", 33)
+        f.createDimension("len_line", 81)
+        f.createDimension("four", 4)
+        f.createDimension("len_name", 33)
+        f.createDimension("num_dim", nd)
+        f.createDimension("num_nodes", nn)
+        f.createDimension("num_elem", cells.shape[0])
+        f.createDimension("num_el_blk", 1)
+        f.createDimension("num_el_in_blk1", cells.shape[0])
+        f.createDimension("num_nod_per_el1", cells.shape[1])
+        t = f.createVariable("time_whole", "d", ("time_step",))
+        t[0] = time
+        f.createVariable("eb_status", "i", ("num_el_blk",))[:] = 1
+        prop = f.createVariable("eb_prop1", "i", ("num_el_blk",))
+        prop[:] = 1
+        prop.name = "ID"
+        for d, nm in enumerate(("coordx", "coordy", "coordz")[:nd]):
+            f.createVariable(nm, "d", ("num_nodes",))[:] = coords[:, d]
+        cn = f.createVariable("coor_names", "c", ("num_dim", "len_name"))
+        for d, nm in enumerate("xyz"[:nd]):
+            cn[d, :] = np.array(list(nm.ljust(33, "\0")), dtype="S1")
+        conn = f.createVariable("connect1", "i", ("num_el_in_blk1", "num_nod_per_el1"))
+        conn[:] = (cells + 1).astype(np.int32)
+        conn.elem_type = "TETRA4" if cells.shape[1] == 4 else "TRI3"
+        if names:
+            f.createDimension("num_nod_var", len(names))
+            nv = f.createVariable("name_nod_var", "c", ("num_nod_var", "len_name"))
+            for i, nm in enumerate(names):
+                if len(nm) > 32:
+                    raise ValueError(f"variable name {nm!r} longer than 32 characters")
+                nv[i, :] = np.array(list(nm.ljust(33, "\0")), dtype="S1")
+            for i, a in enumerate(vals):
+                v = f.createVariable(f"vals_nod_var{i + 1}", "d", ("time_step", "num_nodes"))
+                v[0, :] = a
+
+
 def read_mesh(file_name: str):
-    """(coords, cells, point_data) from a .vtu or legacy .vtk file."""
+    """(coords, cells, point_data) from a .vtu, legacy .vtk or Exodus II
+    (.e, .exo, .ex2, .g; netCDF classic) file."""
     if file_name.endswith(".vtu"):
         c, t, pd, _ = read_vtu(file_name)
         return c, t, pd
     if file_name.endswith(".vtk"):
         return read_vtk_legacy(file_name)
-    raise ValueError(f"unknown mesh format: {file_name} (.vtu and .vtk are read; MOAB .h5m "
-                     f"needs HDF5, which is not available)")
+    if file_name.endswith((".e", ".exo", ".ex2", ".g")):
+        return read_exodus(file_name)
+    raise ValueError(f"unknown mesh format: {file_name} (.vtu, .vtk and Exodus II .e are read; "
+                     f"MOAB .h5m needs HDF5, which is not available)")
 
 
 # ------------------------------------------------------------------ states
