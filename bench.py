@@ -8,14 +8,15 @@ thickness 1.
 
 Headline workload (``value``; one step = one continuation step of every
 branch): mu-continuation from mu = 0 to 1 in ``--steps`` equal steps (default
-50; 20 in the round-end run) of 8 branches (mu_0 = 0, g_b = 0.6 + 0.2 b,
+50) of 8 branches (mu_0 = 0, g_b = 0.6 + 0.2 b,
 psi_0 = 1/sqrt(g_b), exact at mu = 0), each step solved to ||F|| <= 1e-10.
 Combined method: the branches batched in one instance-interleaved state (one
 launch per phase, hipGraph-captured MINRES iterations, autotuned block size),
 FP32 inner MINRES with FP64 Newton (Eisenstat-Walker forcing), the tangent
-predictor (one lockstep solve of J dpsi/dmu = -dF/dmu for all branches per
-step; ``--cont-predictor secant`` for the secant), lockstep retries at branch
-transitions, and the BF16 MFMA MLP surrogate trained online on the run's own
+predictor (one lockstep solve of J dpsi/dmu = -dF/dmu for all branches) on the
+first step and after a step that a branch did not pass from its first guess,
+the secant otherwise (``--cont-predictor`` tangent / secant for one of them
+throughout), lockstep retries at branch transitions, and the BF16 MFMA MLP surrogate trained online on the run's own
 solutions seeding Newton where its guess has the smaller residual (the
 predictor guess otherwise). ``value`` = 8 x steps / wall time of the run
 (training and inference included). Plain baseline: the single-instance FP64
@@ -107,8 +108,10 @@ def parse():
                    help="block_threads,fold[,mode] for the batched continuation (skips the "
                         "autotuner; mode 0 by default)")
     p.add_argument("--no-surrogate", action="store_true")
-    p.add_argument("--cont-predictor", default="tangent", choices=["tangent", "secant"],
-                   help="predictor of the batched continuation")
+    p.add_argument("--cont-predictor", default="auto", choices=["auto", "tangent", "secant"],
+                   help="predictor of the batched continuation (auto: the tangent on the first "
+                        "step and after a step a branch did not pass from its first guess, the "
+                        "secant otherwise)")
     p.add_argument("--cont-smooth", type=int, default=4,
                    help="Jacobi sweeps on the residual of each surrogate guess")
     p.add_argument("--cont-accept", type=float, default=0.5,
@@ -472,12 +475,16 @@ def continuation_phase(args, model, n):
         other["surrogate"] = sst_o
         say(f"continuation: combined at {other_steps} steps {t_o:.2f}s -> "
             f"{other['solves_per_s']:.3f}/s")
-        # The headline run with the other predictor (same process, same
+        # The headline run with the other predictors (same process, same
         # surrogate settings).
-        alt = "secant" if args.cont_predictor == "tangent" else "tangent"
-        t_a, rec_a, _, _ = _batched_run(batch, psi0, span, steps, kw, sur, alt)
-        alt_pred = {"predictor": alt, **_run_summary(nb, steps, t_a, rec_a)}
-        say(f"continuation: {alt} predictor {t_a:.2f}s -> {alt_pred['solves_per_s']:.3f}/s")
+        alt_pred = {}
+        for alt in ("auto", "tangent", "secant"):
+            if alt == args.cont_predictor:
+                continue
+            t_a, rec_a, _, _ = _batched_run(batch, psi0, span, steps, kw, sur, alt)
+            alt_pred[alt] = _run_summary(nb, steps, t_a, rec_a)
+            say(f"continuation: {alt} predictor {t_a:.2f}s -> "
+                f"{alt_pred[alt]['solves_per_s']:.3f}/s")
     del batch
     # Plain FP64 single-instance continuation of every branch.
     branches = list(range(min(args.cont_plain_branches, nb)))
@@ -527,7 +534,7 @@ def continuation_phase(args, model, n):
         "speedup_vs_plain_fp64": comb_rate / plain_rate if plain_rate else None,
         f"at_{other_steps}_steps": other,
         "predictor": args.cont_predictor,
-        "other_predictor": alt_pred,
+        "other_predictors": alt_pred,
         "ablation_solves_per_s": abl,
     })
     return out

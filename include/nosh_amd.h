@@ -495,7 +495,10 @@ int nosh_continuation_run_surrogate(nosh_nls_t nls, double* psi, int nparams,
  * predictor guess -- predictor 1: the tangent psi_k + h dpsi/dmu, with
  * J(psi_k) dpsi/dmu = -dF/dmu solved for all branches in one lockstep FP32
  * MINRES (LOCA's "Tangent", examples/conf.xml:38-41); 2: the secant through
- * the last two states -- or, once trained, from the MLP surrogate guess
+ * the last two states; 3: the tangent on the first step and after a step
+ * that a branch did not pass from its first guess (a transition, which the
+ * secant would extrapolate across), the secant otherwise -- or, once
+ * trained, from the MLP surrogate guess
  * (sopts != NULL). Branches that miss are retried in lockstep: from the
  * predictor guess and from psi_k (at most 20 Newton steps each), then by
  * pseudo-transient continuation ((J + sigma c t) steps) from both; then, one

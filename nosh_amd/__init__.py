@@ -805,16 +805,17 @@ class Batch(_Handle):
                 for i in range(self.ninst)]
 
     def continuation(self, psi0, span: float, steps: int, surrogate: Optional[dict] = None,
-                     options: Optional[NewtonOptions] = None, predictor: str = "tangent",
+                     options: Optional[NewtonOptions] = None, predictor: str = "auto",
                      **kw):
         """Steps every instance (a branch) from its mu_0 to mu_0 + span in
         ``steps`` equal steps, all branches solved together, each step from
         the ``predictor`` guess ("tangent": psi_k + h dpsi/dmu from one
         lockstep solve of all branches; "secant": through the last two
-        states). With ``surrogate`` (SurrogateOptions fields; ``{}`` for the
+        states; "auto": the tangent on the first step and after a step that
+        a branch did not pass from its first guess, else the secant). With ``surrogate`` (SurrogateOptions fields; ``{}`` for the
         defaults) the MLP seeds Newton once trained. Returns (psi, step
         records, surrogate stats or None)."""
-        pred = {"tangent": 1, "secant": 2}[predictor]
+        pred = {"tangent": 1, "secant": 2, "auto": 3}[predictor]
         lib = _load_mlp()
         if not getattr(lib, "_nosh_bcont", False):
             _load_surrogate(lib)

@@ -288,8 +288,9 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
   nosh_surrogate_stats sst{};
   int rc = bcguard([&] {
     NOSH_CHECK(bt && psi && record && steps >= 1, kInvalid, "invalid batch continuation call");
-    NOSH_CHECK(predictor == 1 || predictor == 2, kInvalid,
-               "batch continuation: predictor must be 1 (tangent) or 2 (secant)");
+    NOSH_CHECK(predictor >= 1 && predictor <= 3, kInvalid,
+               "batch continuation: predictor must be 1 (tangent), 2 (secant) or 3 (tangent "
+               "where the secant has no smooth history, else secant)");
     NOSH_CHECK(!so || (so->train_after >= 1 && so->batch >= 64 && so->newton_budget >= 1 &&
                        so->retrain_steps >= 0 && so->smooth_sweeps >= 0 &&
                        so->accept_ratio > 0.0),
@@ -491,13 +492,19 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
     const bool progress = std::getenv("NOSH_PROGRESS") != nullptr;
     std::vector<int> ok(ni), skip(ni), use_sur(ni), maxit(ni), tried_sec(ni);
     int skip_until = 0, losses = 0;
+    bool hard_prev = false;  // a branch of the previous step missed its first try
     std::vector<double> nf_sur, nf_sec;
     for (int k = 1; k <= steps; ++k) {
       const auto t0 = std::chrono::steady_clock::now();
       nosh_batch_continuation_step& r = record[k];
       r = nosh_batch_continuation_step{};
       r.param = k * h;
-      if (predictor == 1) {
+      bool hard = false;
+      // Predictor 3: the tangent on the first step (no history) and after a
+      // step that a branch did not pass from its first guess (a transition:
+      // the secant through it points across the jump), the secant otherwise.
+      const bool tangent = predictor == 1 || (predictor == 3 && (k == 1 || hard_prev));
+      if (tangent) {
         // Tangent guess psi_k + h dpsi/dmu, J(psi_k) dpsi/dmu = -dF/dmu at
         // mu_k, all branches in one lockstep solve.
         for (int i = 0; i < ni; ++i) mu[i] = bt->mu_h[i] + (k - 1) * h;
@@ -508,7 +515,7 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
       }
       for (int i = 0; i < ni; ++i) mu[i] = bt->mu_h[i] + k * h;
       set_mu();
-      if (predictor == 2) {
+      if (!tangent) {
         // Secant guess (psi_k on the first step).
         const double ratio = hprev > 0.0 ? h / hprev : 0.0;
         for (size_t q = 0; q < ng; ++q) {
@@ -605,6 +612,7 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
           maxit[i] = ptc ? o.max_iters : std::min(o.max_iters, kShortNewton);
         }
         if (todo == 0) continue;
+        if (attempt > 0) hard = true;
         const auto ta = std::chrono::steady_clock::now();
         std::vector<int> take(ni);
         for (int i = 0; i < ni; ++i) take[i] = !skip[i];
@@ -857,6 +865,7 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
       for (size_t q = 0; q < ng; ++q) std::swap(gc[q].prev, gc[q].cur);
       copy_state(&GroupCont::cur, true);
       hprev = h;
+      hard_prev = hard;
       r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       done = k + 1;
       if (progress)

@@ -38,7 +38,7 @@ def _oracle_check(c, t, mu, psis, tol=1e-9):
         assert np.linalg.norm(om.f(p)) <= tol, b
 
 
-@pytest.mark.parametrize("predictor", ["tangent", "secant"])
+@pytest.mark.parametrize("predictor", ["auto", "tangent", "secant"])
 def test_batch_continuation_matches_single_instance(predictor):
     """Four (mu_0, g) branches over span 0.2 in 10 steps: every branch ends on
     a solution of F = 0 (oracle) and on the state the single-instance FP64
@@ -128,7 +128,7 @@ def test_batch_continuation_halves_steps_for_failing_branches():
     branch ends on a solution of F = 0 (oracle)."""
     import nosh_amd as nosh
     import torch
-    # Device memory freed with NaN in it: the driver's buffers must not read
+    # Device memory freed with NaN in it: the continuation's buffers must not read
     # anything they did not write (the first secant once computed
     # psi_k + 0 * (psi_k - stale) = NaN in reused memory).
     junk = torch.full((64 << 20,), float("nan"), dtype=torch.float64, device="cuda")

@@ -148,3 +148,32 @@ def test_batch_keeps_the_potential_parameter():
     for k, x in enumerate(batch.split(psi)):
         om.set_params(g[k], mu[k])
         assert np.linalg.norm(om.f(x)) <= 1e-9, k
+
+
+def test_mesh_and_fields_from_an_exodus_file(tmp_path):
+    """A jittered cube and its fields written to an Exodus II file and read
+    back (nosh_amd.io, the reference's .e input path through MOAB): the
+    device mesh built from the file has the oracle's edge numbering and
+    control volumes, and F, J phi and dF/dp with the file's A, V (explicit
+    values) and thickness agree with the oracle to 1e-12."""
+    import nosh_amd as nosh
+    from nosh_amd import io as nio
+    from oracle import oracle as O
+    c0, t0 = mg.kuhn_cube(6, jitter=0.2, seed=8)
+    n = c0.shape[0]
+    V0 = -1.0 + 0.1 * np.sin(c0[:, 0])
+    path = str(tmp_path / "cube.e")
+    print("[exodus mesh: 6^3 jittered cube]", flush=True)
+    nio.write_exodus(path, c0, t0, {"A": mg.magnetic_potential(c0), "V": V0,
+                                     "thickness": np.full(n, 1.3)})
+    c, t, pd = nio.read_mesh(path)
+    gm, om_mesh = nosh.Mesh(c, t), O.Mesh(c, t)
+    assert np.array_equal(gm.my_edges(), om_mesh.edges())
+    assert rel_err(gm.control_volumes(), om_mesh.control_volumes()) < TOL
+    th = float(pd["thickness"][0])
+    model = nosh.Nls(gm, nosh.VectorFieldExplicitValues(gm, pd["A"], 0.0),
+                     nosh.ScalarFieldExplicitValues(gm, pd["V"]), 1.1,
+                     nosh.ScalarFieldConstant(gm, th))
+    om = _oracle(dict(om=om_mesh, A=pd["A"]), pd["V"], pd["V"], "beta",
+                 thickness=np.full(n, th))
+    _check_model(model, om, {"beta": 1.0, "g": 1.1, "mu": 0.3}, ["beta", "g", "mu"], n)
