@@ -464,7 +464,7 @@ typedef struct {
                            pairs and runs this many more Adam steps */
   int smooth_sweeps;    /* batched driver: damped Jacobi sweeps on the residual
                            of every surrogate guess before it is compared with
-                           the secant guess (default 0) */
+                           the predictor guess (default 0) */
   double accept_ratio;  /* a surrogate guess is used when its ||F|| is at most
                            this fraction of the predictor guess's (0.5) */
 } nosh_surrogate_options;

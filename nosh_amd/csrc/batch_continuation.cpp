@@ -3,26 +3,29 @@
 //
 // Every instance of a batch is a branch (mesh, g, mu_0). One step moves all
 // branches by the same increment h in mu and solves them with the lockstep
-// batched Newton (FP32 inner MINRES, FP64 outer). Predictor: the secant
-// through the last two accepted solutions,
-//   psi_sec = psi_k + (h / h_prev) (psi_k - psi_{k-1})   (psi_k on the first step).
+// batched Newton (FP32 inner MINRES, FP64 outer). Predictors: the tangent
+//   psi_k + h dpsi/dmu,   J(psi_k) dpsi/dmu = -dF/dmu,   one lockstep solve;
+// the secant through the last two accepted solutions,
+//   psi_sec = psi_k + (h / h_prev) (psi_k - psi_{k-1})   (psi_k on the first step);
+// or the tangent on the first step and after a step that a branch did not
+// pass from its first guess, the secant otherwise. The chosen predictor's
+// guess is written psi_pred below.
 // With a surrogate, after `train_after` steps a per-vertex MLP (BF16 MFMA,
 // trained on the run's own solutions) seeds Newton with
-//   psi_guess = psi_sec + h^2 sigma Y(x, psi_k, h s, mu, g),   h s = psi_sec - psi_k,
-// sigma the RMS of the training targets (psi_{k+1} - psi_sec) / h^2, then
+//   psi_guess = psi_pred + h^2 sigma Y(x, psi_k, h s, mu, g),   h s = psi_pred - psi_k,
+// sigma the RMS of the training targets (psi_{k+1} - psi_pred) / h^2, then
 // smoothed by smooth_sweeps damped Jacobi sweeps on its residual; a branch
 // starts from the surrogate guess when its ||F|| is at most accept_ratio
-// times the secant guess's (inexact Newton takes as many steps, and often
-// more MINRES iterations, from a guess that is only slightly better), else
-// from the secant guess. A
-// branch whose Newton from the surrogate guess does not converge within the
-// budget, or whose Gibbs energy leaves its linear trend, is solved again
-// from psi_sec; at a branch transition, where Newton from the secant guess
-// may wander, the branches a short try leaves are retried in lockstep by
-// pseudo-transient continuation (shifted Newton steps, J + sigma c t), from
-// the secant guess and then from psi_k; a branch none of these brings to
-// convergence is handed, alone, to the single-instance FP64 solver, and
-// last to batched sub-steps.
+// times psi_pred's (inexact Newton takes as many steps, and often more
+// MINRES iterations, from a guess that is only slightly better), else from
+// psi_pred. A branch whose Newton from the surrogate guess does not converge
+// within the budget, or whose Gibbs energy leaves its linear trend, is
+// solved again from psi_pred; at a branch transition, where Newton from
+// psi_pred may wander, the branches a short try leaves are retried in
+// lockstep by pseudo-transient continuation (shifted Newton steps,
+// J + sigma c t), from psi_pred and then from psi_k; a branch none of these
+// brings to convergence is handed, alone, to the single-instance FP64
+// solver, and last to batched sub-steps.
 //
 // This is the caller the reference delegates to LOCA (natural stepping,
 // executables/nosh-cont/nosh-cont.cpp:226-290,344 with examples/conf.xml:35-190);
@@ -106,7 +109,7 @@ __global__ void k_bc_secant(int64_t m, double r, const double2* __restrict__ cur
 }
 
 // Feature rows (row, b) -> X[(row * B + b) * nin ...]:
-// coordinates * cs (dim), Re/Im psi_k, Re/Im (psi_sec - psi_k), mu_b, g_b.
+// coordinates * cs (dim), Re/Im psi_k, Re/Im (psi_pred - psi_k), mu_b, g_b.
 __global__ void k_bc_features(int64_t n, int bp, int B, int dim, const int32_t* __restrict__ perm,
                               const double* __restrict__ coords, double cs,
                               const double2* __restrict__ cur, const double2* __restrict__ sec,
@@ -129,7 +132,7 @@ __global__ void k_bc_features(int64_t n, int bp, int B, int dim, const int32_t* 
   x[dim + 5] = float(g[inst0 + b]);
 }
 
-// Training target (psi_new - psi_sec) / h^2 per (row, b).
+// Training target (psi_new - psi_pred) / h^2 per (row, b).
 __global__ void k_bc_target(int64_t n, int bp, int B, const double2* __restrict__ pnew,
                             const double2* __restrict__ sec, double ih2, float* __restrict__ T) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -555,7 +558,6 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
         for (int i = 0; i < ni; ++i) {
           use_sur[i] = nf_sur[i] < so->accept_ratio * nf_sec[i];
           maxit[i] = use_sur[i] ? so->newton_budget : std::min(o.max_iters, kShortNewton);
-          tried_sec[i] = !use_sur[i];
           nsur += use_sur[i];
         }
         if (4 * nsur < ni) {
@@ -565,8 +567,13 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
           losses = 0;
         }
         if (nsur > 0) {
+          // The branches left on psi_pred take their short try in the same
+          // lockstep solve (and skip it below); when no branch takes the
+          // surrogate's guess, no solve runs here and all take it below.
+          for (int i = 0; i < ni; ++i) tried_sec[i] = !use_sur[i];
           take_into_psi(&GroupCont::guess, use_sur);
-          const int it = newton_batch(bt, o, st.data(), s, nullptr, maxit.data());
+          const int it = newton_batch(bt, o, st.data(), s, nullptr, maxit.data(), 0, nullptr,
+                                      kStallSteps);
           int mx = 0;
           for (int i = 0; i < ni; ++i) mx = std::max(mx, st[i].newton_iters);
           r.newton_iters += mx;
@@ -588,10 +595,10 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
           sst.fallbacks += r.fallbacks;
         }
       }
-      // Branches not yet accepted, in lockstep: Newton from the secant guess
+      // Branches not yet accepted, in lockstep: Newton from psi_pred
       // (unless a branch already started from it) and then from psi_k, with
       // at most kShortNewton steps each; then, for the branches both leave,
-      // pseudo-transient continuation from the secant guess and then from
+      // pseudo-transient continuation from psi_pred and then from
       // psi_k. Near a branch transition plain Newton wanders (the old branch
       // has ended and J is near singular along the mode that carries the
       // state to the new one), and often converges from one of the two
@@ -641,13 +648,13 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
                        "[batch continuation] step %d %s from %s: %d of %d converged, newton %d "
                        "minres %d, %.3fs\n",
                        k, ptc ? "pseudo-transient retry" : "short try",
-                       from_sec ? "the secant guess" : "psi_k", nc, todo, mx, it,
+                       from_sec ? "the predictor's guess" : "psi_k", nc, todo, mx, it,
                        std::chrono::duration<double>(std::chrono::steady_clock::now() - ta)
                            .count());
       }
       // Branches the short tries did not bring to convergence, one at a time
       // with the single-instance FP64 solver: Newton-MINRES (inner solves to
-      // minres_rtol, no forcing) from the secant guess, then from psi_k, then
+      // minres_rtol, no forcing) from psi_pred, then from psi_k, then
       // the tangent continuation over the step with up to kMaxHalvings step
       // halvings (the plain solver's way across a transition). A lockstep
       // launch costs about as much for one active lane as for all of them
@@ -740,7 +747,7 @@ int nosh_batch_continuation_run(nosh_batch_t bt, double* psi, double span, int s
                          "[batch continuation] step %d branch %d: FP64 fallback %s, newton %d "
                          "minres %d halvings %d, %.3fs\n",
                          k, i,
-                         how == 1 ? "converged from the secant guess"
+                         how == 1 ? "converged from the predictor's guess"
                                   : how == 2 ? "converged from psi_k"
                                              : how == 3 ? "converged by tangent steps"
                                                         : "failed",

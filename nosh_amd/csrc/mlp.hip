@@ -323,21 +323,26 @@ struct InferArgs {
 
 // The weights of a hidden layer are staged through LDS in chunks of TC
 // output tiles (32 TC rows of W) shared by the block's 4 waves, instead of
-// every wave reading the whole layer from L2; the inputs of layer 0 use the
-// same LDS region before the first chunk.
+// every wave reading the whole layer from L2, double-buffered: each chunk's
+// global loads are issued into registers before the MFMA loop over the
+// previous chunk and stored to the other LDS buffer after it, so the L2
+// latency of the staging hides behind the MFMAs (the next layer's first
+// chunk is fetched during the current layer's last). The inputs of layer 0
+// use the same LDS region before the first chunk.
 template <int NT>
 __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
   constexpr int W = NT * 32;
   constexpr int LD = W + 8;            // padded row (bf16) against LDS bank conflicts
-  constexpr int TC = NT < 4 ? NT : 4;  // output tiles per staged weight chunk
+  constexpr int TC = NT < 2 ? NT : 2;  // output tiles per staged weight chunk
   constexpr int NC = NT / TC;          // chunks per layer
   constexpr int WROWS = 32 * TC;
-  constexpr int WBYTES = WROWS * LD * 2 > kInferRows * kInferMaxIn * 4
-                             ? WROWS * LD * 2
-                             : kInferRows * kInferMaxIn * 4;
+  constexpr int BUFB = WROWS * LD * 2;  // bytes of one chunk buffer
+  constexpr int VPT = WROWS * (W / 8) / 256;  // 16-byte vectors per thread and chunk
+  static_assert(VPT >= 1 && WROWS * (W / 8) % 256 == 0, "chunk does not split over 256 threads");
+  constexpr int WBYTES = 2 * BUFB > kInferRows * kInferMaxIn * 4 ? 2 * BUFB
+                                                                 : kInferRows * kInferMaxIn * 4;
   __shared__ __bf16 act[kInferRows][LD];
   __shared__ __attribute__((aligned(16))) unsigned char wbuf[WBYTES];
-  __bf16 (*wl)[LD] = reinterpret_cast<__bf16 (*)[LD]>(wbuf);
   float (*xs)[kInferMaxIn] = reinterpret_cast<float (*)[kInferMaxIn]>(wbuf);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t r0 = int64_t(blockIdx.x) * kInferRows;
@@ -347,6 +352,27 @@ __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
     xs[r][c] = r < rows ? g.X[(r0 + r) * g.nin + c] : 0.0f;
   }
   __syncthreads();
+  // Chunk staging: registers <- global, then LDS buffer <- registers.
+  bf16x8 stage[VPT];
+  auto fetch = [&](int l, int ch) {
+    const __bf16* Wl = g.w + g.woff[l];
+#pragma unroll
+    for (int q = 0; q < VPT; ++q) {
+      const int v = tid + q * 256;
+      const int rr = v / (W / 8), cc = v - rr * (W / 8);
+      stage[q] = *reinterpret_cast<const bf16x8*>(Wl + int64_t(ch * WROWS + rr) * W + cc * 8);
+    }
+  };
+  auto commit = [&](int b) {
+    __bf16 (*wl)[LD] = reinterpret_cast<__bf16 (*)[LD]>(wbuf + b * BUFB);
+#pragma unroll
+    for (int q = 0; q < VPT; ++q) {
+      const int v = tid + q * 256;
+      const int rr = v / (W / 8), cc = v - rr * (W / 8);
+      *reinterpret_cast<bf16x8*>(&wl[rr][cc * 8]) = stage[q];
+    }
+  };
+  if (g.depth > 1) fetch(1, 0);
   // Layer 0: n_in -> W on the vector units, one output column per thread.
   for (int n = tid; n < W; n += 256) {
     float wr[kInferMaxIn];
@@ -358,24 +384,24 @@ __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
       act[r][n] = __bf16(gelu(z));
     }
   }
-  // Hidden layers: each wave owns rows 32 wave .. 32 wave + 31.
+  __syncthreads();  // xs (the first buffers' region) is no longer read
+  if (g.depth > 1) commit(0);
+  __syncthreads();
+  // Hidden layers: each wave owns rows 32 wave .. 32 wave + 31 (it reads and
+  // writes only those rows of act between the block barriers).
   const int row = wave * 32 + (lane & 31);
   const int kh = 8 * (lane >> 5);
+  int buf = 0;
   for (int l = 1; l < g.depth; ++l) {
-    const __bf16* Wl = g.w + g.woff[l];
     f32x16 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t)
       for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
 #pragma unroll
     for (int ch = 0; ch < NC; ++ch) {
-      __syncthreads();  // previous users of wl (xs, or the last chunk) are done
-      for (int v = tid; v < WROWS * (W / 8); v += 256) {
-        const int rr = v / (W / 8), cc = v - rr * (W / 8);
-        *reinterpret_cast<bf16x8*>(&wl[rr][cc * 8]) =
-            *reinterpret_cast<const bf16x8*>(Wl + int64_t(ch * WROWS + rr) * W + cc * 8);
-      }
-      __syncthreads();
+      const bool more = ch + 1 < NC || l + 1 < g.depth;
+      if (more) fetch(ch + 1 < NC ? l : l + 1, ch + 1 < NC ? ch + 1 : 0);
+      __bf16 (*wl)[LD] = reinterpret_cast<__bf16 (*)[LD]>(wbuf + buf * BUFB);
 #pragma unroll 2
       for (int kc = 0; kc < W / 16; ++kc) {
         const bf16x8 a = *reinterpret_cast<const bf16x8*>(&act[row][kc * 16 + kh]);
@@ -386,6 +412,9 @@ __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
               __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[ch * TC + t], 0, 0, 0);
         }
       }
+      if (more) commit(buf ^ 1);
+      __syncthreads();  // the other buffer is complete; this one is free again
+      buf ^= 1;
     }
     const float* bl = g.bias + g.boff[l];
 #pragma unroll
@@ -400,14 +429,20 @@ __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
     }
   }
   __syncthreads();
-  // Last layer: W -> n_out, FP32 dot products over k in order.
+  // Last layer: W -> n_out, FP32 dot products over k in order (16-byte loads
+  // of activations and weights).
   const int L = g.depth;
   for (int i = tid; i < kInferRows * g.nout; i += 256) {
     const int r = i / g.nout, o = i - r * g.nout;
     if (r >= rows) continue;
     const __bf16* wo = g.w + g.woff[L] + int64_t(o) * W;
     float s = g.bias[g.boff[L] + o];
-    for (int k = 0; k < W; ++k) s += float(act[r][k]) * float(wo[k]);
+    for (int k8 = 0; k8 < W / 8; ++k8) {
+      const bf16x8 av = *reinterpret_cast<const bf16x8*>(&act[r][k8 * 8]);
+      const bf16x8 wv = *reinterpret_cast<const bf16x8*>(wo + k8 * 8);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) s += float(av[e]) * float(wv[e]);
+    }
     g.Y[(r0 + r) * g.nout + o] = s;
   }
 }
