@@ -837,15 +837,19 @@ __device__ double rows_minres_a(const BatchArgs& a, const Ctx& c) {
 
 // ------------------------------------------------- L lanes per thread
 // Mapping of the gathering kernels in tuning mode 2 (L = kVecLanes): a thread
-// takes L consecutive lanes of one row (the bp / L threads of a row load the
-// same slot record; no lane shuffles) and gathers 8 L contiguous bytes of x
+// takes L consecutive lanes of one row and gathers 8 L contiguous bytes of x
 // per slot, so a wave keeps L times the gathered bytes in flight of the
-// one-lane mapping for the same address and metadata registers. Launched
-// with block_threads / L threads over the same row tiles as the one-lane
-// kernels: same partial sums, same arithmetic per (row, lane), same bits.
-// At 96^3 x 8 lanes the Chebyshev step takes 114 us with L = 2 (60 VGPRs,
-// 8 waves per SIMD) and 115 us with L = 4 (92 VGPRs, 5 waves), against
-// 124 us for the one-lane kernel.
+// one-lane mapping for the same address and metadata registers; the
+// threads of a row share each slot record (one dword per thread of a quad,
+// broadcast by DPP). Launched with block_threads / L threads over the same
+// row tiles as the one-lane kernels: same partial sums, same arithmetic per
+// (row, lane), same bits. At 96^3 x 8 lanes the Chebyshev step took 114 us
+// with L = 2 (60 VGPRs, 8 waves per SIMD), 115 us with L = 4 (92 VGPRs,
+// 5 waves) and 124 us for the one-lane kernel while every thread loaded the
+// whole 12-byte record; the texture addresser, busy 69 % of the time, spent
+// as many cycles on those records as on the gathers. With one dword per
+// thread: 93.5 us (108.5 us on the same box before; 3 or 4 slots per group
+// instead of 2: 99.4 and 104.2 us).
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 
@@ -928,12 +932,162 @@ __device__ __forceinline__ VCtx<L> make_vctx(const BatchArgs& a, int64_t v0) {
   return c;
 }
 
-// Row product of one row for L lanes: slots two at a time, the next two
-// slots' records loaded (clamped to the row's last slot) after the current
-// gathers, so that waiting for those gathers does not wait for them. A
-// clamped record's gather is loaded and not used.
+// One padded slot's contribution to L lanes: the slot's value at the shared
+// mu (SlotVal) or rebuilt per lane from (alpha, a~) (SlotMeta).
+template <int L, class Rec>
+__device__ __forceinline__ void slot_mac(const Rec& m, const float (&mu)[L], const float2 (&xv)[L],
+                                         float2 (&acc)[L]) {
+  if constexpr (std::is_same<Rec, SlotVal>::value) {
+#pragma unroll
+    for (int l = 0; l < L; ++l) cmac(acc[l].x, acc[l].y, m.vr, m.vi, xv[l]);
+  } else {
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      float vr, vi;
+      slot_value(m.alpha, m.at, mu[l], vr, vi);
+      cmac(acc[l].x, acc[l].y, vr, vi, xv[l]);
+    }
+  }
+}
+
+// Row product of one row for L lanes, NS slots at a time: the NS gathers of
+// a group are issued together and the next group's records (clamped to the
+// row's last slot) right after them, so a wave waits once per group. Every
+// gather of the main loop is used unconditionally (a gather used only under
+// a condition is sunk by the compiler behind the wait for the ones before
+// it, which serialises the round trips). The last w mod NS slots gather
+// from their clamped records and add x = 0 for the slots past the end: a
+// product with x = +0 adds a signed zero to an accumulator that starts at
+// +0 and so never holds -0, which leaves it unchanged, bit for bit. Slot
+// order as in the one-lane kernels: same bits.
+template <int L, int NS, class Rec>
+__device__ __forceinline__ void brow_vec_recs(const Rec* __restrict__ rec, int32_t j0, int32_t w,
+                                              int64_t bp, int b0, const float (&mu)[L],
+                                              const float2* __restrict__ x, float2 (&acc)[L]) {
+  Rec m[NS];
+  int64_t xo[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    m[q] = rec[j0 + min(q, w - 1) * kSliceH];
+    xo[q] = int64_t(m[q].col) * bp + b0;
+  }
+  int32_t k = 0;
+  for (; k + NS <= w; k += NS) {
+    float2 xv[NS][L];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) load_lanes<L>(x + xo[q], xv[q]);
+    Rec n[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) n[q] = rec[j0 + min(k + NS + q, w - 1) * kSliceH];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) slot_mac<L>(m[q], mu, xv[q], acc);
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      m[q] = n[q];
+      xo[q] = int64_t(n[q].col) * bp + b0;
+    }
+  }
+  if (k < w) {
+    float2 xv[NS - 1][L];
+#pragma unroll
+    for (int q = 0; q < NS - 1; ++q) load_lanes<L>(x + xo[q], xv[q]);
+#pragma unroll
+    for (int q = 0; q < NS - 1; ++q) {
+      const bool in = k + q < w;
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        xv[q][l].x = in ? xv[q][l].x : 0.0f;
+        xv[q][l].y = in ? xv[q][l].y : 0.0f;
+      }
+      slot_mac<L>(m[q], mu, xv[q], acc);
+    }
+  }
+}
+
+// Lane i's value of v in every lane of its quad (DPP quad_perm [i,i,i,i]).
+template <int I>
+__device__ __forceinline__ int32_t quad_bcast(int32_t v) {
+  return __builtin_amdgcn_mov_dpp(v, I * 0x55, 0xf, 0xf, false);
+}
+
+// brow_vec_recs for rows whose threads come in whole quads (threads per row
+// a multiple of 4): thread q of a quad loads dword q of each slot record
+// (q = 3 repeats dword 0) and the quad shares the three dwords by DPP, so
+// the record costs one 4-byte access per thread instead of a 12-byte one.
+// The loads and broadcasts run in every thread of the quad (DPP reads its
+// source lane, which must be active; the broadcasts are convergent, so they
+// are not moved under the condition); only threads with an active lane
+// (on) gather and accumulate. Same products in the same order as
+// brow_vec_recs: same bits.
+template <int L, int NS, class Rec>
+__device__ __forceinline__ void brow_vec_quad(const Rec* __restrict__ rec, int32_t j0, int32_t w,
+                                              int64_t bp, int b0, bool on, const float (&mu)[L],
+                                              const float2* __restrict__ x, float2 (&acc)[L]) {
+  static_assert(sizeof(Rec) == 12, "three dwords per record");
+  const int32_t* __restrict__ r32 = reinterpret_cast<const int32_t*>(rec);
+  const int q = int(threadIdx.x & 3);
+  const int qo = q == 3 ? 0 : q;
+  int32_t m[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) m[s] = r32[int64_t(j0 + min(s, w - 1) * kSliceH) * 3 + qo];
+  int32_t k = 0;
+  for (; k + NS <= w; k += NS) {
+    Rec rr[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      int32_t t[3] = {quad_bcast<0>(m[s]), quad_bcast<1>(m[s]), quad_bcast<2>(m[s])};
+      __builtin_memcpy(&rr[s], t, 12);
+    }
+    int32_t n[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      n[s] = r32[int64_t(j0 + min(k + NS + s, w - 1) * kSliceH) * 3 + qo];
+    if (on) {
+      float2 xv[NS][L];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) load_lanes<L>(x + int64_t(rr[s].col) * bp + b0, xv[s]);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) slot_mac<L>(rr[s], mu, xv[s], acc);
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) m[s] = n[s];
+  }
+  if (k < w) {
+    Rec rr[NS - 1];
+#pragma unroll
+    for (int s = 0; s < NS - 1; ++s) {
+      int32_t t[3] = {quad_bcast<0>(m[s]), quad_bcast<1>(m[s]), quad_bcast<2>(m[s])};
+      __builtin_memcpy(&rr[s], t, 12);
+    }
+    if (on) {
+      float2 xv[NS - 1][L];
+#pragma unroll
+      for (int s = 0; s < NS - 1; ++s) load_lanes<L>(x + int64_t(rr[s].col) * bp + b0, xv[s]);
+#pragma unroll
+      for (int s = 0; s < NS - 1; ++s) {
+        const bool in = k + s < w;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          xv[s][l].x = in ? xv[s][l].x : 0.0f;
+          xv[s][l].y = in ? xv[s][l].y : 0.0f;
+        }
+        slot_mac<L>(rr[s], mu, xv[s], acc);
+      }
+    }
+  }
+}
+
+#ifndef NOSH_VEC_SLOTS
+#define NOSH_VEC_SLOTS 2
+#endif
+#ifndef NOSH_VEC_QUAD
+#define NOSH_VEC_QUAD 1
+#endif
+
+// Row product for the thread's L lanes. Every thread of the row calls it
+// (on: the thread has an active lane; T: threads per row, wave-uniform).
 template <int L>
-__device__ __forceinline__ void brow_vec(const BGroup& G, int64_t row, int b0,
+__device__ __forceinline__ void brow_vec(const BGroup& G, int64_t row, int b0, int T, bool on,
                                          const float (&mu)[L], const float2* __restrict__ x,
                                          float2 (&acc)[L]) {
 #pragma unroll
@@ -943,49 +1097,16 @@ __device__ __forceinline__ void brow_vec(const BGroup& G, int64_t row, int b0,
   const int32_t w = (end - beg) / kSliceH;
   if (w <= 0) return;
   const int32_t j0 = beg + int32_t(row & 63);
-  const int64_t bp = G.bp;
-  if (G.kval) {
-    SlotVal m0 = G.kval[j0], m1 = G.kval[j0 + min(1, w - 1) * kSliceH];
-    for (int32_t k = 0; k < w; k += 2) {
-      float2 x0[L], x1[L];
-      load_lanes<L>(x + m0.col * bp + b0, x0);
-      load_lanes<L>(x + m1.col * bp + b0, x1);
-      const SlotVal n0 = G.kval[j0 + min(k + 2, w - 1) * kSliceH];
-      const SlotVal n1 = G.kval[j0 + min(k + 3, w - 1) * kSliceH];
-#pragma unroll
-      for (int l = 0; l < L; ++l) cmac(acc[l].x, acc[l].y, m0.vr, m0.vi, x0[l]);
-      if (k + 1 < w) {
-#pragma unroll
-        for (int l = 0; l < L; ++l) cmac(acc[l].x, acc[l].y, m1.vr, m1.vi, x1[l]);
-      }
-      m0 = n0;
-      m1 = n1;
-    }
-  } else {
-    SlotMeta m0 = G.s32[j0], m1 = G.s32[j0 + min(1, w - 1) * kSliceH];
-    for (int32_t k = 0; k < w; k += 2) {
-      float2 x0[L], x1[L];
-      load_lanes<L>(x + m0.col * bp + b0, x0);
-      load_lanes<L>(x + m1.col * bp + b0, x1);
-      const SlotMeta n0 = G.s32[j0 + min(k + 2, w - 1) * kSliceH];
-      const SlotMeta n1 = G.s32[j0 + min(k + 3, w - 1) * kSliceH];
-#pragma unroll
-      for (int l = 0; l < L; ++l) {
-        float vr, vi;
-        slot_value(m0.alpha, m0.at, mu[l], vr, vi);
-        cmac(acc[l].x, acc[l].y, vr, vi, x0[l]);
-      }
-      if (k + 1 < w) {
-#pragma unroll
-        for (int l = 0; l < L; ++l) {
-          float vr, vi;
-          slot_value(m1.alpha, m1.at, mu[l], vr, vi);
-          cmac(acc[l].x, acc[l].y, vr, vi, x1[l]);
-        }
-      }
-      m0 = n0;
-      m1 = n1;
-    }
+  if (NOSH_VEC_QUAD && (T & 3) == 0) {
+    if (G.kval)
+      brow_vec_quad<L, NOSH_VEC_SLOTS>(G.kval, j0, w, G.bp, b0, on, mu, x, acc);
+    else
+      brow_vec_quad<L, NOSH_VEC_SLOTS>(G.s32, j0, w, G.bp, b0, on, mu, x, acc);
+  } else if (on) {
+    if (G.kval)
+      brow_vec_recs<L, NOSH_VEC_SLOTS>(G.kval, j0, w, G.bp, b0, mu, x, acc);
+    else
+      brow_vec_recs<L, NOSH_VEC_SLOTS>(G.s32, j0, w, G.bp, b0, mu, x, acc);
   }
 }
 
@@ -1038,7 +1159,9 @@ __device__ void rows_cheb_step_v(const BatchArgs& a, const VCtx<L>& c, int mode,
     }
     all = all && act[l];
   }
-  if (!any) return;
+  // Threads without an active lane leave only with their whole wave: the
+  // row product needs every thread of a row (its quads share slot records).
+  if (!__any(any)) return;
   const bool last = (j == a.iters - 1);
   const float2* r = rbuf(G, it);
   const float2* zin = zbuf(G, a.z0 + j - 1);
@@ -1049,7 +1172,8 @@ __device__ void rows_cheb_step_v(const BatchArgs& a, const VCtx<L>& c, int mode,
     if (row >= G.n) break;
     const int64_t idx0 = row * G.bp + c.b0;
     float2 acc[L];
-    brow_vec<L>(G, row, c.b0, mu, zin, acc);
+    brow_vec<L>(G, row, c.b0, c.T, any, mu, zin, acc);
+    if (!any) continue;
     float2 zi[L], rv[L], zpp[L], z[L];
     float b0[L], b1[L], b2[L];
     load_lanes<L>(zin + idx0, zi);
@@ -1102,7 +1226,7 @@ __device__ void rows_minres_a_v(const BatchArgs& a, const VCtx<L>& c, double (&s
     }
     all = all && act[l];
   }
-  if (!any) return;
+  if (!__any(any)) return;  // see rows_cheb_step_v
   const float2* y = zbuf(G, 0);
   const float2* r1 = rbuf(G, it);
   float2* vout = vbuf(G, it);
@@ -1112,7 +1236,8 @@ __device__ void rows_minres_a_v(const BatchArgs& a, const VCtx<L>& c, double (&s
     if (row >= G.n) break;
     const int64_t idx0 = row * G.bp + c.b0;
     float2 acc[L];
-    brow_vec<L>(G, row, c.b0, mu, y, acc);
+    brow_vec<L>(G, row, c.b0, c.T, any, mu, y, acc);
+    if (!any) continue;
     float2 yi[L], rv[L], uu[L], vn[L];
     float b0[L], b1[L], b2[L];
     load_lanes<L>(y + idx0, yi);
