@@ -83,11 +83,16 @@ constexpr int kMaxHalvings = 3;   // step splits (2, 4, 8 sub-steps) for failed 
 // Pseudo-transient retries: initial shift sigma_0 (in units of the control
 // volume times thickness, the mass matrix of F, so independent of the mesh
 // size), forcing cap and Chebyshev degree (96^3 transition mu 0.70 -> 0.75:
-// sigma_0 = 0.1 in 10 steps, 1 in 22, 10 did not converge in 40; forcing caps
-// 1e-3 .. 1e-5 and degrees 5 / 6 measured within 10 % of each other without
-// the shift).
+// sigma_0 = 0.1 in 10 steps, 1 in 22, 10 did not converge in 40). Forcing
+// cap over five continuations (96^3, 8 branches; 20 steps with each
+// predictor, 50 steps auto and secant), total seconds: 1e-4 28.6, 1e-3
+// 28.0, 1e-2 27.7 (the fastest in each of the five), 1e-1 28.0 (one
+// transition in 80 instead of 68 Newton steps). With the cap at 1e-2:
+// degree 5 27.8; sigma_0 = 0.2 27.5, 0.05 24.3, 0.03 31.0, but at 160^3
+// (50 steps, auto and secant) sigma_0 = 0.1 took 122.7 s, 0.05 151.3 s and
+// 0.03 193.9 s (the transition at mu = 0.94 in 36, 75 and 121 steps).
 constexpr double kPtcShift0 = 0.1;
-constexpr double kPtcForcing = 1e-4;
+constexpr double kPtcForcing = 1e-2;
 constexpr int kPtcCheb = 6;
 // Relative MINRES tolerance of the tangent solve: the guess then errs by
 // about kTangentRtol h |dpsi/dmu| from the solve against ~h^2 |psi''| / 2 of
