@@ -110,6 +110,41 @@ def test_two_lanes_per_thread_match_one(setup, threads, fold):
         b1.set_tuning(256, 2, 4, 16)
 
 
+def test_two_lanes_per_thread_match_one_wide_groups(setup):
+    """Mode 2 against mode 0 on groups of 12 and 9 instances (16 lanes: 8
+    threads per row, two quads that each share the slot records by DPP, the
+    threads of lanes 12-15 and 9-15 with no instance). Instances finish their
+    MINRES solves at different iterations, so quads also run with threads
+    whose lanes are all done. Bitwise identical results and counts."""
+    import nosh_amd as nosh
+    meshes, _, models, _, _, _ = setup
+    sub = [(meshes[1], models[1], 12), (meshes[2], models[2], 9)]
+    counts = [k for _, _, k in sub]
+    u = mg.splitmix64_uniform(23, 2 * sum(counts)).reshape(-1, 2)
+    mu, g = 0.3 * u[:, 0], 0.6 + 0.8 * u[:, 1]
+    psi0 = []
+    for (c, t), _, cnt in sub:
+        for q in range(cnt):
+            p = 0.1 * mg.random_state(c.shape[0], 900 + len(psi0))
+            p[0::2] += 1.0
+            psi0.append(p)
+    psi0 = np.concatenate(psi0)
+    batch = nosh.Batch([(m, k) for _, m, k in sub], mu, g)
+    opts = dict(tol=1e-10, max_iters=60, minres_rtol=1e-6, minres_max_iters=4000, cheb_iters=4,
+                cheb_ratio=100.0, forcing=1, forcing_min=1e-6, forcing_max=0.1)
+    out = {}
+    for mode in (0, 2):
+        print(f"[batched wide groups mode={mode}]", flush=True)
+        batch.set_tuning(256, mode, 4, 16, fold=2)
+        psi, st = batch.solve(psi0, **opts)
+        assert all(s["converged"] for s in st), st
+        out[mode] = (psi, [(s["newton_iters"], s["minres_iters"]) for s in st])
+    assert out[0][1] == out[2][1]
+    assert len({m for _, m in out[2][1]}) > 1  # the instances did not all stop together
+    assert np.array_equal(out[0][0], out[2][0])
+    _check_with_oracle([m for m, _, _ in sub], counts, mu, g, batch.split(out[2][0]))
+
+
 def test_batched_autotune(setup, tmp_path, monkeypatch):
     import nosh_amd as nosh
     monkeypatch.setenv("NOSH_AUTOTUNE_CACHE", str(tmp_path / "tune.tsv"))
