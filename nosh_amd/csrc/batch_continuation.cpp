@@ -87,7 +87,9 @@ constexpr int kMaxHalvings = 3;   // step splits (2, 4, 8 sub-steps) for failed 
 // cap over five continuations (96^3, 8 branches; 20 steps with each
 // predictor, 50 steps auto and secant), total seconds: 1e-4 28.6, 1e-3
 // 28.0, 1e-2 27.7 (the fastest in each of the five), 1e-1 28.0 (one
-// transition in 80 instead of 68 Newton steps). With the cap at 1e-2:
+// transition in 80 instead of 68 Newton steps); over eight more (16-40
+// steps, three predictors) 1e-2 43.6 s and 1e-4 165.1 s (two 30-step runs
+// took 894 and 1346 lockstep Newton steps). With the cap at 1e-2:
 // degree 5 27.8; sigma_0 = 0.2 27.5, 0.05 24.3, 0.03 31.0, but at 160^3
 // (50 steps, auto and secant) sigma_0 = 0.1 took 122.7 s, 0.05 151.3 s and
 // 0.03 193.9 s (the transition at mu = 0.94 in 36, 75 and 121 steps).
