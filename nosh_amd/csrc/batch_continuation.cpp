@@ -76,6 +76,11 @@ constexpr int kShortNewton = 20;
 // steps via ||F|| = 3.7e3, falling 3.4x a step; mu 0.95 -> 1.0 from psi_k in
 // 16 steps via ||F|| = 4).
 constexpr int kStallSteps = 8;
+// The pseudo-transient retries run without that stop: a 12-step window
+// saved 2 s in two of thirteen 96^3 continuations (a retry stagnating near
+// 1e-5 for 40 steps at mu = 0.93) but cost 98 s at 160^3, and an 8-step
+// window 19-21 s in two 96^3 runs, where it cut off retries that would
+// have converged and sent the branches to the serial fallback.
 // Chebyshev degree of the single-instance FP64 fallback solves (the plain
 // solver's best).
 constexpr int kFallbackCheb = 6;
