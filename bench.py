@@ -625,10 +625,13 @@ def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev, A):
     copy_gbs = 2 * cbuf.numel() * 8 / t_copy / 1e9
     del cbuf, cdst
     layout = mesh.layout()
-    # Split layout: 20 B per upper slot (column, value), 8 B per lower slot
-    # (column, position of the mirrored value, which comes from L2), two
-    # slice pointers per slice, 56 B per vertex.
-    bytes_layout = (20 * layout["nnz_upper"] + 8 * layout["nnz_lower"] +
+    # Split layout: per upper slot a column and a 16 B value, per lower slot a
+    # column and the 4 B position of the mirrored value (which comes from
+    # L2), columns as 16-bit offsets from the row (or 32-bit indices when the
+    # bandwidth does not allow), two slice pointers per slice, 56 B per
+    # vertex.
+    cb = layout["column_bytes"]
+    bytes_layout = ((16 + cb) * layout["nnz_upper"] + (4 + cb) * layout["nnz_lower"] +
                     8 * (layout["nslices"] + 1) + 56 * n)
     # K.phi and F(psi) (FP64, internal order), canonical bytes 40E + 44n and
     # 40E + 52n.
@@ -701,12 +704,13 @@ def jphi_roofline(args, mesh, model, params, psi0, phi, stream, dev, A):
                              "by the 256 MB Infinity Cache"},
             "kuhn160": k160,
             "layout_bytes_per_launch": bytes_layout,
-            "layout_note": "split sliced ELLPACK (64-row slices): upper slots (column above "
-                           "the row) 4 B column + 16 B value, lower slots 4 B column + 4 B "
-                           "position of the mirrored upper slot (conjugate value, gathered "
-                           "from L2), 8 B per slice, 24 B 2x2 block + 16 B x + 16 B y per "
-                           "vertex",
+            "layout_note": f"split sliced ELLPACK (64-row slices): upper slots (column above "
+                           f"the row) {cb} B column + 16 B value, lower slots {cb} B column + 4 B "
+                           f"position of the mirrored upper slot (conjugate value, gathered "
+                           f"from L2), 8 B per slice, 24 B 2x2 block + 16 B x + 16 B y per "
+                           f"vertex; columns as {'16-bit offsets from the row' if cb == 2 else '32-bit indices'}",
             "nnz_upper": layout["nnz_upper"], "nnz_lower": layout["nnz_lower"],
+            "column_bytes": cb,
             "other_kernels": others}
 
 

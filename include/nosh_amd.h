@@ -91,10 +91,15 @@ int nosh_mesh_permutation(nosh_mesh_t mesh, int32_t* perm);
  * widest row. */
 int nosh_mesh_layout(nosh_mesh_t mesh, int64_t* nnz_pad, int64_t* nslices, int* max_width);
 /* The split layout the operator kernels read: padded slots of the upper part
- * (column above the row: 4 B column + the value) and of the lower part (4 B
- * column + 4 B position of the mirrored upper slot, whose conjugate value it
- * uses). */
+ * (column above the row: column + the value) and of the lower part (column +
+ * 4 B position of the mirrored upper slot, whose conjugate value it uses);
+ * the column is a 16- or 32-bit index (nosh_mesh_layout_column_bytes). */
 int nosh_mesh_layout_split(nosh_mesh_t mesh, int64_t* nnz_upper, int64_t* nnz_lower);
+/* Bytes per column index the split layout's kernels read: 2 when every
+ * column fits a 16-bit offset from its row (the reordered graph's bandwidth
+ * is below 32768), else 4. No counterpart in the reference (its Tpetra
+ * matrices hold 32- or 64-bit global indices); a measurement helper. */
+int nosh_mesh_layout_column_bytes(nosh_mesh_t mesh, int* bytes);
 /* Device vectors between reference order and the internal order (stream
  * may be NULL). Callers that keep state on the device in internal order use
  * the *_apply_internal entry points and skip the permutation per apply. */

@@ -110,6 +110,7 @@ def _load():
         "nosh_mesh_permutation": (i32, [vp, vp]),
         "nosh_mesh_layout": (i32, [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i32)]),
         "nosh_mesh_layout_split": (i32, [vp, C.POINTER(i64), C.POINTER(i64)]),
+        "nosh_mesh_layout_column_bytes": (i32, [vp, C.POINTER(C.c_int)]),
         "nosh_mesh_to_internal": (i32, [vp, vp, vp, vp]),
         "nosh_mesh_to_reference": (i32, [vp, vp, vp, vp]),
         "nosh_vfield_explicit_values_create": (i32, [vp, vp, d, pvp]),
@@ -269,13 +270,17 @@ class Mesh(_Handle):
 
     def layout(self) -> dict:
         """Internal sliced-ELLPACK layout sizes (nnz_pad: the full structure;
-        nnz_upper / nnz_lower: the split layout the operator kernels read)."""
+        nnz_upper / nnz_lower: the split layout the operator kernels read;
+        column_bytes: 2 when its columns are 16-bit offsets from the row,
+        else 4)."""
         nnz, ns, w = C.c_int64(), C.c_int64(), C.c_int()
         _check(_lib.nosh_mesh_layout(self._h, C.byref(nnz), C.byref(ns), C.byref(w)))
         nu, nl = C.c_int64(), C.c_int64()
         _check(_lib.nosh_mesh_layout_split(self._h, C.byref(nu), C.byref(nl)))
+        cb = C.c_int()
+        _check(_lib.nosh_mesh_layout_column_bytes(self._h, C.byref(cb)))
         return {"nnz_pad": nnz.value, "nslices": ns.value, "max_width": w.value,
-                "nnz_upper": nu.value, "nnz_lower": nl.value}
+                "nnz_upper": nu.value, "nnz_lower": nl.value, "column_bytes": cb.value}
 
     def to_internal(self, x_ref_ptr: int, x_int_ptr: int, stream: int = 0):
         _check(_lib.nosh_mesh_to_internal(self._h, C.c_void_p(x_ref_ptr), C.c_void_p(x_int_ptr),

@@ -312,6 +312,13 @@ int nosh_mesh_layout_split(nosh_mesh_t mesh, int64_t* nnz_upper, int64_t* nnz_lo
   });
 }
 
+int nosh_mesh_layout_column_bytes(nosh_mesh_t mesh, int* bytes) {
+  return guard([&] {
+    NOSH_CHECK(mesh && bytes, kInvalid, "null argument");
+    *bytes = mesh->m->uoff.p ? 2 : 4;
+  });
+}
+
 int nosh_mesh_to_internal(nosh_mesh_t mesh, const double* x_ref, double* x_int, void* stream) {
   return guard([&] {
     NOSH_CHECK(mesh && x_ref && x_int, kInvalid, "null argument");

@@ -51,18 +51,25 @@ struct MeshCache {
   int64_t nnz_u = 0, nnz_l = 0;
   DBuf<int32_t> uptr, ucol, uedge;  // nslices + 1, nnz_u, nnz_u
   DBuf<int32_t> lptr, lcol, lpos;   // nslices + 1, nnz_l, nnz_l
+  // The columns again as 16-bit offsets from the row (column - row), when
+  // every one fits (after RCM the offsets are bounded by the bandwidth:
+  // 9,507 for the 96^3 cube, 26,083 for 160^3); empty otherwise.
+  DBuf<int16_t> uoff, loff;  // nnz_u, nnz_l
 };
 
-// Device view of the split layout (kernel argument).
+// Device view of the split layout (kernel argument); uoff / loff are null
+// when the columns are read as 32-bit indices.
 struct SellView {
   const int32_t* uptr;
   const int32_t* ucol;
   const int32_t* lptr;
   const int32_t* lcol;
   const int32_t* lpos;
+  const int16_t* uoff;
+  const int16_t* loff;
 };
 inline SellView sell_view(const MeshCache& m) {
-  return SellView{m.uptr.p, m.ucol.p, m.lptr.p, m.lcol.p, m.lpos.p};
+  return SellView{m.uptr.p, m.ucol.p, m.lptr.p, m.lcol.p, m.lpos.p, m.uoff.p, m.loff.p};
 }
 
 std::unique_ptr<MeshCache> build_mesh_cache(const double* coords, int64_t n, const int32_t* cells,

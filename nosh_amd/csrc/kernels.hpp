@@ -40,20 +40,24 @@ __device__ __forceinline__ int32_t ld_uniform(const int32_t* p) {
 // each value at its position in the upper part (a gather that the
 // neighbouring rows' waves have just brought into L2) and uses its
 // conjugate, sum conj(val_pos_k) x[col_k]. Upper slots first, each part in
-// slot order.
-template <class T>
-__device__ __forceinline__ V2<T> sell_row(int64_t row, const SellView& S,
-                                          const V2<T>* __restrict__ val,
-                                          const V2<T>* __restrict__ x) {
+// slot order. The columns come as 16-bit offsets from the row when the
+// layout has them (2 bytes a slot instead of 4), else as 32-bit indices;
+// the same columns in the same order either way.
+template <class T, bool OFF16>
+__device__ __forceinline__ V2<T> sell_row_cols(int64_t row, const SellView& S,
+                                               const V2<T>* __restrict__ val,
+                                               const V2<T>* __restrict__ x) {
   const int32_t s = uniform(int32_t(row >> 6));
   const int lane = int(row & 63);
+  const int32_t r32 = int32_t(row);
   T ar = T(0), ai = T(0);
   {
     const int32_t b = uniform(S.uptr[s]);
     const int32_t e = uniform(S.uptr[s + 1]);
 #pragma unroll 4
     for (int32_t j = b + lane; j < e; j += kSliceH) {
-      const int32_t c = __builtin_nontemporal_load(S.ucol + j);
+      const int32_t c = OFF16 ? r32 + int32_t(__builtin_nontemporal_load(S.uoff + j))
+                              : __builtin_nontemporal_load(S.ucol + j);
       const V2<T> v = val[j];
       const V2<T> xv = x[c];
       ar += v.x * xv.x - v.y * xv.y;
@@ -65,7 +69,8 @@ __device__ __forceinline__ V2<T> sell_row(int64_t row, const SellView& S,
     const int32_t e = uniform(S.lptr[s + 1]);
 #pragma unroll 4
     for (int32_t j = b + lane; j < e; j += kSliceH) {
-      const int32_t c = __builtin_nontemporal_load(S.lcol + j);
+      const int32_t c = OFF16 ? r32 + int32_t(__builtin_nontemporal_load(S.loff + j))
+                              : __builtin_nontemporal_load(S.lcol + j);
       const int32_t p = __builtin_nontemporal_load(S.lpos + j);
       const V2<T> v = val[p];
       const V2<T> xv = x[c];
@@ -77,6 +82,13 @@ __device__ __forceinline__ V2<T> sell_row(int64_t row, const SellView& S,
   r.x = ar;
   r.y = ai;
   return r;
+}
+
+template <class T>
+__device__ __forceinline__ V2<T> sell_row(int64_t row, const SellView& S,
+                                          const V2<T>* __restrict__ val,
+                                          const V2<T>* __restrict__ x) {
+  return S.uoff ? sell_row_cols<T, true>(row, S, val, x) : sell_row_cols<T, false>(row, S, val, x);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

@@ -120,6 +120,27 @@ def test_batch_continuation_surrogate_seeds_and_gains():
     _oracle_check(c, t, MU0 + span, batch.split(psi))
 
 
+def test_surrogate_that_never_wins_leaves_the_run_unchanged():
+    """A surrogate whose guesses are never accepted (accept_ratio 1e-12) is
+    trained and probed, and every branch still takes its short try from the
+    predictor's guess: the same Newton and MINRES counts at every step and
+    the same final states, bit for bit, as the run without a surrogate."""
+    c, t, model, batch, psi0 = _setup()
+    span, steps, P = 0.24, 16, 6
+    runs = {}
+    for label, sur in (("plain", None),
+                       ("probed", dict(train_after=P, train_steps=100, batch=4096, lr=2e-3,
+                                       newton_budget=4, accept_ratio=1e-12))):
+        print(f"[surrogate never wins: {label}]", flush=True)
+        psi, rec, st = batch.continuation(psi0, span, steps, surrogate=sur, predictor="auto", **KW)
+        assert all(r["converged"] == len(G) for r in rec)
+        runs[label] = (psi, [(r["newton_iters"], r["minres_iters"]) for r in rec], rec, st)
+    assert runs["probed"][3]["trained"] == 1 and runs["probed"][3]["steps_seeded"] == 0
+    assert any(r["pred_error"] != r["sec_error"] for r in runs["probed"][2])  # it was probed
+    assert runs["probed"][1] == runs["plain"][1]
+    assert np.array_equal(runs["probed"][0], runs["plain"][0])
+
+
 def test_batch_continuation_halves_steps_for_failing_branches():
     """A large step (0 -> 0.1 in mu) with a 3-step Newton limit: the step
     fails from every guess in the lockstep tries and in the single-instance

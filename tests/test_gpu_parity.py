@@ -109,6 +109,48 @@ def test_keo_f_jac_dfdp(case, mu):
             assert rel_err(got, ref) < TOL, name
 
 
+def test_operators_with_32bit_columns():
+    """The operator kernels read the split layout's columns as 16-bit offsets
+    from the row when the reordered graph's bandwidth allows (every mesh
+    above), else as 32-bit indices: a 33^3 cube with shuffled vertices and no
+    reordering (bandwidth ~39 k) takes the 32-bit path, and K phi, F, J phi,
+    P phi and the Chebyshev apply still match the oracle."""
+    import nosh_amd as nosh
+    from oracle import oracle as O
+    c0, t0 = mg.kuhn_cube(33)
+    n = c0.shape[0]
+    perm = np.random.default_rng(5).permutation(n)
+    inv = np.empty(n, dtype=np.int64)
+    inv[perm] = np.arange(n)
+    coords, cells = c0[perm], inv[t0].astype(t0.dtype)
+    gm = nosh.Mesh(coords, cells, reorder=False)
+    assert gm.layout()["column_bytes"] == 4
+    assert nosh.Mesh(*mg.kuhn_cube(6)).layout()["column_bytes"] == 2
+    A = mg.magnetic_potential(coords)
+    mu = 0.3
+    mvp = nosh.VectorFieldExplicitValues(gm, A, mu)
+    t = nosh.ScalarFieldConstant(gm, 1.0)
+    model = nosh.Nls(gm, mvp, nosh.ScalarFieldConstant(gm, -1.0), 1.0, t, None, "mu")
+    om = O.Model(O.Mesh(coords, cells), A)
+    om.set_params(1.0, mu)
+    psi, phi = mg.random_state(n, 0), mg.random_state(n, 1)
+    params = {"g": 1.0, "mu": mu}
+    keo = nosh.Keo(gm, t, mvp)
+    keo.set_parameters({"mu": mu})
+    assert rel_err(keo.apply(phi), om.keo(phi)) < TOL
+    assert rel_err(model.eval_f(psi, params), om.f(psi)) < TOL
+    J = model.create_W_op()
+    J.rebuild(params, psi)
+    om.jac_rebuild(psi)
+    assert rel_err(J.apply(phi), om.jac(phi)) < TOL
+    P = model.create_W_prec()
+    P.rebuild(params, psi)
+    om.prec_rebuild(psi)
+    assert rel_err(P.apply_operator(phi), om.prec_op(phi)) < TOL
+    P.set_chebyshev(5, 30.0, 0.0)
+    assert rel_err(P.apply(phi), om.cheb(phi, 5, 30.0)) < TOL
+
+
 @pytest.mark.parametrize("mu", [0.1])
 def test_keo_regularized(case, mu):
     mvp, V, t, model, om = _objects(case, mu)
