@@ -149,6 +149,16 @@ def test_operators_with_32bit_columns():
     assert rel_err(P.apply_operator(phi), om.prec_op(phi)) < TOL
     P.set_chebyshev(5, 30.0, 0.0)
     assert rel_err(P.apply(phi), om.cheb(phi, 5, 30.0)) < TOL
+    # The FP32 kernels on the same layout: FP32-inner Newton from psi = 1
+    # reaches a state whose oracle residual is at the FP64 tolerance.
+    psi0 = np.zeros(2 * n)
+    psi0[0::2] = 1.0
+    psi_m, st = nosh.newton_solve(model, psi0, params, tol=1e-10, max_iters=40,
+                                  minres_rtol=1e-10, minres_max_iters=3000, cheb_iters=4,
+                                  cheb_ratio=30.0, precision=1, forcing=1, forcing_min=1e-6,
+                                  forcing_max=0.1)
+    assert st["converged"], st
+    assert np.linalg.norm(om.f(psi_m)) <= 1e-9
 
 
 @pytest.mark.parametrize("mu", [0.1])
