@@ -1077,12 +1077,8 @@ __device__ __forceinline__ void brow_vec_quad(const Rec* __restrict__ rec, int32
   }
 }
 
-#ifndef NOSH_VEC_SLOTS
-#define NOSH_VEC_SLOTS 2
-#endif
-#ifndef NOSH_VEC_QUAD
-#define NOSH_VEC_QUAD 1
-#endif
+// Slots per group in the two-lane row product (3 and 4 measured slower).
+constexpr int kVecSlots = 2;
 
 // Row product for the thread's L lanes. Every thread of the row calls it
 // (on: the thread has an active lane; T: threads per row, wave-uniform).
@@ -1097,16 +1093,16 @@ __device__ __forceinline__ void brow_vec(const BGroup& G, int64_t row, int b0, i
   const int32_t w = (end - beg) / kSliceH;
   if (w <= 0) return;
   const int32_t j0 = beg + int32_t(row & 63);
-  if (NOSH_VEC_QUAD && (T & 3) == 0) {
+  if ((T & 3) == 0) {
     if (G.kval)
-      brow_vec_quad<L, NOSH_VEC_SLOTS>(G.kval, j0, w, G.bp, b0, on, mu, x, acc);
+      brow_vec_quad<L, kVecSlots>(G.kval, j0, w, G.bp, b0, on, mu, x, acc);
     else
-      brow_vec_quad<L, NOSH_VEC_SLOTS>(G.s32, j0, w, G.bp, b0, on, mu, x, acc);
+      brow_vec_quad<L, kVecSlots>(G.s32, j0, w, G.bp, b0, on, mu, x, acc);
   } else if (on) {
     if (G.kval)
-      brow_vec_recs<L, NOSH_VEC_SLOTS>(G.kval, j0, w, G.bp, b0, mu, x, acc);
+      brow_vec_recs<L, kVecSlots>(G.kval, j0, w, G.bp, b0, mu, x, acc);
     else
-      brow_vec_recs<L, NOSH_VEC_SLOTS>(G.s32, j0, w, G.bp, b0, mu, x, acc);
+      brow_vec_recs<L, kVecSlots>(G.s32, j0, w, G.bp, b0, mu, x, acc);
   }
 }
 
